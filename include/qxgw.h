@@ -1,0 +1,201 @@
+/*
+ * qxgw.h -- C ABI of the MI355X (gfx950) G0W0/evGW correlation path.
+ *
+ * Drop-in boundary for QuasiX's correlation hot path: the polarizability Pi(i w), the
+ * screened interaction Wc = (I - Pi)^-1 - I, the contour-deformation self-energy
+ * (imaginary-axis quadrature + residues) and the Newton quasiparticle solve.  Every entry
+ * point names the reference interface it replaces (paths relative to the reference root).
+ *
+ * Conventions
+ *   - Plain pointers and sizes; no framework types.  Functions ending in _dev take DEVICE
+ *     pointers (HIP device memory of the handle's device) for every array argument; the
+ *     descriptor and stats structs are host memory.  Functions without the suffix take
+ *     host pointers and copy internally.
+ *   - Row-major ("C order") arrays, FP64 unless stated.  B_ov is the reference's iaP block,
+ *     [nocc*nvir][naux] (quasix/src/evgw.rs:105-114).
+ *   - Status codes map one-to-one onto QuasixError (quasix_core/src/common/mod.rs:18-67);
+ *     qx_last_error() returns the message of the last failure on the handle.
+ *   - One handle per host thread; calls on one handle are serialised on its stream.
+ */
+#ifndef QXGW_H
+#define QXGW_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define QXGW_API __attribute__((visibility("default")))
+#define QXGW_VERSION_MAJOR 0
+#define QXGW_VERSION_MINOR 1
+
+typedef struct qx_context* qx_handle;
+
+typedef enum {
+    QX_OK = 0,
+    QX_ERR_INVALID_INPUT = 1,       /* QuasixError::InvalidInput */
+    QX_ERR_DIMENSION_MISMATCH = 2,  /* QuasixError::DimensionMismatch */
+    QX_ERR_NUMERICAL = 3,           /* QuasixError::NumericalError */
+    QX_ERR_CONVERGENCE = 4,         /* QuasixError::ConvergenceError */
+    QX_ERR_NOT_IMPLEMENTED = 5,     /* QuasixError::NotImplemented */
+    QX_ERR_DEVICE = 6,              /* HIP runtime failure (no QuasixError counterpart) */
+    QX_ERR_ALLOC = 7                /* device allocation failure */
+} qx_status;
+
+typedef enum {
+    QX_PREC_FP64 = 0,    /* FP64 MFMA everywhere (the GPU oracle path) */
+    QX_PREC_FP32 = 1,    /* Pi on FP32 MFMA, FP32 accumulation */
+    QX_PREC_BF16X3 = 2   /* Pi on BF16 MFMA with hi+lo split operands, FP32 accumulation */
+} qx_precision;
+
+/* NewtonSolverConfig, quasix_core/src/qp/newton_solver.rs:56-103 (same fields/defaults). */
+typedef struct {
+    int max_iterations;             /* 50 */
+    double energy_tolerance;        /* 1e-8 Ha */
+    double initial_damping;         /* 1.0 */
+    double min_damping;             /* 0.1 */
+    double max_damping;             /* 1.0 */
+    double max_step;                /* 1.0 Ha */
+    double derivative_delta;        /* 1e-6 Ha */
+    int use_richardson;             /* 1 */
+    double z_min;                   /* 0.1 */
+    double z_max;                   /* 0.999 */
+    int use_bisection_fallback;     /* 1 */
+    int max_bisection_iterations;   /* 30 */
+} qx_newton_config;
+
+/* Shape and options of one G0W0 solve. */
+typedef struct {
+    int nocc, nvir, naux;      /* orbital / auxiliary dimensions */
+    int nfreq;                 /* Gauss-Legendre imaginary frequencies */
+    double x0;                 /* grid scale, w = x0 (1+t)/(1-t)  (correlation_ac.rs:94-114) */
+    double eta;                /* broadening of the CD Green's function and residues */
+    int precision;             /* qx_precision of the Pi build */
+    int ntarget;               /* number of target orbitals */
+    int npair;                 /* number of (target, partner) rows of B_pair */
+    qx_newton_config newton;
+} qx_g0w0_desc;
+
+typedef struct {
+    int chol_fallbacks;        /* frequencies whose Cholesky failed (Gauss-Jordan used) */
+    int singular_fallbacks;    /* frequencies replaced by Wc = 0 (identity fallback) */
+    int residue_targets;       /* targets solved on the residue (host-driven) path */
+    int residue_jobs;          /* real-axis Wc evaluations performed */
+    int newton_failures;       /* targets that fell back to the linearised solution */
+    double seconds;            /* wall time of the call */
+} qx_stats;
+
+/* Per-target state reported in `state` arrays. */
+#define QX_QP_NEWTON 0
+#define QX_QP_BISECTION 1
+#define QX_QP_LINEARIZED 2
+
+/* ---- handle ------------------------------------------------------------------------ */
+QXGW_API qx_status qx_create(qx_handle* out, int device);
+QXGW_API void qx_destroy(qx_handle h);
+QXGW_API const char* qx_last_error(qx_handle h);
+/* Use a caller-owned hipStream_t (NULL = the handle's own stream). */
+QXGW_API qx_status qx_set_stream(qx_handle h, void* hip_stream);
+QXGW_API qx_status qx_synchronize(qx_handle h);
+/* Release the cached device workspace. */
+QXGW_API qx_status qx_release_workspace(qx_handle h);
+QXGW_API void qx_newton_config_default(qx_newton_config* c);
+QXGW_API void qx_g0w0_desc_default(qx_g0w0_desc* d);
+
+/* ---- profiling (no reference counterpart: per-phase device time for roofline reports) -- */
+/* Phases timed with HIP events on the handle's stream while profiling is enabled. */
+#define QX_PHASE_PREP 0      /* frequency weights, mixed-precision operand split */
+#define QX_PHASE_PI 1        /* Pi build (one launch per solve, all frequencies) */
+#define QX_PHASE_POTRF 2     /* blocked Cholesky of all eps_f */
+#define QX_PHASE_TRTRI 3     /* L^-1 */
+#define QX_PHASE_WNM 4       /* Wc_nm contraction */
+#define QX_PHASE_NEWTON 5    /* device Newton kernel */
+#define QX_PHASE_RESIDUE 6   /* residue (real-axis Wc) jobs */
+#define QX_NPHASE 8
+/* Enable (1) or disable (0) profiling; both reset the accumulated times. */
+QXGW_API qx_status qx_set_profiling(qx_handle h, int enable);
+/* ms[QX_NPHASE]: accumulated milliseconds; counts[QX_NPHASE]: number of timed intervals. */
+QXGW_API qx_status qx_get_profile(qx_handle h, double* ms, long long* counts);
+
+/* ---- A1: frequency grid (host only) -------------------------------------------------- */
+/* Gauss-Legendre nodes/weights on [-1,1], ascending.
+ * Replaces quasix_core::freq::imaginary_grid::gauss_legendre_grid (imaginary_grid.rs:42;
+ * PyO3 quasix_core.gauss_legendre_grid, quasix_core/src/python/mod.rs:29). */
+QXGW_API qx_status qx_gauss_legendre_grid(int n, double* nodes, double* weights);
+/* w = x0 (1+t)/(1-t), W = w_GL 2 x0 / (1-t)^2.
+ * Replaces correlation_ac::get_scaled_legendre_roots (correlation_ac.rs:94-114). */
+QXGW_API qx_status qx_scaled_legendre_roots(int nw, double x0, double* freqs, double* wts);
+
+/* ---- A2: polarizability ------------------------------------------------------------- */
+/* pi_out[f][P][Q] = 4 sum_ia B[ia][P] B[ia][Q] e_ia / (freqs[f]^2 + e_ia^2), full symmetric.
+ * Replaces correlation_ac::get_rho_response (correlation_ac.rs:184-211) for all
+ * frequencies at once and dielectric::compute_polarizability_p0 (polarizability.rs:169). */
+QXGW_API qx_status qx_rho_response_dev(qx_handle h, int precision, int nocc, int nvir,
+                                       int naux, const double* mo_energy, const double* B_ov,
+                                       int nfreq, const double* freqs, double* pi_out);
+
+/* ---- A3: screened interaction -------------------------------------------------------- */
+/* wc_out[f] = (I - pi[f])^-1 - I with the Cholesky -> LU -> identity chain of
+ * correlation_ac.rs:376-401 (and dielectric::compute_W_single_frequency, screening.rs:334).
+ * fallbacks (nullable, host): number of frequencies that needed the LU fallback. */
+QXGW_API qx_status qx_screened_wc_dev(qx_handle h, int naux, int nfreq, const double* pi,
+                                      double* wc_out, int* fallbacks);
+
+/* ---- A4: Wc_nm on the imaginary axis --------------------------------------------------- */
+/* wnm[f][j] = B_pair[j]^T Wc(i freqs[f]) B_pair[j] for every pair row; Pi from B_ov and
+ * mo_energy.  Replaces the per-target contraction of correlation_ac::get_sigma_diag
+ * (correlation_ac.rs:404-421) and selfenergy::frequency_cache (frequency_cache.rs:220). */
+QXGW_API qx_status qx_wmn_imag_dev(qx_handle h, int precision, int nocc, int nvir, int naux,
+                                   const double* mo_energy, const double* B_ov, int nfreq,
+                                   const double* freqs, int npair, const double* B_pair,
+                                   double* wnm);
+
+/* ---- A5/A6: CD self-energy at given energies ---------------------------------------------- */
+/* sigma[t][q] (complex, re/im interleaved) = I_n(E) + R_n(E) at E = energies[t][q] for target
+ * n = targets[t], given wnm from qx_wmn_imag_dev.  Pairs must be sorted by target.
+ * Replaces selfenergy::correlation_pyscf_cd::compute_sigma_c_pyscf_cd
+ * (correlation_pyscf_cd.rs:60-298, there at E = 0 without residues) and the unwired
+ * freq::contour_deformation::CDIntegrator (contour_deformation.rs:142-422). */
+QXGW_API qx_status qx_sigma_cd_dev(qx_handle h, const qx_g0w0_desc* d, const double* mo_energy,
+                                   const double* mo_energy_ref, const double* B_ov,
+                                   const double* B_pair, const int* pair_target,
+                                   const int* pair_m, const int* targets, const double* freqs,
+                                   const double* wts, const double* wnm, int nq,
+                                   const double* energies, double* sigma);
+
+/* ---- A9/A10: fused G0W0 solve ---------------------------------------------------------- */
+/* One G0W0 solve: Pi at all frequencies -> FP64 Cholesky -> Wc_nm -> Newton on the CD
+ * self-energy for every target.  Pairs (pair_target, pair_m, B_pair[npair][naux]) list the
+ * non-zero rows b_nm of each target, sorted by target.  static_terms[t] = Sigma_x - V_xc.
+ * mo_energy_ref (nullable): mean-field energies anchoring the QP equation and the eta sign
+ * (defaults to mo_energy).  guess (nullable): Newton start; NULL = linearised guess.
+ * Outputs per target: qp_energy, z_factor, sigma_re/sigma_im = Sigma_c(E_QP), state.
+ * Replaces the iteration-0 path of qp::evgw::EvGWDriver::run_evgw_blocks
+ * (quasix_core/src/qp/evgw.rs:603-783) with NewtonQPSolver::solve (newton_solver.rs:174). */
+QXGW_API qx_status qx_g0w0_cd_dev(qx_handle h, const qx_g0w0_desc* d, const double* mo_energy,
+                                  const double* mo_energy_ref, const double* B_ov,
+                                  const double* B_pair, const int* pair_target,
+                                  const int* pair_m, const int* targets,
+                                  const double* static_terms, const double* guess,
+                                  double* qp_energy, double* z_factor, double* sigma_re,
+                                  double* sigma_im, int* state, qx_stats* stats);
+/* qx_g0w0_cd_dev for an ov-only tensor (oo and vv blocks zero): the pairs are every
+ * virtual partner of an occupied target and every occupied partner of a virtual target,
+ * gathered on the device from B_ov; desc->npair is ignored.  Same reference counterpart. */
+QXGW_API qx_status qx_g0w0_cd_ov_dev(qx_handle h, const qx_g0w0_desc* d,
+                                     const double* mo_energy, const double* mo_energy_ref,
+                                     const double* B_ov, const int* targets,
+                                     const double* static_terms, const double* guess,
+                                     double* qp_energy, double* z_factor, double* sigma_re,
+                                     double* sigma_im, int* state, qx_stats* stats);
+/* Host-pointer variant of qx_g0w0_cd_dev. */
+QXGW_API qx_status qx_g0w0_cd(qx_handle h, const qx_g0w0_desc* d, const double* mo_energy,
+                              const double* mo_energy_ref, const double* B_ov,
+                              const double* B_pair, const int* pair_target, const int* pair_m,
+                              const int* targets, const double* static_terms,
+                              const double* guess, double* qp_energy, double* z_factor,
+                              double* sigma_re, double* sigma_im, int* state, qx_stats* stats);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* QXGW_H */

@@ -1,0 +1,692 @@
+"""CPU FP64 restatement of QuasiX's G0W0/evGW correlation path -- TEST INFRASTRUCTURE ONLY.
+
+This module is the oracle: the checker that the MI355X HIP path is compared against.
+Only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of ``bench.py``
+may import it, and only as the checker or as the timed CPU baseline -- never as the
+product path.  The product path (``quasix_amd``) fails loudly if its HIP library is
+missing; it never routes through this file.
+
+Every function cites the reference file:line it restates (paths relative to the
+reference root).  numpy/scipy supply DGEMM / DPOTRF / DPOTRI / DGETRF through their
+bundled OpenBLAS, mirroring the reference's ndarray-linalg + OpenBLAS backend.
+
+Conventions (fixed once, identical in the oracle and on the GPU):
+  * e_ia = eps_i - eps_a (< 0);  Pi(iw) = 4 * sum_ia B_ia B_ia^T * e_ia / (w^2 + e_ia^2)
+    (quasix_core/src/selfenergy/correlation_ac.rs:184-211).  Pi is negative semidefinite,
+    eps = I - Pi is SPD,  Wc = eps^-1 - I  (correlation_ac.rs:376-401).
+  * B_nm for the self-energy is the symmetrised full tensor assembled from the
+    iaP / miP / abP blocks (quasix/src/evgw.rs:161-205).
+  * Contour deformation (CD): Sigma_c(E) = I(E) + R(E), with the imaginary-axis
+    Green's function shifted by -i*eta*sign(eps_n - eps_m)
+    (quasix_core/src/selfenergy/correlation_pyscf_cd.rs:209-226) and residues of
+    Wc at z = |E - eps_m| + i*eta for the poles between the Fermi level and E
+    (Golze, Dvorak & Rinke, Front. Chem. 7, 377 (2019); PySCF gw_cd, which
+    correlation_pyscf_cd.rs:46 cites).  The reference has no live residue code
+    (freq/contour_deformation.rs:158-221 is unwired), so this part is pinned by the
+    brute-force Casida result in ``oracle/casida.py`` rather than by reference values.
+
+Parity status per function (checks live in tests/test_oracle_*.py):
+  gauss_legendre_nodes_weights, get_scaled_legendre_roots  pinned (closed form n=2,
+      numpy leggauss, reference tests correlation_ac.rs:722-752, imaginary_grid.rs:276+)
+  get_rho_response / compute_polarizability_p0              pinned (brute-force outer
+      products, reference known answer polarizability.rs:432+, symmetry :813)
+  screened_wc                                               pinned (Woodbury / eigen
+      identities, zero-Pi test correlation_fixed.rs)
+  wmn_imag, sigma_cd, residues                              pinned by Casida brute force
+  get_sigma_diag, thiele, pade_thiele, ac_pade_thiele_diag  pinned (rational-function
+      exactness, reference tests correlation_ac.rs:755-810)
+  NewtonQPSolver                                            pinned (reference tests
+      newton_solver.rs:609-732 restated as known answers)
+  evgw_cd                                                   parity unpinned beyond its
+      parts (no reference evGW numbers exist without PySCF inputs)
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+import scipy.linalg as sla
+
+__all__ = [
+    "gauss_legendre_nodes_weights", "get_scaled_legendre_roots",
+    "transform_to_imaginary_axis_pyscf", "transform_weights_pyscf",
+    "make_inputs", "assemble_mnP", "pair_list",
+    "get_rho_response", "compute_polarizability_p0", "rho_response_complex",
+    "screened_wc", "wmn_imag", "sigma_cd_eval", "NewtonQPSolver", "g0w0_cd",
+    "evgw_cd", "get_sigma_diag", "thiele", "pade_thiele", "ac_pade_thiele_diag",
+    "linearized_qp_ac", "CONFIGS",
+]
+
+# --------------------------------------------------------------------------------------
+# A1. Frequency grid
+# --------------------------------------------------------------------------------------
+
+def gauss_legendre_nodes_weights(n: int):
+    """Golub-Welsch GL nodes/weights on [-1, 1].
+
+    Restates quasix_core/src/selfenergy/correlation_ac.rs:122-160: tridiagonal Jacobi
+    matrix with beta_i = i / sqrt((2i-1)(2i+1)), nodes = eigenvalues (DSYEV, ascending),
+    weights = 2 * v_0^2.
+    """
+    if n == 0:
+        return np.zeros(0), np.zeros(0)
+    if n == 1:
+        return np.array([0.0]), np.array([2.0])
+    i = np.arange(1, n, dtype=np.float64)
+    beta = i / np.sqrt((2.0 * i - 1.0) * (2.0 * i + 1.0))
+    J = np.diag(beta, 1) + np.diag(beta, -1)
+    w, v = np.linalg.eigh(J)
+    return w, 2.0 * v[0, :] ** 2
+
+
+def get_scaled_legendre_roots(nw: int, x0: float = 0.5):
+    """Map GL nodes to [0, inf): w = x0 (1+t)/(1-t), W = w_GL * 2 x0 / (1-t)^2.
+
+    Restates correlation_ac.rs:94-114.
+    """
+    t, w = gauss_legendre_nodes_weights(nw)
+    omt = 1.0 - t
+    return x0 * (1.0 + t) / omt, w * 2.0 * x0 / (omt * omt)
+
+
+def transform_to_imaginary_axis_pyscf(nodes, x0=0.5):
+    """Restates quasix_core/src/freq/imaginary_grid.rs:130-145 (1e12 guard at t -> 1)."""
+    nodes = np.asarray(nodes, dtype=np.float64)
+    out = np.empty_like(nodes)
+    for k, x in enumerate(nodes):
+        out[k] = 1e12 if abs(1.0 - x) < 1e-14 else x0 * (1.0 + x) / (1.0 - x)
+    return out
+
+
+def transform_weights_pyscf(weights, nodes, x0=0.5):
+    """Restates quasix_core/src/freq/imaginary_grid.rs:196-213 (0 weight at t -> 1)."""
+    out = np.empty(len(weights))
+    for k, (w, x) in enumerate(zip(weights, nodes)):
+        out[k] = 0.0 if abs(1.0 - x) < 1e-14 else w * 2.0 * x0 / ((1.0 - x) * (1.0 - x))
+    return out
+
+
+# --------------------------------------------------------------------------------------
+# Synthetic inputs (the five workloads the path is measured on)
+# --------------------------------------------------------------------------------------
+
+CONFIGS = {
+    0: dict(nocc=5, nvir=19, naux=84, nfreq=32),
+    1: dict(nocc=40, nvir=400, naux=1800, nfreq=48),
+    3: dict(nocc=20, nvir=200, naux=900, nfreq=32),
+    4: dict(nocc=100, nvir=1200, naux=5000, nfreq=64),
+}
+
+
+def make_inputs(nocc: int, nvir: int, naux: int, seed: int = 0):
+    """Seeded synthetic G0W0 inputs.
+
+    eps_occ ~ U[-1.5, -0.3], eps_vir ~ U[0.05, 3.0] (each sorted);
+    B = U diag(s) V^T with Gaussian QR-orthonormalised U (N_ov x r), V (naux x r),
+    r = min(N_ov, naux), s_k = 0.5 exp(-k/20).  B is returned as [nocc*nvir, naux]
+    (the reference's iaP layout, quasix/src/evgw.rs:105-114).  The draws happen in the
+    order eps_occ, eps_vir, U, V.
+    """
+    rng = np.random.default_rng(seed)
+    e_occ = np.sort(rng.uniform(-1.5, -0.3, nocc))
+    e_vir = np.sort(rng.uniform(0.05, 3.0, nvir))
+    nov = nocc * nvir
+    r = min(nov, naux)
+    U = np.linalg.qr(rng.standard_normal((nov, r)))[0]
+    V = np.linalg.qr(rng.standard_normal((naux, r)))[0]
+    s = 0.5 * np.exp(-np.arange(r) / 20.0)
+    B = (U * s) @ V.T
+    return np.concatenate([e_occ, e_vir]), np.ascontiguousarray(B)
+
+
+def assemble_mnP(iaP, miP, abP, nocc, nvir, naux):
+    """Full symmetric (n_mo, n_mo, n_aux) tensor from the three DF blocks.
+
+    Restates quasix/src/evgw.rs:161-205: fill mn[:, :nocc] from miP, mn[:nocc, nocc:]
+    from iaP, mn[nocc:, nocc:] from abP, then symmetrise mn[m,n] = mn[n,m] = average.
+    """
+    nmo = nocc + nvir
+    mn = np.zeros((nmo, nmo, naux))
+    mn[:, :nocc, :] = np.asarray(miP).reshape(nmo, nocc, naux)
+    mn[:nocc, nocc:, :] = np.asarray(iaP).reshape(nocc, nvir, naux)
+    mn[nocc:, nocc:, :] = np.asarray(abP).reshape(nvir, nvir, naux)
+    return 0.5 * (mn + mn.transpose(1, 0, 2))
+
+
+def ov_only_blocks(B, nocc, nvir):
+    """miP/abP blocks consistent with an ov-only tensor (oo and vv blocks are zero)."""
+    naux = B.shape[1]
+    nmo = nocc + nvir
+    miP = np.zeros((nmo, nocc, naux))
+    miP[nocc:, :, :] = B.reshape(nocc, nvir, naux).transpose(1, 0, 2)
+    abP = np.zeros((nvir * nvir, naux))
+    return miP.reshape(nmo * nocc, naux), abP
+
+
+def pair_list(mnP, targets):
+    """(target index, partner m) pairs whose B_nm row is not identically zero.
+
+    Rows that are exactly zero contribute exactly zero to Wc_nm, so dropping them
+    changes no result; the GPU path receives the same list.
+    """
+    pt, pm, rows = [], [], []
+    for t, n in enumerate(targets):
+        for m in range(mnP.shape[1]):
+            row = mnP[n, m]
+            if np.any(row != 0.0):
+                pt.append(t)
+                pm.append(m)
+                rows.append(row)
+    naux = mnP.shape[2]
+    return (np.asarray(pt, dtype=np.int32), np.asarray(pm, dtype=np.int32),
+            np.asarray(rows, dtype=np.float64).reshape(len(rows), naux))
+
+
+# --------------------------------------------------------------------------------------
+# A2. Polarizability
+# --------------------------------------------------------------------------------------
+
+def get_rho_response(omega: float, mo_energy, B_ov, nocc: int):
+    """Pi(i omega) = 4 * B^T diag(e/(w^2+e^2)) B,  B_ov laid out [nocc*nvir, naux].
+
+    Restates correlation_ac.rs:184-211 (there lpq_ov is [naux, nocc, nvir]; the product
+    is the same).
+    """
+    e = np.asarray(mo_energy)
+    eia = (e[:nocc, None] - e[None, nocc:]).reshape(-1)
+    chi = eia / (omega * omega + eia * eia)
+    return 4.0 * (B_ov * chi[:, None]).T @ B_ov
+
+
+def compute_polarizability_p0(ia_P, mo_energy, mo_occ, freqs):
+    """Stage API: ia_P [nocc, nvir, naux] -> P0 [nf, naux, naux].
+
+    Restates quasix_core/src/dielectric/polarizability.rs:169-279 (same weights
+    4 e_ia/(w^2+e_ia^2), occupied = mo_occ > 0.5, explicit symmetrisation).
+    """
+    ia_P = np.asarray(ia_P)
+    nocc, nvir, naux = ia_P.shape
+    occ = [i for i, o in enumerate(mo_occ) if o > 0.5]
+    vir = [i for i, o in enumerate(mo_occ) if o < 0.5]
+    if len(occ) != nocc or len(vir) != nvir:
+        raise ValueError("Occupation mismatch")
+    e = np.asarray(mo_energy)
+    eia = (e[occ][:, None] - e[vir][None, :]).reshape(-1)
+    B = ia_P.reshape(nocc * nvir, naux)
+    out = np.empty((len(freqs), naux, naux))
+    for k, w in enumerate(freqs):
+        wgt = 4.0 * eia / (w * w + eia * eia)
+        p = (B * wgt[:, None]).T @ B
+        out[k] = 0.5 * (p + p.T)
+    return out
+
+
+def rho_response_complex(z: complex, mo_energy, B_ov, nocc: int):
+    """Pi(z) = 2 * B^T diag(1/(z + e) - 1/(z - e)) B at complex z.
+
+    At z = i*omega this equals get_rho_response exactly.  Used for the CD residues at
+    z = omega_r + i*eta (PySCF gw_cd real-axis response, cited by
+    correlation_pyscf_cd.rs:46; the reference's own residue code is unwired).
+    """
+    e = np.asarray(mo_energy)
+    eia = (e[:nocc, None] - e[None, nocc:]).reshape(-1)
+    d = 2.0 * (1.0 / (z + eia) - 1.0 / (z - eia))
+    return (B_ov * d[:, None]).T @ B_ov.astype(np.complex128)
+
+
+# --------------------------------------------------------------------------------------
+# A3. Screened interaction
+# --------------------------------------------------------------------------------------
+
+def screened_wc(pi):
+    """Wc = (I - Pi)^-1 - I with the reference's fallback chain.
+
+    Restates correlation_ac.rs:376-401: Cholesky inverse (DPOTRF+DPOTRI, ``invc``) ->
+    LU inverse -> identity (i.e. Wc = 0) if both fail.
+    """
+    n = pi.shape[0]
+    eps = np.eye(n) - pi
+    try:
+        c, low = sla.cho_factor(eps, lower=True)
+        inv = sla.cho_solve((c, low), np.eye(n))
+    except (np.linalg.LinAlgError, sla.LinAlgError):
+        try:
+            inv = np.linalg.inv(eps)
+        except np.linalg.LinAlgError:
+            inv = np.eye(n)
+    return inv - np.eye(n)
+
+
+def wmn_imag(mo_energy, B_ov, nocc, B_pair, freqs):
+    """Wc_nm(i w_k) = b_nm^T Wc(i w_k) b_nm for every pair row, every frequency.
+
+    Restates the per-target contraction of correlation_ac.rs:404-421
+    (q_nm = Wc^T L[:, n, :], w_m = sum_Q q_nm * L[:, :, n]) for an explicit pair list.
+    Returns [nf, npair].
+    """
+    out = np.empty((len(freqs), B_pair.shape[0]))
+    for k, w in enumerate(freqs):
+        wc = screened_wc(get_rho_response(w, mo_energy, B_ov, nocc))
+        out[k] = np.einsum("jp,jp->j", B_pair @ wc, B_pair)
+    return out
+
+
+# --------------------------------------------------------------------------------------
+# A4-A6. CD self-energy: imaginary-axis quadrature + residues
+# --------------------------------------------------------------------------------------
+
+@dataclass
+class CDContext:
+    """Everything the CD self-energy of a set of targets needs, once Wc_nm(iw) is known."""
+    mo_energy: np.ndarray          # energies in the Green's function / poles / Pi
+    mo_energy_ref: np.ndarray      # energies deciding the eta sign (mean-field eps)
+    nocc: int
+    B_ov: np.ndarray               # [N_ov, naux] for the real-axis Pi of residues
+    targets: np.ndarray            # orbital index per target
+    pair_t: np.ndarray             # target slot per pair
+    pair_m: np.ndarray             # partner orbital per pair
+    B_pair: np.ndarray             # [npair, naux]
+    freqs: np.ndarray
+    wts: np.ndarray
+    wnm: np.ndarray                # [nf, npair]
+    eta: float = 0.01
+    res_cache: dict = field(default_factory=dict)
+
+    @property
+    def ef(self):
+        e = self.mo_energy
+        return 0.5 * (e[self.nocc - 1] + e[self.nocc])
+
+
+def wc_real_axis(ctx: CDContext, z: complex, b: np.ndarray) -> complex:
+    """b^T [(I - Pi(z))^-1 - I] b for one pair row at complex frequency z (dense LU)."""
+    key = (z, id(b))
+    if key in ctx.res_cache:
+        return ctx.res_cache[key]
+    pi = rho_response_complex(z, ctx.mo_energy, ctx.B_ov, ctx.nocc)
+    eps = np.eye(pi.shape[0], dtype=np.complex128) - pi
+    x = np.linalg.solve(eps, b.astype(np.complex128))
+    val = complex(b @ x - b @ b)
+    ctx.res_cache[key] = val
+    return val
+
+
+def sigma_cd_eval(ctx: CDContext, t: int, E: float) -> complex:
+    """Sigma_c_n(E) = I_n(E) + R_n(E) for target slot t.
+
+    I_n(E) = -(1/pi) sum_k W_k sum_m Wc_nm(i w_k) * g,
+      g = emo / (emo^2 + w_k^2),  emo = E - eps_m - i eta sign(eps_n - eps_m)
+      (sign = +1 if eps_n > eps_m else -1; correlation_pyscf_cd.rs:209-226, evaluated at
+      the QP energy E instead of 0).
+    R_n(E) = +sum_{ef < eps_m < E} Wc_nm(E - eps_m + i eta)
+             -sum_{E < eps_m < ef} Wc_nm(eps_m - E + i eta)    (CD residues).
+    """
+    n = ctx.targets[t]
+    sel = ctx.pair_t == t
+    m = ctx.pair_m[sel]
+    w = ctx.wnm[:, sel]                              # [nf, np]
+    em = ctx.mo_energy[m]
+    sign = np.where(ctx.mo_energy_ref[n] > ctx.mo_energy_ref[m], 1.0, -1.0)
+    emo = (E - em) - 1j * ctx.eta * sign             # [np]
+    g = ctx.wts[:, None] * emo[None, :] / (emo[None, :] ** 2 + (ctx.freqs ** 2)[:, None])
+    sig = -np.sum(w * g) / math.pi
+    ef = ctx.ef
+    rows = np.nonzero(sel)[0]
+    for j, mm in zip(rows, m):
+        e_m = ctx.mo_energy[mm]
+        if E > ef and ef < e_m < E:
+            sig += wc_real_axis(ctx, complex(E - e_m, ctx.eta), ctx.B_pair[j])
+        elif E <= ef and E < e_m < ef:
+            sig -= wc_real_axis(ctx, complex(e_m - E, ctx.eta), ctx.B_pair[j])
+    return complex(sig)
+
+
+# --------------------------------------------------------------------------------------
+# A9. Newton QP solver
+# --------------------------------------------------------------------------------------
+
+@dataclass
+class NewtonConfig:
+    """Defaults restate quasix_core/src/qp/newton_solver.rs:84-103."""
+    max_iterations: int = 50
+    energy_tolerance: float = 1e-8
+    initial_damping: float = 1.0
+    min_damping: float = 0.1
+    max_damping: float = 1.0
+    max_step: float = 1.0
+    derivative_delta: float = 1e-6
+    use_richardson: bool = True
+    z_min: float = 0.1
+    z_max: float = 0.999
+    use_bisection_fallback: bool = True
+    max_bisection_iterations: int = 30
+
+
+@dataclass
+class NewtonResult:
+    qp_energy: float
+    residual: float
+    iterations: int
+    converged: bool
+    sigma_c: complex
+    z_factor: float
+    method: str
+
+
+class ConvergenceError(RuntimeError):
+    pass
+
+
+class NewtonQPSolver:
+    """Restates quasix_core/src/qp/newton_solver.rs:128-402.
+
+    sigma_c_eval takes omega = E - fermi_level (as the reference's closure does).
+    """
+
+    def __init__(self, epsilon_hf, sigma_x, vxc, fermi_level, config=None):
+        self.c = config or NewtonConfig()
+        self.eps = epsilon_hf
+        self.sx = sigma_x
+        self.vxc = vxc
+        self.ef = fermi_level
+
+    def _deriv(self, f, omega):          # newton_solver.rs:282-303
+        de = self.c.derivative_delta
+        if self.c.use_richardson:
+            p1 = f(omega + de).real
+            m1 = f(omega - de).real
+            p2 = f(omega + 2 * de).real
+            m2 = f(omega - 2 * de).real
+            return (8.0 * (p1 - m1) - (p2 - m2)) / (12.0 * de)
+        return (f(omega + de).real - f(omega - de).real) / (2 * de)
+
+    def _z(self, f, energy):             # newton_solver.rs:306-315
+        d = self._deriv(f, energy - self.ef)
+        return min(max(1.0 / (1.0 - d), self.c.z_min), self.c.z_max)
+
+    def _residual(self, f, energy):      # newton_solver.rs:394-401
+        return energy - self.eps - self.sx - f(energy - self.ef).real + self.vxc
+
+    def solve(self, f, initial_guess):   # newton_solver.rs:174-279
+        c = self.c
+        energy = initial_guess
+        damping = c.initial_damping
+        prev = math.inf
+        for it in range(c.max_iterations):
+            omega = energy - self.ef
+            sc = f(omega)
+            res = energy - self.eps - self.sx - sc.real + self.vxc
+            ares = abs(res)
+            if ares < c.energy_tolerance:
+                return NewtonResult(energy, ares, it + 1, True, sc, self._z(f, energy), "newton")
+            fp = 1.0 - self._deriv(f, omega)
+            if abs(fp) < 1e-10:
+                step = -math.copysign(1.0, res) * min(c.max_step, ares)
+            else:
+                step = -res / fp
+            step = min(max(step, -c.max_step), c.max_step)
+            if ares > prev:
+                damping = max(damping * 0.5, c.min_damping)
+            elif ares < 0.5 * prev and damping < c.max_damping:
+                damping = min(damping * 1.2, c.max_damping)
+            energy += damping * step
+            prev = ares
+        if c.use_bisection_fallback:
+            return self._bisect(f, initial_guess)
+        raise ConvergenceError("Newton-Raphson failed to converge")
+
+    def _bisect(self, f, guess):         # newton_solver.rs:317-391
+        c = self.c
+        width = 2.0
+        lo, hi = guess - width, guess + width
+        if self._residual(f, lo) * self._residual(f, hi) > 0.0:
+            for k in range(1, 5):
+                w = width * 2.0 ** k
+                lo, hi = guess - w, guess + w
+                if self._residual(f, lo) * self._residual(f, hi) <= 0.0:
+                    break
+        for it in range(c.max_bisection_iterations):
+            mid = 0.5 * (lo + hi)
+            fm = self._residual(f, mid)
+            if abs(fm) < c.energy_tolerance or 0.5 * (hi - lo) < c.energy_tolerance:
+                return NewtonResult(mid, abs(fm), it + 1, True, f(mid - self.ef),
+                                    self._z(f, mid), "bisection")
+            if fm * self._residual(f, lo) < 0.0:
+                hi = mid
+            else:
+                lo = mid
+        raise ConvergenceError("Bisection fallback failed to converge")
+
+
+def linearized_guess(f, eps_p, static, ef, de=1e-6, z_min=0.1, z_max=0.999):
+    """Iteration-0 Newton start: E = eps + Z (static + ReSigma(eps)), forward-difference Z.
+
+    Restates quasix_core/src/qp/evgw.rs:730-741.
+    """
+    om = eps_p - ef
+    s0 = f(om).real
+    s1 = f(om + de).real
+    z = min(max(1.0 / (1.0 - (s1 - s0) / de), z_min), z_max)
+    return eps_p + z * (static + s0)
+
+
+# --------------------------------------------------------------------------------------
+# Full G0W0 solve (CD + Newton) and evGW
+# --------------------------------------------------------------------------------------
+
+def frontier_targets(nocc, nmo, below=5, above=5):
+    """HOMO-below .. LUMO+above (clipped to the orbital range)."""
+    lo = max(0, nocc - 1 - below)
+    hi = min(nmo, nocc + above + 1)
+    return np.arange(lo, hi, dtype=np.int64)
+
+
+def g0w0_cd(mo_energy, B_ov, nocc, targets, freqs, wts, *, pairs=None, static=None,
+            eta=0.01, mo_energy_ref=None, newton=None, guesses=None):
+    """One G0W0 solve: Pi -> Wc -> Wc_nm(iw) -> Newton on Sigma_c = I + R.
+
+    Follows the iteration-0 branch of quasix_core/src/qp/evgw.rs:603-783 with the AC
+    Sigma replaced by the CD Sigma.  ``pairs`` = (pair_t, pair_m, B_pair); default is the
+    ov-only pair list.  ``static`` = Sigma_x - V_xc per target (default 0).
+    Returns dict(E, Z, sigma, wnm, ...).
+    """
+    e = np.asarray(mo_energy, dtype=np.float64)
+    targets = np.asarray(targets)
+    nt = len(targets)
+    if pairs is None:
+        miP, abP = ov_only_blocks(B_ov, nocc, len(e) - nocc)
+        mn = assemble_mnP(B_ov, miP, abP, nocc, len(e) - nocc, B_ov.shape[1])
+        pairs = pair_list(mn, targets)
+    pt, pm, Bp = pairs
+    static = np.zeros(nt) if static is None else np.asarray(static, dtype=np.float64)
+    wnm = wmn_imag(e, B_ov, nocc, Bp, freqs)
+    ctx = CDContext(e, e if mo_energy_ref is None else np.asarray(mo_energy_ref), nocc, B_ov,
+                    targets, pt, pm, Bp, np.asarray(freqs), np.asarray(wts), wnm, eta)
+    ef = ctx.ef
+    E = np.empty(nt)
+    Z = np.empty(nt)
+    S = np.empty(nt, dtype=np.complex128)
+    conv = np.empty(nt, dtype=bool)
+    for t, n in enumerate(targets):
+        f = lambda om, t=t: sigma_cd_eval(ctx, t, om + ef)
+        g = linearized_guess(f, e[n], static[t], ef) if guesses is None else guesses[t]
+        solver = NewtonQPSolver(e[n], static[t], 0.0, ef, newton)
+        try:
+            r = solver.solve(f, g)
+            E[t], Z[t], S[t], conv[t] = r.qp_energy, r.z_factor, r.sigma_c, r.converged
+        except ConvergenceError:        # evgw.rs:766-781: fall back to linearized
+            om = g - ef
+            s0 = f(om)
+            s1 = f(om + 1e-6)
+            z = min(max(1.0 / (1.0 - (s1.real - s0.real) / 1e-6), 0.1), 0.999)
+            E[t], Z[t], S[t], conv[t] = e[n] + z * (static[t] + s0.real), z, s0, False
+    return dict(E=E, Z=Z, sigma=S, converged=conv, wnm=wnm, pairs=pairs, ef=ef)
+
+
+def evgw_cd(mo_energy, B_ov, nocc, targets, freqs, wts, *, n_iter=8, damping=0.5,
+            eta=0.01, tol=1e-4, static=None):
+    """Eigenvalue-self-consistent GW on the CD self-energy.
+
+    Follows quasix_core/src/qp/evgw.rs:603-1027: iteration 0 is G0W0 on the mean-field
+    energies; iteration k > 0 rebuilds Pi, the Green's function poles and the residues
+    from the previous QP energies (:626-633); iteration 1+ mixes with ``damping``
+    (:842-850); Newton restarts from the previous QP energy (:742).  Orbitals outside
+    ``targets`` keep their mean-field energies.  Convergence: max |dE| over the targets
+    below ``tol`` (:931-1027 with the frontier window equal to the target set).
+    """
+    e0 = np.asarray(mo_energy, dtype=np.float64)
+    targets = np.asarray(targets)
+    cur = e0.copy()
+    hist = []
+    res = None
+    for it in range(n_iter):
+        guesses = None if it == 0 else cur[targets]
+        res = _g0w0_iter(cur, e0, B_ov, nocc, targets, freqs, wts, static, eta, guesses)
+        new = cur.copy()
+        if it == 0:
+            new[targets] = res["E"]
+        else:
+            new[targets] = (1.0 - damping) * cur[targets] + damping * res["E"]
+        dmax = float(np.max(np.abs(new[targets] - cur[targets])))
+        hist.append(dict(cycle=it, qp_energies=new.copy(), energy_change=dmax))
+        cur = new
+        if it > 0 and dmax < tol:
+            break
+    return dict(E=cur, history=hist, last=res)
+
+
+def _g0w0_iter(cur, e0, B_ov, nocc, targets, freqs, wts, static, eta, guesses):
+    """One evGW step: Pi/poles from ``cur``; QP equation anchored at the mean-field e0."""
+    nt = len(targets)
+    static = np.zeros(nt) if static is None else np.asarray(static)
+    miP, abP = ov_only_blocks(B_ov, nocc, len(cur) - nocc)
+    mn = assemble_mnP(B_ov, miP, abP, nocc, len(cur) - nocc, B_ov.shape[1])
+    pt, pm, Bp = pair_list(mn, targets)
+    wnm = wmn_imag(cur, B_ov, nocc, Bp, freqs)
+    ctx = CDContext(cur, e0, nocc, B_ov, np.asarray(targets), pt, pm, Bp,
+                    np.asarray(freqs), np.asarray(wts), wnm, eta)
+    ef = ctx.ef
+    E = np.empty(nt)
+    Z = np.empty(nt)
+    for t, n in enumerate(targets):
+        f = lambda om, t=t: sigma_cd_eval(ctx, t, om + ef)
+        g = linearized_guess(f, e0[n], static[t], ef) if guesses is None else guesses[t]
+        r = NewtonQPSolver(e0[n], static[t], 0.0, ef).solve(f, g)
+        E[t], Z[t] = r.qp_energy, r.z_factor
+    return dict(E=E, Z=Z, wnm=wnm)
+
+
+# --------------------------------------------------------------------------------------
+# A4/A7/A8. Analytic-continuation path (the reference's live Sigma method)
+# --------------------------------------------------------------------------------------
+
+def get_sigma_diag(mo_energy, lpq, orbs, nocc, nw=100, x0=0.5, iw_cutoff=5.0):
+    """Sigma_c(i nu) on the AC grid.  Restates correlation_ac.rs:241-449.
+
+    lpq is the full [naux, nmo, nmo] tensor.  Returns (sigma [norbs, nw_sigma],
+    omega [norbs, nw_sigma]) as complex arrays.
+    """
+    e = np.asarray(mo_energy)
+    naux, nmo, _ = lpq.shape
+    if nocc == 0 or nocc >= nmo:
+        raise ValueError(f"Invalid nocc={nocc}")
+    freqs, wts = get_scaled_legendre_roots(nw, x0)
+    idx = [k for k, f in enumerate(freqs) if f < iw_cutoff]
+    nws = min(len(idx) + 1, nw + 1)
+    ef = 0.5 * (e[nocc - 1] + e[nocc])
+    om_occ = np.zeros(nws, dtype=np.complex128)
+    om_vir = np.zeros(nws, dtype=np.complex128)
+    for k, fi in enumerate(idx):
+        if k + 1 < nws:
+            om_occ[k + 1] = -1j * freqs[fi]
+            om_vir[k + 1] = 1j * freqs[fi]
+    emo_occ = om_occ[None, :] + ef - e[:, None]
+    emo_vir = om_vir[None, :] + ef - e[:, None]
+    lpq_ov = lpq[:, :nocc, nocc:]
+    B_ov = lpq_ov.transpose(1, 2, 0).reshape(nocc * (nmo - nocc), naux)
+    sigma = np.zeros((len(orbs), nws), dtype=np.complex128)
+    omega = np.array([om_occ if o < nocc else om_vir for o in orbs])
+    for w in range(nw):
+        pi = get_rho_response(freqs[w], e, B_ov, nocc)
+        eps = np.eye(naux) - pi
+        try:
+            c = sla.cho_factor(eps, lower=True)
+            inv = sla.cho_solve(c, np.eye(naux))
+        except (np.linalg.LinAlgError, sla.LinAlgError):
+            try:
+                inv = np.linalg.inv(eps)
+            except np.linalg.LinAlgError:
+                inv = np.eye(naux)
+        pinv = inv - np.eye(naux)
+        g_occ = wts[w] * emo_occ / (emo_occ ** 2 + freqs[w] ** 2)
+        g_vir = wts[w] * emo_vir / (emo_vir ** 2 + freqs[w] ** 2)
+        for i, n in enumerate(orbs):
+            q = pinv.T @ lpq[:, n, :]
+            wm = np.sum(q * lpq[:, :, n], axis=0)
+            g = g_occ if n < nocc else g_vir
+            sigma[i] += (-1.0 / math.pi) * (wm @ g.real + 1j * (wm @ g.imag))
+    return sigma, omega
+
+
+def thiele(fn, zn):
+    """Thiele reciprocal-difference coefficients.  Restates correlation_ac.rs:470-509."""
+    fn = np.asarray(fn, dtype=np.complex128)
+    zn = np.asarray(zn, dtype=np.complex128)
+    nfit = len(fn)
+    if nfit == 0:
+        return np.zeros(0, dtype=np.complex128)
+    g = np.zeros((nfit, nfit), dtype=np.complex128)
+    g[:, 0] = fn
+    for j in range(1, nfit):
+        for i in range(j, nfit):
+            d = (zn[i] - zn[j - 1]) * g[i, j - 1]
+            g[i, j] = 1e10 if abs(d) < 1e-14 else (g[j - 1, j - 1] - g[i, j - 1]) / d
+    return np.diag(g).copy()
+
+
+def pade_thiele(omega, zn, coeff):
+    """Continued-fraction evaluation.  Restates correlation_ac.rs:530-566."""
+    nfit = len(coeff)
+    if nfit == 0:
+        return 0j
+    if nfit == 1:
+        return complex(coeff[0])
+    z = complex(omega, 0.0)
+    x = coeff[nfit - 1] * (z - zn[nfit - 2])
+    for i in range(nfit - 2, 0, -1):
+        d = 1.0 + x
+        x = 1e10 if abs(d) < 1e-14 else coeff[i] * (z - zn[i - 1]) / d
+    d = 1.0 + x
+    return complex(coeff[0]) if abs(d) < 1e-14 else complex(coeff[0] / d)
+
+
+def ac_pade_thiele_diag(sigma, omega):
+    """Point selection 1,7,..,37 then every 4th; even count.  Restates :587-654."""
+    norbs, nw = sigma.shape
+    idx = list(range(1, min(40, nw), 6))
+    if idx:
+        idx += list(range(idx[-1] + 4, nw, 4))
+    nfit = (len(idx) // 2) * 2
+    idx = idx[:nfit]
+    coeff = np.zeros((nfit, norbs), dtype=np.complex128)
+    ofit = omega[:, idx]
+    for p in range(norbs):
+        coeff[:, p] = thiele(sigma[p, idx], ofit[p])
+    return coeff, ofit
+
+
+def linearized_qp_ac(mo_energy, sigma_x, vxc, coeff, ofit, orbs, nocc, de=1e-6):
+    """E = eps + Z (Sigma_x + ReSigma_c(eps) - Vxc), forward-difference Z (no clamp).
+
+    Restates correlation_ac.rs:684-714.
+    """
+    e = np.asarray(mo_energy)
+    ef = 0.5 * (e[nocc - 1] + e[nocc])
+    out = np.empty(len(orbs))
+    for i, p in enumerate(orbs):
+        s = pade_thiele(e[p] - ef, ofit[i], coeff[:, i]).real
+        sp = pade_thiele(e[p] - ef + de, ofit[i], coeff[:, i]).real
+        z = 1.0 / (1.0 - (sp - s) / de)
+        out[i] = e[p] + z * (sigma_x[p] + s - vxc[p])
+    return out

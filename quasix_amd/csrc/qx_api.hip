@@ -1,0 +1,1016 @@
+// C ABI of the G0W0/evGW correlation path (see include/qxgw.h).
+//
+// Host driver for the HIP kernels: owns the stream and a grow-only device workspace,
+// sequences Pi -> Cholesky -> L^-1 -> Wc_nm -> Newton for one solve, runs the numerical
+// fallbacks of the reference (Cholesky -> LU -> identity, correlation_ac.rs:376-401;
+// Newton -> bisection -> linearised, newton_solver.rs:262-268 and evgw.rs:766-781), and
+// drives the residue evaluations of the contour deformation when a Newton iterate crosses
+// an orbital energy (the only part of a solve that needs host decisions per iteration).
+#include "../../include/qxgw.h"
+#include "qx_internal.h"
+#include "qx_newton.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+using namespace qx;
+
+struct qx_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    std::string err;
+    std::map<std::string, std::pair<void*, size_t>> bufs;
+    // profiling: HIP events around each phase, resolved after the solve's final sync
+    bool prof = false;
+    double phase_ms[QX_NPHASE] = {0};
+    long long phase_n[QX_NPHASE] = {0};
+    std::vector<hipEvent_t> ev_free;
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_pending;
+};
+
+namespace {
+
+struct QxFail {
+    qx_status st;
+};
+
+void set_err(qx_handle h, const char* fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    h->err = buf;
+}
+
+#define HIPCK(h, expr)                                                                    \
+    do {                                                                                  \
+        hipError_t e_ = (expr);                                                           \
+        if (e_ != hipSuccess) {                                                           \
+            set_err(h, "HIP error %s at %s:%d (%s)", hipGetErrorString(e_), __FILE__,     \
+                    __LINE__, #expr);                                                     \
+            throw QxFail{QX_ERR_DEVICE};                                                  \
+        }                                                                                 \
+    } while (0)
+
+#define REQUIRE(h, cond, code, ...)                                                       \
+    do {                                                                                  \
+        if (!(cond)) {                                                                    \
+            set_err(h, __VA_ARGS__);                                                      \
+            throw QxFail{code};                                                           \
+        }                                                                                 \
+    } while (0)
+
+hipEvent_t take_event(qx_handle h) {
+    if (!h->ev_free.empty()) {
+        hipEvent_t e = h->ev_free.back();
+        h->ev_free.pop_back();
+        return e;
+    }
+    hipEvent_t e;
+    HIPCK(h, hipEventCreate(&e));
+    return e;
+}
+
+// Records start/stop events of one phase on the handle's stream while profiling is on.
+struct Phase {
+    qx_handle h;
+    int id;
+    hipEvent_t a = nullptr;
+    Phase(qx_handle h_, int id_) : h(h_), id(id_) {
+        if (h->prof) {
+            a = take_event(h);
+            HIPCK(h, hipEventRecord(a, h->stream));
+        }
+    }
+    ~Phase() {
+        if (!a) return;
+        hipEvent_t b = take_event(h);
+        if (hipEventRecord(b, h->stream) == hipSuccess) h->ev_pending.push_back({id, {a, b}});
+    }
+};
+
+void resolve_profile(qx_handle h) {
+    if (h->ev_pending.empty()) return;
+    HIPCK(h, hipStreamSynchronize(h->stream));
+    for (auto& p : h->ev_pending) {
+        float ms = 0.f;
+        HIPCK(h, hipEventElapsedTime(&ms, p.second.first, p.second.second));
+        h->phase_ms[p.first] += ms;
+        h->phase_n[p.first] += 1;
+        h->ev_free.push_back(p.second.first);
+        h->ev_free.push_back(p.second.second);
+    }
+    h->ev_pending.clear();
+}
+
+template <typename T>
+T* ws(qx_handle h, const std::string& name, size_t count) {
+    const size_t bytes = std::max<size_t>(count * sizeof(T), 256);
+    auto it = h->bufs.find(name);
+    if (it != h->bufs.end() && it->second.second >= bytes) return static_cast<T*>(it->second.first);
+    if (it != h->bufs.end()) {
+        HIPCK(h, hipStreamSynchronize(h->stream));
+        HIPCK(h, hipFree(it->second.first));
+        h->bufs.erase(it);
+    }
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) {
+        set_err(h, "device allocation of %zu bytes for '%s' failed: %s", bytes, name.c_str(),
+                hipGetErrorString(e));
+        throw QxFail{QX_ERR_ALLOC};
+    }
+    h->bufs[name] = {p, bytes};
+    return static_cast<T*>(p);
+}
+
+template <typename T>
+void d2h(qx_handle h, T* dst, const T* src, size_t n) {
+    if (n == 0) return;
+    HIPCK(h, hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyDeviceToHost, h->stream));
+    HIPCK(h, hipStreamSynchronize(h->stream));
+}
+template <typename T>
+void h2d(qx_handle h, T* dst, const T* src, size_t n) {
+    if (n == 0) return;
+    HIPCK(h, hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyHostToDevice, h->stream));
+    HIPCK(h, hipStreamSynchronize(h->stream));
+}
+
+// ---- small utility kernels --------------------------------------------------------------
+__global__ void fill_i32_kernel(int* p, int v, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+__global__ void fill_f64_kernel(double* p, double v, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+// dst[r][0:ncol] = src[r][0:ncol], dst[r][ncol:ldd] = 0
+__global__ void pad_rows_kernel(const double* __restrict__ src, int nrow, int ncol,
+                                double* __restrict__ dst, int ldd) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < (size_t)nrow * ldd;
+         i += (size_t)gridDim.x * blockDim.x) {
+        const size_t r = i / ldd, c = i % ldd;
+        dst[i] = c < (size_t)ncol ? src[r * ncol + c] : 0.0;
+    }
+}
+// eps[f][npad][npad] = I - pi[f][naux][naux], identity in the padding
+__global__ void eps_from_pi_kernel(const double* __restrict__ pi, int naux, int npad, int nf,
+                                   double* __restrict__ eps) {
+    const size_t total = (size_t)nf * npad * npad;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total;
+         i += (size_t)gridDim.x * blockDim.x) {
+        const size_t f = i / ((size_t)npad * npad), rc = i % ((size_t)npad * npad);
+        const int r = rc / npad, c = rc % npad;
+        double v = r == c ? 1.0 : 0.0;
+        if (r < naux && c < naux) v -= pi[(f * naux + r) * naux + c];
+        eps[i] = v;
+    }
+}
+
+// ov-only pair rows: occupied target n pairs with every virtual a (row n*nvir + a of B_ov),
+// virtual target n = nocc + a pairs with every occupied i (row i*nvir + a).
+__global__ void ov_pairs_kernel(const double* __restrict__ B, int nocc, int nvir, int naux,
+                                const int* __restrict__ targets, const int* __restrict__ off,
+                                int nt, int* __restrict__ pt, int* __restrict__ pm,
+                                double* __restrict__ rows) {
+    const int t = blockIdx.y;
+    if (t >= nt) return;
+    const int n = targets[t];
+    const int j0 = off[t], cnt = off[t + 1] - off[t];
+    for (int k = blockIdx.x; k < cnt; k += gridDim.x) {
+        const int j = j0 + k;
+        int r, m;
+        if (n < nocc) { r = n * nvir + k; m = nocc + k; }
+        else { r = k * nvir + (n - nocc); m = k; }
+        if (threadIdx.x == 0) { pt[j] = t; pm[j] = m; }
+        for (int c = threadIdx.x; c < naux; c += blockDim.x)
+            rows[(size_t)j * naux + c] = B[(size_t)r * naux + c];
+    }
+}
+
+dim3 grid1d(size_t n) {
+    size_t b = (n + 255) / 256;
+    if (b > 65535) b = 65535;
+    if (b < 1) b = 1;
+    return dim3((unsigned)b);
+}
+
+void fill_i32(qx_handle h, int* p, int v, size_t n) {
+    hipLaunchKernelGGL(fill_i32_kernel, grid1d(n), dim3(256), 0, h->stream, p, v, n);
+    HIPCK(h, hipGetLastError());
+}
+void fill_f64(qx_handle h, double* p, double v, size_t n) {
+    hipLaunchKernelGGL(fill_f64_kernel, grid1d(n), dim3(256), 0, h->stream, p, v, n);
+    HIPCK(h, hipGetLastError());
+}
+
+// ---- grids --------------------------------------------------------------------------------
+// Gauss-Legendre by Newton iteration on P_n (the variant of
+// quasix_core/src/freq/gauss_legendre.rs:24-86); nodes ascending, exact +- symmetry.
+void gauss_legendre(int n, std::vector<double>& x, std::vector<double>& w) {
+    x.assign(n, 0.0);
+    w.assign(n, 0.0);
+    if (n == 1) {
+        w[0] = 2.0;
+        return;
+    }
+    const double pi = 3.14159265358979323846;
+    for (int i = 0; i < (n + 1) / 2; ++i) {
+        double z = std::cos(pi * (i + 0.75) / (n + 0.5));
+        double dp = 0.0;
+        for (int it = 0; it < 100; ++it) {
+            double p0 = 1.0, p1 = z;
+            for (int k = 2; k <= n; ++k) {
+                const double p2 = ((2.0 * k - 1.0) * z * p1 - (k - 1.0) * p0) / k;
+                p0 = p1;
+                p1 = p2;
+            }
+            dp = n * (z * p1 - p0) / (z * z - 1.0);
+            const double dz = p1 / dp;
+            z -= dz;
+            if (std::fabs(dz) < 1e-16) break;
+        }
+        {   // derivative at the converged root
+            double p0 = 1.0, p1 = z;
+            for (int k = 2; k <= n; ++k) {
+                const double p2 = ((2.0 * k - 1.0) * z * p1 - (k - 1.0) * p0) / k;
+                p0 = p1;
+                p1 = p2;
+            }
+            dp = n * (z * p1 - p0) / (z * z - 1.0);
+        }
+        const double wt = 2.0 / ((1.0 - z * z) * dp * dp);
+        x[n - 1 - i] = z;
+        x[i] = -z;
+        w[n - 1 - i] = wt;
+        w[i] = wt;
+    }
+    if (n % 2 == 1) x[n / 2] = 0.0;
+}
+
+void scaled_roots(int nw, double x0, std::vector<double>& f, std::vector<double>& w) {
+    std::vector<double> t, g;
+    gauss_legendre(nw, t, g);
+    f.resize(nw);
+    w.resize(nw);
+    for (int i = 0; i < nw; ++i) {
+        const double omt = 1.0 - t[i];
+        f[i] = x0 * (1.0 + t[i]) / omt;
+        w[i] = g[i] * 2.0 * x0 / (omt * omt);
+    }
+}
+
+NewtonParams to_params(const qx_newton_config& c) {
+    NewtonParams p;
+    p.max_iterations = c.max_iterations;
+    p.energy_tolerance = c.energy_tolerance;
+    p.initial_damping = c.initial_damping;
+    p.min_damping = c.min_damping;
+    p.max_damping = c.max_damping;
+    p.max_step = c.max_step;
+    p.derivative_delta = c.derivative_delta;
+    p.z_min = c.z_min;
+    p.z_max = c.z_max;
+    p.use_richardson = c.use_richardson;
+    p.use_bisection = c.use_bisection_fallback;
+    p.max_bisection_iterations = c.max_bisection_iterations;
+    return p;
+}
+
+// ---- Wc_nm(i w) for all frequencies ------------------------------------------------------
+struct WnmResult {
+    int chol_fallbacks = 0;
+    int singular = 0;
+};
+
+WnmResult build_wnm(qx_handle h, int precision, int nocc, int nvir, int naux,
+                    const double* eps_dev, const double* B_ov, int nf, const double* freqs_dev,
+                    int npair, const double* B_pair, double* wnm_out) {
+    WnmResult r;
+    const int nov = nocc * nvir;
+    const int npad = round_up(naux, NB), nblk = npad / NB, kpad = round_up(nov, 32);
+    hipStream_t st = h->stream;
+    double* s = ws<double>(h, "pi_weights", (size_t)nf * kpad);
+    double* eps_mat = ws<double>(h, "eps_mat", (size_t)nf * npad * npad);
+    double* linv = ws<double>(h, "linv_diag", (size_t)nf * nblk * NB * NB);
+    int* fail = ws<int>(h, "chol_fail", nf);
+    PiOperands ops;
+    {
+        Phase ph(h, QX_PHASE_PREP);
+        HIPCK(h, launch_pi_weights(eps_dev, nocc, nvir, nf, freqs_dev, s, st));
+        if (precision == QX_PREC_BF16X3) {
+            uint16_t* hi = ws<uint16_t>(h, "bt_hi", (size_t)npad * kpad);
+            uint16_t* lo = ws<uint16_t>(h, "bt_lo", (size_t)npad * kpad);
+            float* f32 = ws<float>(h, "bt_f32", (size_t)npad * kpad);
+            HIPCK(h, launch_pi_prepare_mixed(B_ov, nov, naux, npad, kpad, hi, lo, f32, st));
+            ops.bt_hi = hi;
+            ops.bt_lo = lo;
+            ops.bt_f32 = f32;
+            ops.kpad = kpad;
+        }
+        fill_i32(h, fail, 0, nf);
+    }
+    {
+        Phase ph(h, QX_PHASE_PI);
+        HIPCK(h, launch_pi_syrk(precision, B_ov, ops, s, nov, naux, npad, nf, eps_mat,
+                                PiOut::EpsLower, st));
+    }
+    {
+        Phase ph(h, QX_PHASE_POTRF);
+        HIPCK(h, launch_potrf_batched(eps_mat, linv, npad, nf, fail, st));
+    }
+    std::vector<int> fail_h(nf);
+    d2h(h, fail_h.data(), fail, nf);
+    for (int f = 0; f < nf; ++f) r.chol_fallbacks += fail_h[f] != 0;
+
+    double* bp = ws<double>(h, "bpairs", (size_t)std::max(npair, 1) * npad);
+    if (npair > 0) {
+        hipLaunchKernelGGL(pad_rows_kernel, grid1d((size_t)npair * npad), dim3(256), 0, st,
+                           B_pair, npair, naux, bp, npad);
+        HIPCK(h, hipGetLastError());
+    }
+    double* wpart = ws<double>(h, "wpart", (size_t)nf * nblk * std::max(npair, 1));
+    if (r.chol_fallbacks == 0) {
+        double* x = ws<double>(h, "xinv", (size_t)nf * npad * npad);
+        {
+            Phase ph(h, QX_PHASE_TRTRI);
+            HIPCK(h, launch_trtri_batched(eps_mat, linv, x, npad, nf, st));
+        }
+        if (npair > 0) {
+            Phase ph(h, QX_PHASE_WNM);
+            HIPCK(h, launch_wnm(precision, bp, x, npair, npad, nf, wpart, wnm_out, st));
+        }
+    } else {
+        // Reference chain: Cholesky failed somewhere -> LU inverse (here Gauss-Jordan with
+        // partial pivoting on every frequency) -> identity if singular.
+        double* pi = ws<double>(h, "xinv", (size_t)nf * npad * npad);
+        fill_f64(h, pi, 0.0, (size_t)nf * npad * npad);
+        HIPCK(h, launch_pi_syrk(precision, B_ov, ops, s, nov, naux, npad, nf, pi, PiOut::PiFull,
+                                st));
+        double* work = ws<double>(h, "gj_work", (size_t)nf * npad * 2 * npad + nf);
+        double* wc = eps_mat;
+        HIPCK(h, launch_lu_fallback_wc(pi, 0, work, wc, npad, nf, nullptr, st));
+        std::vector<int> sing(nf);
+        d2h(h, sing.data(), reinterpret_cast<int*>(work + (size_t)nf * npad * 2 * npad), nf);
+        for (int f = 0; f < nf; ++f) r.singular += sing[f] != 0;
+        if (npair > 0)
+            HIPCK(h, launch_wnm_from_wc(bp, wc, npair, npad, nf, wpart, wnm_out, st));
+    }
+    return r;
+}
+
+// ---- host view of a CD problem for the residue path ------------------------------------
+struct HostCD {
+    qx_handle h;
+    int nocc, nvir, naux, nmo, nt, npair, nf;
+    const double* B_ov;            // device
+    const double* eps_g_dev;       // device
+    const double* B_pair;          // device [npair][naux]
+    std::vector<double> eps_g, eps_ref;
+    std::vector<int> targets, pair_m, pair_off;
+    double ef, eta;
+    SigmaProblem prob;             // device problem (all targets)
+    int residue_jobs = 0;
+
+    // Sigma_c(E) for target t at nq energies: I on the GPU quadrature kernel, residues as
+    // batched real-axis solves.
+    void eval(int t, const double* E, int nq, C2* out) {
+        SigmaProblem sub = prob;
+        sub.nt = 1;
+        sub.targets = prob.targets + t;
+        sub.pair_off = prob.pair_off + t;
+        sub.static_terms = prob.static_terms + t;
+        double* e_dev = ws<double>(h, "hcd_E", 8);
+        double* o_dev = ws<double>(h, "hcd_out", 16);
+        h2d(h, e_dev, E, nq);
+        HIPCK(h, launch_sigma_imag_eval(sub, e_dev, nq, o_dev, h->stream));
+        std::vector<double> o(2 * nq);
+        d2h(h, o.data(), o_dev, 2 * nq);
+        for (int q = 0; q < nq; ++q) out[q] = C2{o[2 * q], o[2 * q + 1]};
+        // residue jobs
+        std::vector<double> om;
+        std::vector<int> row, qidx;
+        std::vector<double> sgn;
+        for (int q = 0; q < nq; ++q)
+            for (int j = pair_off[t]; j < pair_off[t + 1]; ++j) {
+                const double em = eps_g[pair_m[j]];
+                if (E[q] > ef && ef < em && em < E[q]) {
+                    om.push_back(E[q] - em); row.push_back(j); qidx.push_back(q); sgn.push_back(1.0);
+                } else if (E[q] <= ef && E[q] < em && em < ef) {
+                    om.push_back(em - E[q]); row.push_back(j); qidx.push_back(q); sgn.push_back(-1.0);
+                }
+            }
+        const int nj = (int)om.size();
+        if (nj == 0) return;
+        residue_jobs += nj;
+        const size_t per = (size_t)naux * (naux + 1) * 2;
+        int chunk = (int)std::max<size_t>(1, std::min<size_t>(64, (size_t)(2ull << 30) / (per * 8)));
+        for (int j0 = 0; j0 < nj; j0 += chunk) {
+            const int cnt = std::min(chunk, nj - j0);
+            double* om_dev = ws<double>(h, "res_om", chunk);
+            double* rows = ws<double>(h, "res_rows", (size_t)chunk * naux);
+            double* work = ws<double>(h, "res_work", (size_t)chunk * per);
+            double* wout = ws<double>(h, "res_out", (size_t)chunk * 2);
+            h2d(h, om_dev, om.data() + j0, cnt);
+            for (int k = 0; k < cnt; ++k)
+                HIPCK(h, hipMemcpyAsync(rows + (size_t)k * naux, B_pair + (size_t)row[j0 + k] * naux,
+                                        naux * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+            Phase ph(h, QX_PHASE_RESIDUE);
+            HIPCK(h, launch_residue_jobs(B_ov, eps_g_dev, nocc, nvir, naux, cnt, om_dev, eta,
+                                         rows, work, wout, h->stream));
+            std::vector<double> w(2 * cnt);
+            d2h(h, w.data(), wout, 2 * cnt);
+            for (int k = 0; k < cnt; ++k) {
+                out[qidx[j0 + k]].re += sgn[j0 + k] * w[2 * k];
+                out[qidx[j0 + k]].im += sgn[j0 + k] * w[2 * k + 1];
+            }
+        }
+    }
+};
+
+struct HostEval {
+    HostCD* cd;
+    int t;
+    bool operator()(const double* E, int nq, C2* out) {
+        cd->eval(t, E, nq, out);
+        return false;
+    }
+};
+
+// Solve target t on the host-driven path; returns the QX_QP_* state.
+int host_solve(HostCD& cd, const NewtonParams& np, int t, double static_t, bool have_guess,
+               double guess, double& E, double& Z, C2& s) {
+    HostEval ev{&cd, t};
+    const int n = cd.targets[t];
+    int iters = 0;
+    const int st = newton_core(ev, np, cd.eps_ref[n], static_t, cd.ef, have_guess, guess, E, Z,
+                               s, iters);
+    if (st == kNewtonConverged) return QX_QP_NEWTON;
+    if (st == kNewtonBisection) return QX_QP_BISECTION;
+    // Linearised fallback at the Newton start (evgw.rs:766-781).
+    double g = guess;
+    const double de = 1e-6;
+    if (!have_guess) {
+        const double om = cd.eps_ref[n] - cd.ef;
+        double Es[2] = {om + cd.ef, (om + np.derivative_delta) + cd.ef};
+        C2 r[2];
+        ev(Es, 2, r);
+        double z = 1.0 / (1.0 - (r[1].re - r[0].re) / np.derivative_delta);
+        z = std::min(std::max(z, np.z_min), np.z_max);
+        g = cd.eps_ref[n] + z * (static_t + r[0].re);
+    }
+    const double om = g - cd.ef;
+    double Es[2] = {om + cd.ef, (om + de) + cd.ef};
+    C2 r[2];
+    ev(Es, 2, r);
+    double z = 1.0 / (1.0 - (r[1].re - r[0].re) / de);
+    z = std::min(std::max(z, 0.1), 0.999);
+    E = cd.eps_ref[n] + z * (static_t + r[0].re);
+    Z = z;
+    s = r[0];
+    return QX_QP_LINEARIZED;
+}
+
+void validate_desc(qx_handle h, const qx_g0w0_desc* d) {
+    REQUIRE(h, d != nullptr, QX_ERR_INVALID_INPUT, "descriptor is NULL");
+    REQUIRE(h, d->nocc > 0 && d->nvir > 0, QX_ERR_INVALID_INPUT,
+            "Invalid nocc=%d/nvir=%d: both must be positive", d->nocc, d->nvir);
+    REQUIRE(h, d->naux > 0, QX_ERR_INVALID_INPUT, "naux must be positive (got %d)", d->naux);
+    REQUIRE(h, d->nfreq > 0, QX_ERR_INVALID_INPUT, "nfreq must be positive (got %d)", d->nfreq);
+    REQUIRE(h, d->x0 > 0.0, QX_ERR_INVALID_INPUT, "x0 must be positive");
+    REQUIRE(h, d->eta >= 0.0, QX_ERR_INVALID_INPUT, "eta must be non-negative");
+    REQUIRE(h, d->precision >= 0 && d->precision <= 2, QX_ERR_INVALID_INPUT,
+            "unknown precision %d", d->precision);
+    REQUIRE(h, d->ntarget > 0, QX_ERR_INVALID_INPUT, "ntarget must be positive");
+    REQUIRE(h, d->npair >= 0, QX_ERR_INVALID_INPUT, "npair must be non-negative");
+    REQUIRE(h, (long long)d->nocc * d->nvir < (1ll << 31), QX_ERR_INVALID_INPUT,
+            "nocc*nvir overflows int");
+}
+
+// Builds the host+device CD problem (pair CSR, grid) shared by g0w0 and sigma_cd.
+void setup_cd(qx_handle h, const qx_g0w0_desc* d, const double* mo_energy,
+              const double* mo_energy_ref, const double* B_ov, const double* B_pair,
+              const int* pair_target, const int* pair_m, const int* targets,
+              const double* static_terms, const double* freqs_dev, const double* wts_dev,
+              const double* wnm, HostCD& cd) {
+    const int nmo = d->nocc + d->nvir, nt = d->ntarget, npair = d->npair;
+    cd.h = h;
+    cd.nocc = d->nocc;
+    cd.nvir = d->nvir;
+    cd.naux = d->naux;
+    cd.nmo = nmo;
+    cd.nt = nt;
+    cd.npair = npair;
+    cd.nf = d->nfreq;
+    cd.B_ov = B_ov;
+    cd.eps_g_dev = mo_energy;
+    cd.B_pair = B_pair;
+    cd.eta = d->eta;
+    cd.eps_g.resize(nmo);
+    cd.eps_ref.resize(nmo);
+    d2h(h, cd.eps_g.data(), mo_energy, nmo);
+    d2h(h, cd.eps_ref.data(), mo_energy_ref ? mo_energy_ref : mo_energy, nmo);
+    cd.targets.resize(nt);
+    d2h(h, cd.targets.data(), targets, nt);
+    std::vector<int> pt(npair);
+    cd.pair_m.resize(npair);
+    d2h(h, pt.data(), pair_target, npair);
+    d2h(h, cd.pair_m.data(), pair_m, npair);
+    for (int t = 0; t < nt; ++t)
+        REQUIRE(h, cd.targets[t] >= 0 && cd.targets[t] < nmo, QX_ERR_INVALID_INPUT,
+                "target %d out of range [0,%d)", cd.targets[t], nmo);
+    cd.pair_off.assign(nt + 1, 0);
+    for (int j = 0; j < npair; ++j) {
+        REQUIRE(h, pt[j] >= 0 && pt[j] < nt, QX_ERR_INVALID_INPUT, "pair_target[%d]=%d out of range",
+                j, pt[j]);
+        REQUIRE(h, j == 0 || pt[j] >= pt[j - 1], QX_ERR_INVALID_INPUT,
+                "pairs must be sorted by target (pair %d)", j);
+        REQUIRE(h, cd.pair_m[j] >= 0 && cd.pair_m[j] < nmo, QX_ERR_INVALID_INPUT,
+                "pair_m[%d]=%d out of range", j, cd.pair_m[j]);
+        cd.pair_off[pt[j] + 1]++;
+    }
+    for (int t = 0; t < nt; ++t) cd.pair_off[t + 1] += cd.pair_off[t];
+    cd.ef = 0.5 * (cd.eps_g[d->nocc - 1] + cd.eps_g[d->nocc]);
+    int* off_dev = ws<int>(h, "pair_off", nt + 1);
+    h2d(h, off_dev, cd.pair_off.data(), nt + 1);
+    SigmaProblem& p = cd.prob;
+    p.nt = nt;
+    p.npair = npair;
+    p.nf = d->nfreq;
+    p.nocc = d->nocc;
+    p.nmo = nmo;
+    p.targets = targets;
+    p.pair_off = off_dev;
+    p.pair_m = pair_m;
+    p.wnm = wnm;
+    p.eps_g = mo_energy;
+    p.eps_ref = mo_energy_ref ? mo_energy_ref : mo_energy;
+    p.freqs = freqs_dev;
+    p.wts = wts_dev;
+    p.static_terms = static_terms;
+    p.eta = d->eta;
+    p.ef = cd.ef;
+}
+
+qx_status run_g0w0(qx_handle h, const qx_g0w0_desc* d, const double* mo_energy,
+                   const double* mo_energy_ref, const double* B_ov, const double* B_pair,
+                   const int* pair_target, const int* pair_m, const int* targets,
+                   const double* static_terms, const double* guess, double* qp_energy,
+                   double* z_factor, double* sigma_re, double* sigma_im, int* state,
+                   qx_stats* stats) {
+    const auto t0 = std::chrono::steady_clock::now();
+    validate_desc(h, d);
+    REQUIRE(h, mo_energy && B_ov && targets && qp_energy && z_factor && sigma_re && sigma_im,
+            QX_ERR_INVALID_INPUT, "required pointer is NULL");
+    REQUIRE(h, d->npair == 0 || (B_pair && pair_target && pair_m), QX_ERR_INVALID_INPUT,
+            "pair arrays are NULL");
+    const int nf = d->nfreq, nt = d->ntarget, npair = d->npair;
+    std::vector<double> fr, wt;
+    scaled_roots(nf, d->x0, fr, wt);
+    double* fr_dev = ws<double>(h, "freqs", nf);
+    double* wt_dev = ws<double>(h, "wts", nf);
+    h2d(h, fr_dev, fr.data(), nf);
+    h2d(h, wt_dev, wt.data(), nf);
+    double* zero_static = nullptr;
+    if (!static_terms) {
+        zero_static = ws<double>(h, "zero_static", nt);
+        fill_f64(h, zero_static, 0.0, nt);
+        static_terms = zero_static;
+    }
+    double* wnm = ws<double>(h, "wnm", (size_t)nf * std::max(npair, 1));
+    HostCD cd;
+    setup_cd(h, d, mo_energy, mo_energy_ref, B_ov, B_pair, pair_target, pair_m, targets,
+             static_terms, fr_dev, wt_dev, wnm, cd);
+    WnmResult wr = build_wnm(h, d->precision, d->nocc, d->nvir, d->naux, mo_energy, B_ov, nf,
+                             fr_dev, npair, B_pair, wnm);
+    const NewtonParams np = to_params(d->newton);
+    int* st_dev = ws<int>(h, "newton_state", nt);
+    int* it_dev = ws<int>(h, "newton_iters", nt);
+    {
+        Phase ph(h, QX_PHASE_NEWTON);
+        HIPCK(h, launch_newton_cd(cd.prob, np, guess, qp_energy, z_factor, sigma_re, sigma_im,
+                                  st_dev, it_dev, h->stream));
+    }
+    std::vector<int> st(nt);
+    d2h(h, st.data(), st_dev, nt);
+    int residue_targets = 0, newton_fail = 0;
+    std::vector<double> guess_h;
+    if (guess) {
+        guess_h.resize(nt);
+        d2h(h, guess_h.data(), guess, nt);
+    }
+    std::vector<double> stat_h(nt);
+    d2h(h, stat_h.data(), static_terms, nt);
+    for (int t = 0; t < nt; ++t) {
+        if (st[t] == kNewtonConverged) { st[t] = QX_QP_NEWTON; continue; }
+        if (st[t] == kNewtonBisection) { st[t] = QX_QP_BISECTION; continue; }
+        ++residue_targets;
+        double E, Z;
+        C2 s;
+        st[t] = host_solve(cd, np, t, stat_h[t], guess != nullptr, guess ? guess_h[t] : 0.0, E, Z, s);
+        if (st[t] == QX_QP_LINEARIZED) ++newton_fail;
+        h2d(h, qp_energy + t, &E, 1);
+        h2d(h, z_factor + t, &Z, 1);
+        h2d(h, sigma_re + t, &s.re, 1);
+        h2d(h, sigma_im + t, &s.im, 1);
+    }
+    if (state) h2d(h, state, st.data(), nt);
+    HIPCK(h, hipStreamSynchronize(h->stream));
+    resolve_profile(h);
+    if (stats) {
+        stats->chol_fallbacks = wr.chol_fallbacks;
+        stats->singular_fallbacks = wr.singular;
+        stats->residue_targets = residue_targets;
+        stats->residue_jobs = cd.residue_jobs;
+        stats->newton_failures = newton_fail;
+        stats->seconds =
+            std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    }
+    return QX_OK;
+}
+
+template <class F>
+qx_status guarded(qx_handle h, F fn) {
+    if (!h) return QX_ERR_INVALID_INPUT;
+    try {
+        HIPCK(h, hipSetDevice(h->device));
+        return fn();
+    } catch (const QxFail& f) {
+        return f.st;
+    } catch (const std::exception& e) {
+        set_err(h, "internal error: %s", e.what());
+        return QX_ERR_NUMERICAL;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+qx_status qx_create(qx_handle* out, int device) {
+    if (!out) return QX_ERR_INVALID_INPUT;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return QX_ERR_DEVICE;
+    if (device < 0 || device >= n) return QX_ERR_INVALID_INPUT;
+    qx_context* c = new qx_context();
+    c->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess) {
+        delete c;
+        return QX_ERR_DEVICE;
+    }
+    c->own_stream = true;
+    *out = c;
+    return QX_OK;
+}
+
+void qx_destroy(qx_handle h) {
+    if (!h) return;
+    (void)hipSetDevice(h->device);
+    (void)hipStreamSynchronize(h->stream);
+    for (auto& kv : h->bufs) (void)hipFree(kv.second.first);
+    for (auto& p : h->ev_pending) {
+        (void)hipEventDestroy(p.second.first);
+        (void)hipEventDestroy(p.second.second);
+    }
+    for (hipEvent_t e : h->ev_free) (void)hipEventDestroy(e);
+    if (h->own_stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+}
+
+qx_status qx_set_profiling(qx_handle h, int enable) {
+    return guarded(h, [&] {
+        resolve_profile(h);
+        h->prof = enable != 0;
+        for (int i = 0; i < QX_NPHASE; ++i) {
+            h->phase_ms[i] = 0.0;
+            h->phase_n[i] = 0;
+        }
+        return QX_OK;
+    });
+}
+
+qx_status qx_get_profile(qx_handle h, double* ms, long long* counts) {
+    return guarded(h, [&] {
+        resolve_profile(h);
+        for (int i = 0; i < QX_NPHASE; ++i) {
+            if (ms) ms[i] = h->phase_ms[i];
+            if (counts) counts[i] = h->phase_n[i];
+        }
+        return QX_OK;
+    });
+}
+
+const char* qx_last_error(qx_handle h) { return h ? h->err.c_str() : "null handle"; }
+
+qx_status qx_set_stream(qx_handle h, void* s) {
+    return guarded(h, [&] {
+        HIPCK(h, hipStreamSynchronize(h->stream));
+        if (h->own_stream) HIPCK(h, hipStreamDestroy(h->stream));
+        if (s) {
+            h->stream = static_cast<hipStream_t>(s);
+            h->own_stream = false;
+        } else {
+            HIPCK(h, hipStreamCreate(&h->stream));
+            h->own_stream = true;
+        }
+        return QX_OK;
+    });
+}
+
+qx_status qx_synchronize(qx_handle h) {
+    return guarded(h, [&] {
+        HIPCK(h, hipStreamSynchronize(h->stream));
+        return QX_OK;
+    });
+}
+
+qx_status qx_release_workspace(qx_handle h) {
+    return guarded(h, [&] {
+        HIPCK(h, hipStreamSynchronize(h->stream));
+        for (auto& kv : h->bufs) HIPCK(h, hipFree(kv.second.first));
+        h->bufs.clear();
+        return QX_OK;
+    });
+}
+
+void qx_newton_config_default(qx_newton_config* c) {
+    if (!c) return;
+    c->max_iterations = 50;
+    c->energy_tolerance = 1e-8;
+    c->initial_damping = 1.0;
+    c->min_damping = 0.1;
+    c->max_damping = 1.0;
+    c->max_step = 1.0;
+    c->derivative_delta = 1e-6;
+    c->use_richardson = 1;
+    c->z_min = 0.1;
+    c->z_max = 0.999;
+    c->use_bisection_fallback = 1;
+    c->max_bisection_iterations = 30;
+}
+
+void qx_g0w0_desc_default(qx_g0w0_desc* d) {
+    if (!d) return;
+    std::memset(d, 0, sizeof *d);
+    d->nfreq = 48;
+    d->x0 = 0.5;
+    d->eta = 0.01;
+    d->precision = QX_PREC_FP64;
+    qx_newton_config_default(&d->newton);
+}
+
+qx_status qx_gauss_legendre_grid(int n, double* nodes, double* weights) {
+    if (n <= 0 || !nodes || !weights) return QX_ERR_INVALID_INPUT;
+    std::vector<double> x, w;
+    gauss_legendre(n, x, w);
+    std::copy(x.begin(), x.end(), nodes);
+    std::copy(w.begin(), w.end(), weights);
+    return QX_OK;
+}
+
+qx_status qx_scaled_legendre_roots(int nw, double x0, double* freqs, double* wts) {
+    if (nw <= 0 || !freqs || !wts) return QX_ERR_INVALID_INPUT;
+    std::vector<double> f, w;
+    scaled_roots(nw, x0, f, w);
+    std::copy(f.begin(), f.end(), freqs);
+    std::copy(w.begin(), w.end(), wts);
+    return QX_OK;
+}
+
+qx_status qx_rho_response_dev(qx_handle h, int precision, int nocc, int nvir, int naux,
+                              const double* mo_energy, const double* B_ov, int nfreq,
+                              const double* freqs, double* pi_out) {
+    return guarded(h, [&] {
+        REQUIRE(h, nocc > 0 && nvir > 0 && naux > 0 && nfreq > 0, QX_ERR_INVALID_INPUT,
+                "dimensions must be positive");
+        REQUIRE(h, precision >= 0 && precision <= 2, QX_ERR_INVALID_INPUT, "unknown precision");
+        REQUIRE(h, mo_energy && B_ov && freqs && pi_out, QX_ERR_INVALID_INPUT, "NULL pointer");
+        const int nov = nocc * nvir, kpad = round_up(nov, 32), npad = round_up(naux, NB);
+        double* s = ws<double>(h, "pi_weights", (size_t)nfreq * kpad);
+        HIPCK(h, launch_pi_weights(mo_energy, nocc, nvir, nfreq, freqs, s, h->stream));
+        PiOperands ops;
+        if (precision == QX_PREC_BF16X3) {
+            uint16_t* hi = ws<uint16_t>(h, "bt_hi", (size_t)npad * kpad);
+            uint16_t* lo = ws<uint16_t>(h, "bt_lo", (size_t)npad * kpad);
+            float* f32 = ws<float>(h, "bt_f32", (size_t)npad * kpad);
+            HIPCK(h, launch_pi_prepare_mixed(B_ov, nov, naux, npad, kpad, hi, lo, f32, h->stream));
+            ops.bt_hi = hi;
+            ops.bt_lo = lo;
+            ops.bt_f32 = f32;
+            ops.kpad = kpad;
+        }
+        HIPCK(h, launch_pi_syrk(precision, B_ov, ops, s, nov, naux, naux, nfreq, pi_out,
+                                PiOut::PiFull, h->stream));
+        HIPCK(h, hipStreamSynchronize(h->stream));
+        return QX_OK;
+    });
+}
+
+qx_status qx_screened_wc_dev(qx_handle h, int naux, int nfreq, const double* pi, double* wc_out,
+                             int* fallbacks) {
+    return guarded(h, [&] {
+        REQUIRE(h, naux > 0 && nfreq > 0, QX_ERR_INVALID_INPUT, "dimensions must be positive");
+        REQUIRE(h, pi && wc_out, QX_ERR_INVALID_INPUT, "NULL pointer");
+        const int npad = round_up(naux, NB), nblk = npad / NB;
+        double* eps = ws<double>(h, "eps_mat", (size_t)nfreq * npad * npad);
+        hipLaunchKernelGGL(eps_from_pi_kernel, grid1d((size_t)nfreq * npad * npad), dim3(256), 0,
+                           h->stream, pi, naux, npad, nfreq, eps);
+        HIPCK(h, hipGetLastError());
+        double* linv = ws<double>(h, "linv_diag", (size_t)nfreq * nblk * NB * NB);
+        int* fail = ws<int>(h, "chol_fail", nfreq);
+        fill_i32(h, fail, 0, nfreq);
+        HIPCK(h, launch_potrf_batched(eps, linv, npad, nfreq, fail, h->stream));
+        std::vector<int> fh(nfreq);
+        d2h(h, fh.data(), fail, nfreq);
+        int nfail = 0;
+        for (int v : fh) nfail += v != 0;
+        if (nfail == 0) {
+            double* x = ws<double>(h, "xinv", (size_t)nfreq * npad * npad);
+            HIPCK(h, launch_trtri_batched(eps, linv, x, npad, nfreq, h->stream));
+            HIPCK(h, launch_wc_from_linv(x, wc_out, npad, naux, nfreq, h->stream));
+        } else {
+            double* work = ws<double>(h, "gj_work", (size_t)nfreq * naux * 2 * naux + nfreq);
+            HIPCK(h, launch_lu_fallback_wc(pi, 0, work, wc_out, naux, nfreq, nullptr, h->stream));
+        }
+        HIPCK(h, hipStreamSynchronize(h->stream));
+        if (fallbacks) *fallbacks = nfail;
+        return QX_OK;
+    });
+}
+
+qx_status qx_wmn_imag_dev(qx_handle h, int precision, int nocc, int nvir, int naux,
+                          const double* mo_energy, const double* B_ov, int nfreq,
+                          const double* freqs, int npair, const double* B_pair, double* wnm) {
+    return guarded(h, [&] {
+        REQUIRE(h, nocc > 0 && nvir > 0 && naux > 0 && nfreq > 0 && npair >= 0,
+                QX_ERR_INVALID_INPUT, "dimensions must be positive");
+        REQUIRE(h, precision >= 0 && precision <= 2, QX_ERR_INVALID_INPUT, "unknown precision");
+        REQUIRE(h, mo_energy && B_ov && freqs && (npair == 0 || (B_pair && wnm)),
+                QX_ERR_INVALID_INPUT, "NULL pointer");
+        build_wnm(h, precision, nocc, nvir, naux, mo_energy, B_ov, nfreq, freqs, npair, B_pair, wnm);
+        HIPCK(h, hipStreamSynchronize(h->stream));
+        return QX_OK;
+    });
+}
+
+qx_status qx_sigma_cd_dev(qx_handle h, const qx_g0w0_desc* d, const double* mo_energy,
+                          const double* mo_energy_ref, const double* B_ov, const double* B_pair,
+                          const int* pair_target, const int* pair_m, const int* targets,
+                          const double* freqs, const double* wts, const double* wnm, int nq,
+                          const double* energies, double* sigma) {
+    return guarded(h, [&] {
+        validate_desc(h, d);
+        REQUIRE(h, nq > 0 && energies && sigma && freqs && wts && targets && mo_energy,
+                QX_ERR_INVALID_INPUT, "NULL pointer or nq <= 0");
+        HostCD cd;
+        double* zero_static = ws<double>(h, "zero_static", d->ntarget);
+        fill_f64(h, zero_static, 0.0, d->ntarget);
+        setup_cd(h, d, mo_energy, mo_energy_ref, B_ov, B_pair, pair_target, pair_m, targets,
+                 zero_static, freqs, wts, wnm, cd);
+        std::vector<double> E((size_t)d->ntarget * nq);
+        d2h(h, E.data(), energies, E.size());
+        std::vector<double> out(E.size() * 2);
+        for (int t = 0; t < d->ntarget; ++t)
+            for (int q0 = 0; q0 < nq; q0 += 5) {
+                const int cnt = std::min(5, nq - q0);
+                C2 s[5];
+                cd.eval(t, &E[(size_t)t * nq + q0], cnt, s);
+                for (int q = 0; q < cnt; ++q) {
+                    out[((size_t)t * nq + q0 + q) * 2] = s[q].re;
+                    out[((size_t)t * nq + q0 + q) * 2 + 1] = s[q].im;
+                }
+            }
+        h2d(h, sigma, out.data(), out.size());
+        return QX_OK;
+    });
+}
+
+qx_status qx_g0w0_cd_dev(qx_handle h, const qx_g0w0_desc* d, const double* mo_energy,
+                         const double* mo_energy_ref, const double* B_ov, const double* B_pair,
+                         const int* pair_target, const int* pair_m, const int* targets,
+                         const double* static_terms, const double* guess, double* qp_energy,
+                         double* z_factor, double* sigma_re, double* sigma_im, int* state,
+                         qx_stats* stats) {
+    return guarded(h, [&] {
+        return run_g0w0(h, d, mo_energy, mo_energy_ref, B_ov, B_pair, pair_target, pair_m,
+                        targets, static_terms, guess, qp_energy, z_factor, sigma_re, sigma_im,
+                        state, stats);
+    });
+}
+
+qx_status qx_g0w0_cd_ov_dev(qx_handle h, const qx_g0w0_desc* d, const double* mo_energy,
+                            const double* mo_energy_ref, const double* B_ov, const int* targets,
+                            const double* static_terms, const double* guess, double* qp_energy,
+                            double* z_factor, double* sigma_re, double* sigma_im, int* state,
+                            qx_stats* stats) {
+    return guarded(h, [&] {
+        REQUIRE(h, d != nullptr && targets != nullptr, QX_ERR_INVALID_INPUT, "NULL argument");
+        const int nt = d->ntarget;
+        REQUIRE(h, nt > 0, QX_ERR_INVALID_INPUT, "ntarget must be positive");
+        std::vector<int> tg(nt), off(nt + 1, 0);
+        d2h(h, tg.data(), targets, nt);
+        for (int t = 0; t < nt; ++t) {
+            REQUIRE(h, tg[t] >= 0 && tg[t] < d->nocc + d->nvir, QX_ERR_INVALID_INPUT,
+                    "target %d out of range", tg[t]);
+            off[t + 1] = off[t] + (tg[t] < d->nocc ? d->nvir : d->nocc);
+        }
+        const int npair = off[nt];
+        int* off_dev = ws<int>(h, "ov_off", nt + 1);
+        h2d(h, off_dev, off.data(), nt + 1);
+        int* pt = ws<int>(h, "ov_pt", npair);
+        int* pm = ws<int>(h, "ov_pm", npair);
+        double* rows = ws<double>(h, "ov_rows", (size_t)npair * d->naux);
+        const int maxcnt = std::max(d->nocc, d->nvir);
+        hipLaunchKernelGGL(ov_pairs_kernel, dim3(std::min(maxcnt, 1024), nt), dim3(256), 0,
+                           h->stream, B_ov, d->nocc, d->nvir, d->naux, targets, off_dev, nt, pt,
+                           pm, rows);
+        HIPCK(h, hipGetLastError());
+        qx_g0w0_desc dd = *d;
+        dd.npair = npair;
+        return run_g0w0(h, &dd, mo_energy, mo_energy_ref, B_ov, rows, pt, pm, targets,
+                        static_terms, guess, qp_energy, z_factor, sigma_re, sigma_im, state,
+                        stats);
+    });
+}
+
+qx_status qx_g0w0_cd(qx_handle h, const qx_g0w0_desc* d, const double* mo_energy,
+                     const double* mo_energy_ref, const double* B_ov, const double* B_pair,
+                     const int* pair_target, const int* pair_m, const int* targets,
+                     const double* static_terms, const double* guess, double* qp_energy,
+                     double* z_factor, double* sigma_re, double* sigma_im, int* state,
+                     qx_stats* stats) {
+    return guarded(h, [&] {
+        validate_desc(h, d);
+        const size_t nmo = d->nocc + d->nvir, nov = (size_t)d->nocc * d->nvir;
+        const size_t nt = d->ntarget, np = d->npair;
+        std::vector<void*> owned;
+        auto up = [&](const void* src, size_t bytes) -> void* {
+            if (!src) return nullptr;
+            void* p = nullptr;
+            if (hipMalloc(&p, std::max<size_t>(bytes, 8)) != hipSuccess) {
+                set_err(h, "device allocation of %zu bytes failed", bytes);
+                throw QxFail{QX_ERR_ALLOC};
+            }
+            owned.push_back(p);
+            HIPCK(h, hipMemcpy(p, src, bytes, hipMemcpyHostToDevice));
+            return p;
+        };
+        auto down = [&](size_t bytes) -> void* {
+            void* p = nullptr;
+            if (hipMalloc(&p, std::max<size_t>(bytes, 8)) != hipSuccess) {
+                set_err(h, "device allocation of %zu bytes failed", bytes);
+                throw QxFail{QX_ERR_ALLOC};
+            }
+            owned.push_back(p);
+            return p;
+        };
+        qx_status rc = QX_OK;
+        try {
+            auto* e = (double*)up(mo_energy, nmo * 8);
+            auto* er = (double*)up(mo_energy_ref, nmo * 8);
+            auto* b = (double*)up(B_ov, nov * d->naux * 8);
+            auto* bp = (double*)up(B_pair, np * d->naux * 8);
+            auto* pt = (int*)up(pair_target, np * 4);
+            auto* pm = (int*)up(pair_m, np * 4);
+            auto* tg = (int*)up(targets, nt * 4);
+            auto* sx = (double*)up(static_terms, nt * 8);
+            auto* gs = (double*)up(guess, nt * 8);
+            auto* oe = (double*)down(nt * 8);
+            auto* oz = (double*)down(nt * 8);
+            auto* osr = (double*)down(nt * 8);
+            auto* osi = (double*)down(nt * 8);
+            auto* ost = (int*)down(nt * 4);
+            rc = run_g0w0(h, d, e, er, b, bp, pt, pm, tg, sx, gs, oe, oz, osr, osi, ost, stats);
+            if (rc == QX_OK) {
+                HIPCK(h, hipMemcpy(qp_energy, oe, nt * 8, hipMemcpyDeviceToHost));
+                HIPCK(h, hipMemcpy(z_factor, oz, nt * 8, hipMemcpyDeviceToHost));
+                HIPCK(h, hipMemcpy(sigma_re, osr, nt * 8, hipMemcpyDeviceToHost));
+                HIPCK(h, hipMemcpy(sigma_im, osi, nt * 8, hipMemcpyDeviceToHost));
+                if (state) HIPCK(h, hipMemcpy(state, ost, nt * 4, hipMemcpyDeviceToHost));
+            }
+        } catch (...) {
+            (void)hipStreamSynchronize(h->stream);
+            for (void* p : owned) (void)hipFree(p);
+            throw;
+        }
+        (void)hipStreamSynchronize(h->stream);
+        for (void* p : owned) (void)hipFree(p);
+        return rc;
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,120 @@
+// Internal declarations shared by the qxgw HIP sources (not part of the C ABI).
+//
+// Device data layout (all row-major, FP64 unless stated):
+//   eps        [nmo]                      orbital energies
+//   B_ov       [N_ov][naux]               DF tensor, ia rows (reference iaP layout)
+//   eps_mat    [nf][npad][npad]           dielectric matrices eps = I - Pi, then L (lower)
+//   xinv       [nf][npad][npad]           L^-1 (lower)
+//   linv_diag  [nf][nblk][NB][NB]         inverses of the diagonal NB x NB blocks of L
+//   B_pairs    [npair][npad]              target-partner rows b_nm (zero padded)
+//   wnm        [nf][npair]                Wc_nm(i w_f) = b_nm^T Wc(i w_f) b_nm
+// npad = naux rounded up to a multiple of NB; the padding block of eps is the identity.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstddef>
+
+namespace qx {
+
+constexpr int NB = 64;          // Cholesky / trtri block size
+constexpr int kMaxTargetsPerLaunch = 1 << 20;
+
+inline int round_up(int x, int m) { return (x + m - 1) / m * m; }
+inline int ceil_div(int x, int m) { return (x + m - 1) / m; }
+
+// ---- Polarizability (pi_kernels.hip) ---------------------------------------------------
+// Pi is built from the frequency weights s_f(r) = 4 e_r / (w_f^2 + e_r^2),
+// e_r = eps_i - eps_a for row r = i*nvir + a.
+enum class PiOut { EpsLower = 0, PiFull = 1 };
+
+struct PiOperands {              // per-solve operand copies for the mixed kernels
+    const uint16_t* bt_hi = nullptr;   // [npad][kpad] bf16 high part of B^T
+    const uint16_t* bt_lo = nullptr;   // [npad][kpad] bf16 low part of B^T
+    const float*    bt_f32 = nullptr;  // [npad][kpad] B^T in FP32
+    int kpad = 0;                      // N_ov rounded up to 32
+};
+
+// s[f][r] table; eia from eps.  s has layout [nf][nov].
+hipError_t launch_pi_weights(const double* eps, int nocc, int nvir, int nf,
+                             const double* freqs, double* s, hipStream_t st);
+// B^T split copies for the mixed kernels.
+hipError_t launch_pi_prepare_mixed(const double* B, int nov, int naux, int npad, int kpad,
+                                   uint16_t* bt_hi, uint16_t* bt_lo, float* bt_f32,
+                                   hipStream_t st);
+// precision: 0 = FP64 MFMA, 1 = FP32 MFMA, 2 = BF16x3 MFMA (FP32 accumulation)
+hipError_t launch_pi_syrk(int precision, const double* B, const PiOperands& ops,
+                          const double* s, int nov, int naux, int npad, int nf,
+                          double* out, PiOut mode, hipStream_t st);
+
+// ---- Factorisation (chol_kernels.hip) --------------------------------------------------
+hipError_t launch_potrf_batched(double* a, double* linv_diag, int npad, int nf,
+                                int* fail, hipStream_t st);
+hipError_t launch_trtri_batched(const double* l, const double* linv_diag, double* x,
+                                int npad, int nf, hipStream_t st);
+// eps^-1 - I from X = L^-1 :  W = X^T X - I  (full symmetric)
+hipError_t launch_wc_from_linv(const double* x, double* wc, int npad, int naux, int nf,
+                               hipStream_t st);
+// Dense fallback: Gauss-Jordan with partial pivoting, one workgroup per matrix, for the
+// frequencies whose Cholesky failed (fail[f] != 0).  Writes (I - Pi)^-1 - I into wc or,
+// if singular, zero (the identity fallback of correlation_ac.rs:396).
+hipError_t launch_lu_fallback_wc(const double* pi_full_or_eps, int is_eps, double* work,
+                                 double* wc, int npad, int nf, const int* fail,
+                                 hipStream_t st);
+
+// ---- Contractions and self-energy (sigma_kernels.hip) ----------------------------------
+// wnm[f][j] = 2 b_j.(M b_j) + |M b_j|^2,  M = X - I  (precision as for Pi)
+hipError_t launch_wnm(int precision, const double* bpairs, const double* x, int npair,
+                      int npad, int nf, double* wpart, double* wnm, hipStream_t st);
+// wnm from an explicit Wc: wnm[f][j] = b_j^T Wc_f b_j  (FP64)
+hipError_t launch_wnm_from_wc(const double* bpairs, const double* wc, int npair, int npad,
+                              int nf, double* wpart, double* wnm, hipStream_t st);
+
+struct NewtonParams {
+    int max_iterations;
+    double energy_tolerance;
+    double initial_damping, min_damping, max_damping, max_step, derivative_delta;
+    double z_min, z_max;
+    int use_richardson, use_bisection, max_bisection_iterations;
+};
+
+struct SigmaProblem {            // everything the CD quadrature needs (device pointers)
+    int nt, npair, nf, nocc, nmo;
+    const int* targets;          // [nt] orbital index
+    const int* pair_off;         // [nt+1] CSR offsets into pairs (pairs sorted by target)
+    const int* pair_m;           // [npair]
+    const double* wnm;           // [nf][npair]
+    const double* eps_g;         // [nmo] energies of the Green's function, poles and ef
+    const double* eps_ref;       // [nmo] mean-field energies (QP equation, eta sign)
+    const double* freqs;         // [nf]
+    const double* wts;           // [nf]
+    const double* static_terms;  // [nt] Sigma_x - V_xc
+    double eta;
+    double ef;
+};
+
+// Status codes written per target by the Newton kernel.
+enum NewtonState : int {
+    kNewtonConverged = 0, kNewtonBisection = 1, kNewtonFailed = 2, kNewtonNeedsResidue = 3
+};
+
+// Device Newton loop (one wavefront per target), imaginary-axis part only; targets whose
+// iterates reach a residue pole stop with kNewtonNeedsResidue.
+hipError_t launch_newton_cd(const SigmaProblem& p, const NewtonParams& np,
+                            const double* guess,        // nullable: linearised guess
+                            double* e_out, double* z_out, double* sig_re, double* sig_im,
+                            int* state, int* iters, hipStream_t st);
+// I_n(E) at nE energies per target: E[t][q] -> out[t][q] (complex, interleaved re/im)
+hipError_t launch_sigma_imag_eval(const SigmaProblem& p, const double* energies, int nq,
+                                  double* out, hipStream_t st);
+
+// Residues: for job q, W = b^T[(I - Pi(z_q))^-1 - I] b, z_q = omega_q + i eta.
+// Builds complex Pi(z_q) (FP64), solves with a one-workgroup complex LU with partial
+// pivoting, and also returns dW/domega.
+hipError_t launch_residue_jobs(const double* B, const double* eps, int nocc, int nvir,
+                               int naux, int njobs, const double* omegas, double eta,
+                               const double* brows,      // [njobs][naux]
+                               double* work,             // [njobs][naux][2*naux+2] complex
+                               double* w_out,            // [njobs][2] re, im
+                               hipStream_t st);
+
+}  // namespace qx

@@ -1,0 +1,476 @@
+// Screened-interaction contraction, CD self-energy quadrature, residues and the Newton
+// quasiparticle solve.
+//
+// Reference:
+//   * Wc_nm(i w) = b_nm^T Wc(i w) b_nm: quasix_core/src/selfenergy/correlation_ac.rs:404-421
+//     (per-target DGEMM q_nm = Wc^T L[:,n,:], column dot with L[:,:,n]).  Here the pairs of
+//     all targets form one matrix Bp [npair][naux] and, with eps = L L^T and M = L^-1 - I,
+//         Wc_nm = |L^-1 b|^2 - |b|^2 = 2 b.(M b) + |M b|^2,
+//     one triangular GEMM with the row dots fused into its epilogue (no cancellation: the
+//     product M b is small exactly when Wc is small).
+//   * Imaginary-axis quadrature with the eta-shifted Green's function:
+//     correlation_pyscf_cd.rs:209-226 (emo = E - eps_m - i eta sign(eps_n - eps_m),
+//     g = w_k emo / (emo^2 + w_k^2), Sigma = -(1/pi) sum W g).
+//   * Residues Wc_nm(|E - eps_m| + i eta) for the poles between the Fermi level and E
+//     (contour deformation; the reference's CDIntegrator is unwired).
+//   * Newton: quasix_core/src/qp/newton_solver.rs:174-391 (Richardson 5-point derivative,
+//     adaptive damping, step clip, bisection fallback); iteration-0 linearised guess
+//     quasix_core/src/qp/evgw.rs:730-741.
+#include "qx_internal.h"
+#include "qx_newton.h"
+
+namespace qx {
+namespace {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+constexpr int P = NB + 1;
+constexpr double kPi = 3.14159265358979323846;
+
+__device__ __forceinline__ d4 mfma64(double a, double b, d4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+// ---- Wc_nm contraction -----------------------------------------------------------------
+// grid (ceil(npair/64), nblk, nf): tile of 64 pairs x 64 auxiliary columns (block pb).
+__global__ __launch_bounds__(256) void wnm_linv_kernel(const double* __restrict__ bp,
+                                                       const double* __restrict__ x, int npair,
+                                                       int npad, double* __restrict__ wpart) {
+    __shared__ double Bs[NB][P];
+    __shared__ double Ms[NB][P];
+    const int j0 = blockIdx.x * NB, pb = blockIdx.y, f = blockIdx.z;
+    const int nblk = gridDim.y;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const double* X = x + (size_t)f * npad * npad;
+    d4 acc[4] = {d4{0, 0, 0, 0}, d4{0, 0, 0, 0}, d4{0, 0, 0, 0}, d4{0, 0, 0, 0}};
+    for (int qb = 0; qb <= pb; ++qb) {
+        for (int idx = tid; idx < NB * NB; idx += 256) {
+            const int r = idx / NB, c = idx % NB;
+            const int j = j0 + r;
+            Bs[r][c] = j < npair ? bp[(size_t)j * npad + qb * NB + c] : 0.0;
+            double m = X[(size_t)(pb * NB + r) * npad + qb * NB + c];
+            if (qb == pb && r == c) m -= 1.0;
+            Ms[r][c] = m;
+        }
+        __syncthreads();
+        const int r = wave * 16 + (lane & 15);
+        #pragma unroll
+        for (int kk = 0; kk < NB; kk += 4) {
+            const int q = kk + (lane >> 4);
+            const double a = Bs[r][q];
+            #pragma unroll
+            for (int bj = 0; bj < 4; ++bj) acc[bj] = mfma64(a, Ms[bj * 16 + (lane & 15)][q], acc[bj]);
+        }
+        __syncthreads();
+    }
+    // epilogue: b[j][P] for P in block pb
+    for (int idx = tid; idx < NB * NB; idx += 256) {
+        const int r = idx / NB, c = idx % NB;
+        const int j = j0 + r;
+        Bs[r][c] = j < npair ? bp[(size_t)j * npad + pb * NB + c] : 0.0;
+    }
+    __syncthreads();
+    double rs[4] = {0, 0, 0, 0};
+    #pragma unroll
+    for (int bj = 0; bj < 4; ++bj)
+        #pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int r = wave * 16 + 4 * (lane >> 4) + v, c = bj * 16 + (lane & 15);
+            const double y = acc[bj][v];
+            rs[v] += 2.0 * Bs[r][c] * y + y * y;
+        }
+    #pragma unroll
+    for (int v = 0; v < 4; ++v)
+        for (int off = 1; off < 16; off <<= 1) rs[v] += __shfl_xor(rs[v], off, 64);
+    if ((lane & 15) == 0)
+        #pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int j = j0 + wave * 16 + 4 * (lane >> 4) + v;
+            if (j < npair) wpart[((size_t)f * nblk + pb) * npair + j] = rs[v];
+        }
+}
+
+__global__ void wnm_reduce_kernel(const double* __restrict__ wpart, int npair, int nparts,
+                                  double* __restrict__ wnm) {
+    const int f = blockIdx.y;
+    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < npair; j += gridDim.x * blockDim.x) {
+        double s = 0.0;
+        for (int b = 0; b < nparts; ++b) s += wpart[((size_t)f * nparts + b) * npair + j];
+        wnm[(size_t)f * npair + j] = s;
+    }
+}
+
+// wnm from an explicit Wc [nf][npad][npad]: b^T Wc b, tile 64 pairs x 64 cols.
+__global__ __launch_bounds__(256) void wnm_wc_kernel(const double* __restrict__ bp,
+                                                     const double* __restrict__ wc, int npair,
+                                                     int npad, double* __restrict__ wpart) {
+    __shared__ double Bs[NB][P];
+    __shared__ double Ws[NB][P];
+    const int j0 = blockIdx.x * NB, pb = blockIdx.y, f = blockIdx.z;
+    const int nblk = gridDim.y;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const double* W = wc + (size_t)f * npad * npad;
+    d4 acc[4] = {d4{0, 0, 0, 0}, d4{0, 0, 0, 0}, d4{0, 0, 0, 0}, d4{0, 0, 0, 0}};
+    for (int qb = 0; qb < nblk; ++qb) {
+        for (int idx = tid; idx < NB * NB; idx += 256) {
+            const int r = idx / NB, c = idx % NB;
+            const int j = j0 + r;
+            Bs[r][c] = j < npair ? bp[(size_t)j * npad + qb * NB + c] : 0.0;
+            Ws[r][c] = W[(size_t)(pb * NB + r) * npad + qb * NB + c];
+        }
+        __syncthreads();
+        const int r = wave * 16 + (lane & 15);
+        #pragma unroll
+        for (int kk = 0; kk < NB; kk += 4) {
+            const int q = kk + (lane >> 4);
+            const double a = Bs[r][q];
+            #pragma unroll
+            for (int bj = 0; bj < 4; ++bj) acc[bj] = mfma64(a, Ws[bj * 16 + (lane & 15)][q], acc[bj]);
+        }
+        __syncthreads();
+    }
+    for (int idx = tid; idx < NB * NB; idx += 256) {
+        const int r = idx / NB, c = idx % NB;
+        const int j = j0 + r;
+        Bs[r][c] = j < npair ? bp[(size_t)j * npad + pb * NB + c] : 0.0;
+    }
+    __syncthreads();
+    double rs[4] = {0, 0, 0, 0};
+    #pragma unroll
+    for (int bj = 0; bj < 4; ++bj)
+        #pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int r = wave * 16 + 4 * (lane >> 4) + v, c = bj * 16 + (lane & 15);
+            rs[v] += Bs[r][c] * acc[bj][v];
+        }
+    #pragma unroll
+    for (int v = 0; v < 4; ++v)
+        for (int off = 1; off < 16; off <<= 1) rs[v] += __shfl_xor(rs[v], off, 64);
+    if ((lane & 15) == 0)
+        #pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int j = j0 + wave * 16 + 4 * (lane >> 4) + v;
+            if (j < npair) wpart[((size_t)f * nblk + pb) * npair + j] = rs[v];
+        }
+}
+
+// ---- CD quadrature ------------------------------------------------------------------------
+
+__device__ __forceinline__ double wave_sum(double v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// I_n(E_q) for nq energies of target t (whole wavefront); returns 1 if any E_q has a
+// residue pole among the target's partners.
+template <int NQ>
+__device__ int eval_imag(const SigmaProblem& p, int t, const double (&E)[NQ], int nq,
+                         C2 (&out)[NQ]) {
+    const int lane = threadIdx.x & 63;
+    const int n = p.targets[t];
+    const double en_ref = p.eps_ref[n];
+    double sre[NQ], sim[NQ];
+    #pragma unroll
+    for (int q = 0; q < NQ; ++q) { sre[q] = 0.0; sim[q] = 0.0; }
+    int pole = 0;
+    for (int j = p.pair_off[t] + lane; j < p.pair_off[t + 1]; j += 64) {
+        const int m = p.pair_m[j];
+        const double em = p.eps_g[m];
+        const double c = -p.eta * (en_ref > p.eps_ref[m] ? 1.0 : -1.0);
+        #pragma unroll
+        for (int q = 0; q < nq; ++q) {
+            const double Eq = E[q];
+            if ((Eq > p.ef && p.ef < em && em < Eq) || (Eq <= p.ef && Eq < em && em < p.ef))
+                pole = 1;
+        }
+        for (int k = 0; k < p.nf; ++k) {
+            const double w = p.wnm[(size_t)k * p.npair + j];
+            const double wk = p.wts[k], w2 = p.freqs[k] * p.freqs[k];
+            #pragma unroll
+            for (int q = 0; q < nq; ++q) {
+                const double a = E[q] - em;
+                // emo = a + i c ; den = emo^2 + w2 ; g = wk emo / den
+                const double dre = a * a - c * c + w2, dim = 2.0 * a * c;
+                const double inv = wk / (dre * dre + dim * dim);
+                const double gre = (a * dre + c * dim) * inv;
+                const double gim = (c * dre - a * dim) * inv;
+                sre[q] += w * gre;
+                sim[q] += w * gim;
+            }
+        }
+    }
+    #pragma unroll
+    for (int q = 0; q < nq; ++q) {
+        out[q].re = -wave_sum(sre[q]) / kPi;
+        out[q].im = -wave_sum(sim[q]) / kPi;
+    }
+    for (int off = 32; off > 0; off >>= 1) pole |= __shfl_xor(pole, off, 64);
+    return pole;
+}
+
+__global__ __launch_bounds__(64) void sigma_imag_eval_kernel(SigmaProblem p,
+                                                             const double* __restrict__ energies,
+                                                             int nq, double* __restrict__ out) {
+    const int t = blockIdx.x;
+    for (int q0 = 0; q0 < nq; q0 += 5) {
+        double E[5];
+        const int cnt = nq - q0 < 5 ? nq - q0 : 5;
+        #pragma unroll
+        for (int q = 0; q < 5; ++q) E[q] = q < cnt ? energies[(size_t)t * nq + q0 + q] : 0.0;
+        C2 s[5];
+        eval_imag<5>(p, t, E, cnt, s);
+        if ((threadIdx.x & 63) == 0)
+            #pragma unroll
+            for (int q = 0; q < cnt; ++q) {
+                out[((size_t)t * nq + q0 + q) * 2] = s[q].re;
+                out[((size_t)t * nq + q0 + q) * 2 + 1] = s[q].im;
+            }
+    }
+}
+
+struct DeviceEval {
+    const SigmaProblem* p;
+    int t;
+    __device__ bool operator()(const double* E, int nq, C2* out) {
+        double Ev[5];
+        C2 s[5];
+        #pragma unroll
+        for (int q = 0; q < 5; ++q) Ev[q] = q < nq ? E[q] : 0.0;
+        const int pole = eval_imag<5>(*p, t, Ev, nq, s);
+        #pragma unroll
+        for (int q = 0; q < nq; ++q) out[q] = s[q];
+        return pole != 0;
+    }
+};
+
+__global__ __launch_bounds__(64) void newton_cd_kernel(SigmaProblem p, NewtonParams np,
+                                                       const double* __restrict__ guess,
+                                                       double* e_out, double* z_out,
+                                                       double* sig_re, double* sig_im,
+                                                       int* state, int* iters) {
+    const int t = blockIdx.x;
+    const int n = p.targets[t];
+    DeviceEval ev{&p, t};
+    double E = 0.0, Z = 0.0;
+    C2 s = {0.0, 0.0};
+    int it = 0;
+    const int stt = newton_core(ev, np, p.eps_ref[n], p.static_terms[t], p.ef, guess != nullptr,
+                                guess ? guess[t] : 0.0, E, Z, s, it);
+    if ((threadIdx.x & 63) == 0) {
+        e_out[t] = E;
+        z_out[t] = Z;
+        sig_re[t] = s.re;
+        sig_im[t] = s.im;
+        state[t] = stt;
+        iters[t] = it;
+    }
+}
+
+// ---- Residue jobs ---------------------------------------------------------------------------
+// eps(z) = I - Pi(z),  Pi(z) = 2 sum_r B_r B_r^T [1/(z + e_r) - 1/(z - e_r)],  z = w + i eta.
+// work per job: [naux][naux+1] complex (augmented with b).
+__global__ __launch_bounds__(256) void residue_build_kernel(
+    const double* __restrict__ B, const double* __restrict__ eps, int nocc, int nvir, int naux,
+    const double* __restrict__ omegas, double eta, const double* __restrict__ brows,
+    double* __restrict__ work) {
+    __shared__ double Bp[16][17];
+    __shared__ double Bq[16][17];
+    __shared__ double dre[16], dim[16];
+    const int job = blockIdx.z;
+    const int p0 = blockIdx.y * 16, q0 = blockIdx.x * 16;
+    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+    const int nov = nocc * nvir;
+    const double zr = omegas[job], zi = eta;
+    double are = 0.0, aim = 0.0;
+    for (int r0 = 0; r0 < nov; r0 += 16) {
+        const int r = r0 + ty;
+        Bp[ty][tx] = (r < nov && p0 + tx < naux) ? B[(size_t)r * naux + p0 + tx] : 0.0;
+        Bq[ty][tx] = (r < nov && q0 + tx < naux) ? B[(size_t)r * naux + q0 + tx] : 0.0;
+        if (threadIdx.x < 16) {
+            const int rr = r0 + threadIdx.x;
+            double d_re = 0.0, d_im = 0.0;
+            if (rr < nov) {
+                const double e = eps[rr / nvir] - eps[nocc + rr % nvir];
+                // 1/(z+e) - 1/(z-e)
+                const double ar = zr + e, br = zr - e;
+                const double i1 = 1.0 / (ar * ar + zi * zi), i2 = 1.0 / (br * br + zi * zi);
+                d_re = 2.0 * (ar * i1 - br * i2);
+                d_im = 2.0 * (-zi * i1 + zi * i2);
+            }
+            dre[threadIdx.x] = d_re;
+            dim[threadIdx.x] = d_im;
+        }
+        __syncthreads();
+        for (int k = 0; k < 16; ++k) {
+            const double bb = Bp[k][ty] * Bq[k][tx];
+            are += dre[k] * bb;
+            aim += dim[k] * bb;
+        }
+        __syncthreads();
+    }
+    const int pp = p0 + ty, qq = q0 + tx;
+    if (pp < naux && qq < naux) {
+        double* W = work + (size_t)job * naux * (naux + 1) * 2;
+        const size_t o = ((size_t)pp * (naux + 1) + qq) * 2;
+        W[o] = (pp == qq ? 1.0 : 0.0) - are;
+        W[o + 1] = -aim;
+        if (qq == 0) {
+            const size_t ob = ((size_t)pp * (naux + 1) + naux) * 2;
+            W[ob] = brows[(size_t)job * naux + pp];
+            W[ob + 1] = 0.0;
+        }
+    }
+}
+
+// Gaussian elimination with partial pivoting on [eps | b], then back substitution;
+// w_out[job] = b^T x - b^T b (complex).
+__global__ __launch_bounds__(1024) void residue_solve_kernel(int naux,
+                                                             const double* __restrict__ brows,
+                                                             double* __restrict__ work,
+                                                             double* __restrict__ w_out) {
+    const int job = blockIdx.x, tid = threadIdx.x, nt = blockDim.x;
+    const int n = naux, ld = naux + 1;
+    double* W = work + (size_t)job * n * ld * 2;
+    __shared__ double rv[1024];
+    __shared__ int ri[1024];
+    __shared__ double red_re[1024], red_im[1024];
+    for (int k = 0; k < n; ++k) {
+        double best = -1.0;
+        int bi = k;
+        for (int r = k + tid; r < n; r += nt) {
+            const size_t o = ((size_t)r * ld + k) * 2;
+            const double v = W[o] * W[o] + W[o + 1] * W[o + 1];
+            if (v > best) { best = v; bi = r; }
+        }
+        rv[tid] = best;
+        ri[tid] = bi;
+        __syncthreads();
+        for (int s = nt / 2; s > 0; s >>= 1) {
+            if (tid < s && (rv[tid + s] > rv[tid] || (rv[tid + s] == rv[tid] && ri[tid + s] < ri[tid]))) {
+                rv[tid] = rv[tid + s];
+                ri[tid] = ri[tid + s];
+            }
+            __syncthreads();
+        }
+        const int piv = ri[0];
+        __syncthreads();
+        if (piv != k)
+            for (int c = k + tid; c < ld; c += nt) {
+                const size_t a = ((size_t)k * ld + c) * 2, b = ((size_t)piv * ld + c) * 2;
+                const double t0 = W[a], t1 = W[a + 1];
+                W[a] = W[b]; W[a + 1] = W[b + 1];
+                W[b] = t0; W[b + 1] = t1;
+            }
+        __syncthreads();
+        const size_t okk = ((size_t)k * ld + k) * 2;
+        const double pr = W[okk], pi = W[okk + 1];
+        const double pd = pr * pr + pi * pi;
+        const double ir = pd > 0.0 ? pr / pd : 0.0, ii = pd > 0.0 ? -pi / pd : 0.0;
+        // eliminate rows below: W[r][c] -= (W[r][k]/W[k][k]) W[k][c]
+        const int rows = n - k - 1, cols = ld - k - 1;
+        for (int idx = tid; idx < rows * cols; idx += nt) {
+            const int r = k + 1 + idx / cols, c = k + 1 + idx % cols;
+            const size_t ork = ((size_t)r * ld + k) * 2, okc = ((size_t)k * ld + c) * 2;
+            const double lr = W[ork] * ir - W[ork + 1] * ii, li = W[ork] * ii + W[ork + 1] * ir;
+            const size_t orc = ((size_t)r * ld + c) * 2;
+            W[orc] -= lr * W[okc] - li * W[okc + 1];
+            W[orc + 1] -= lr * W[okc + 1] + li * W[okc];
+        }
+        __syncthreads();
+    }
+    // back substitution (serial over rows, parallel dot products)
+    for (int k = n - 1; k >= 0; --k) {
+        double sre = 0.0, sim = 0.0;
+        for (int c = k + 1 + tid; c < n; c += nt) {
+            const size_t okc = ((size_t)k * ld + c) * 2, oc = ((size_t)c * ld + n) * 2;
+            sre += W[okc] * W[oc] - W[okc + 1] * W[oc + 1];
+            sim += W[okc] * W[oc + 1] + W[okc + 1] * W[oc];
+        }
+        red_re[tid] = sre;
+        red_im[tid] = sim;
+        __syncthreads();
+        for (int s = nt / 2; s > 0; s >>= 1) {
+            if (tid < s) { red_re[tid] += red_re[tid + s]; red_im[tid] += red_im[tid + s]; }
+            __syncthreads();
+        }
+        if (tid == 0) {
+            const size_t ob = ((size_t)k * ld + n) * 2, okk = ((size_t)k * ld + k) * 2;
+            const double yr = W[ob] - red_re[0], yi = W[ob + 1] - red_im[0];
+            const double pr = W[okk], pi = W[okk + 1], pd = pr * pr + pi * pi;
+            W[ob] = (yr * pr + yi * pi) / pd;
+            W[ob + 1] = (yi * pr - yr * pi) / pd;
+        }
+        __syncthreads();
+    }
+    double sre = 0.0, sim = 0.0, sbb = 0.0;
+    for (int r = tid; r < n; r += nt) {
+        const double b = brows[(size_t)job * n + r];
+        const size_t o = ((size_t)r * ld + n) * 2;
+        sre += b * W[o];
+        sim += b * W[o + 1];
+        sbb += b * b;
+    }
+    red_re[tid] = sre - sbb;
+    red_im[tid] = sim;
+    __syncthreads();
+    for (int s = nt / 2; s > 0; s >>= 1) {
+        if (tid < s) { red_re[tid] += red_re[tid + s]; red_im[tid] += red_im[tid + s]; }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        w_out[(size_t)job * 2] = red_re[0];
+        w_out[(size_t)job * 2 + 1] = red_im[0];
+    }
+}
+
+}  // namespace
+
+hipError_t launch_wnm(int precision, const double* bpairs, const double* x, int npair, int npad,
+                      int nf, double* wpart, double* wnm, hipStream_t st) {
+    (void)precision;
+    const int nblk = npad / NB;
+    dim3 grid(ceil_div(npair, NB), nblk, nf);
+    hipLaunchKernelGGL(wnm_linv_kernel, grid, dim3(256), 0, st, bpairs, x, npair, npad, wpart);
+    dim3 g2(ceil_div(npair, 256), nf);
+    hipLaunchKernelGGL(wnm_reduce_kernel, g2, dim3(256), 0, st, wpart, npair, nblk, wnm);
+    return hipGetLastError();
+}
+
+hipError_t launch_wnm_from_wc(const double* bpairs, const double* wc, int npair, int npad, int nf,
+                              double* wpart, double* wnm, hipStream_t st) {
+    const int nblk = npad / NB;
+    dim3 grid(ceil_div(npair, NB), nblk, nf);
+    hipLaunchKernelGGL(wnm_wc_kernel, grid, dim3(256), 0, st, bpairs, wc, npair, npad, wpart);
+    dim3 g2(ceil_div(npair, 256), nf);
+    hipLaunchKernelGGL(wnm_reduce_kernel, g2, dim3(256), 0, st, wpart, npair, nblk, wnm);
+    return hipGetLastError();
+}
+
+hipError_t launch_newton_cd(const SigmaProblem& p, const NewtonParams& np, const double* guess,
+                            double* e_out, double* z_out, double* sig_re, double* sig_im,
+                            int* state, int* iters, hipStream_t st) {
+    if (p.nt <= 0) return hipSuccess;
+    hipLaunchKernelGGL(newton_cd_kernel, dim3(p.nt), dim3(64), 0, st, p, np, guess, e_out, z_out,
+                       sig_re, sig_im, state, iters);
+    return hipGetLastError();
+}
+
+hipError_t launch_sigma_imag_eval(const SigmaProblem& p, const double* energies, int nq,
+                                  double* out, hipStream_t st) {
+    if (p.nt <= 0 || nq <= 0) return hipSuccess;
+    hipLaunchKernelGGL(sigma_imag_eval_kernel, dim3(p.nt), dim3(64), 0, st, p, energies, nq, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_residue_jobs(const double* B, const double* eps, int nocc, int nvir, int naux,
+                               int njobs, const double* omegas, double eta, const double* brows,
+                               double* work, double* w_out, hipStream_t st) {
+    if (njobs <= 0) return hipSuccess;
+    dim3 g1(ceil_div(naux, 16), ceil_div(naux, 16), njobs);
+    hipLaunchKernelGGL(residue_build_kernel, g1, dim3(256), 0, st, B, eps, nocc, nvir, naux,
+                       omegas, eta, brows, work);
+    hipLaunchKernelGGL(residue_solve_kernel, dim3(njobs), dim3(1024), 0, st, naux, brows, work,
+                       w_out);
+    return hipGetLastError();
+}
+
+}  // namespace qx

@@ -1,0 +1,336 @@
+"""G0W0 / evGW drivers on the MI355X path (the ``quasix.gw`` / ``quasix.quasix.evgw`` surface).
+
+Every number comes from libqxgw.so (HIP kernels behind the C ABI in include/qxgw.h).
+torch provides device buffers and the stream; no arithmetic of the path runs in Python.
+
+Entry points
+  * ``g0w0(...)``         one G0W0 solve (Pi -> Wc -> CD self-energy -> Newton) on a device
+                          tensor set; the MI355X counterpart of the iteration-0 branch of
+                          quasix_core/src/qp/evgw.rs:603-783.
+  * ``evgw(...)``         eigenvalue self-consistency, same arguments, defaults and result
+                          keys as quasix.quasix.evgw (quasix/src/evgw.rs:22-453).
+  * ``run_evgw(config)``  dict front-end of quasix.quasix.run_evgw (quasix/src/evgw.rs:456-677).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from . import _lib
+
+HARTREE_TO_EV = 27.211386245988
+
+
+def _dev(device):
+    if not torch.cuda.is_available():
+        raise RuntimeError("quasix_amd needs a ROCm GPU (torch.cuda.is_available() is False)")
+    return torch.device("cuda", device)
+
+
+def _ptr(t):
+    return C.c_void_p(t.data_ptr()) if t is not None else None
+
+
+def _f64(x, dev):
+    return torch.as_tensor(np.ascontiguousarray(x, dtype=np.float64) if not torch.is_tensor(x)
+                           else x, dtype=torch.float64, device=dev).contiguous()
+
+
+def _i32(x, dev):
+    return torch.as_tensor(np.ascontiguousarray(x, dtype=np.int32) if not torch.is_tensor(x)
+                           else x, dtype=torch.int32, device=dev).contiguous()
+
+
+def frontier_targets(nocc: int, nmo: int, below: int = 5, above: int = 5) -> np.ndarray:
+    """HOMO-below .. LUMO+above, clipped to the orbital range."""
+    return np.arange(max(0, nocc - 1 - below), min(nmo, nocc + above + 1), dtype=np.int32)
+
+
+def make_desc(nocc, nvir, naux, nfreq=48, x0=0.5, eta=0.01, precision="fp64", ntarget=1,
+              npair=0, **newton) -> _lib.G0W0Desc:
+    d = _lib.G0W0Desc()
+    _lib.load().qx_g0w0_desc_default(C.byref(d))
+    d.nocc, d.nvir, d.naux = int(nocc), int(nvir), int(naux)
+    d.nfreq, d.x0, d.eta = int(nfreq), float(x0), float(eta)
+    d.precision = _lib.PREC[precision] if isinstance(precision, str) else int(precision)
+    d.ntarget, d.npair = int(ntarget), int(npair)
+    for k, v in newton.items():
+        setattr(d.newton, k, v)
+    return d
+
+
+@dataclass
+class Pairs:
+    """Target-partner rows b_nm (symmetrised full tensor, zero rows dropped), on device."""
+    target: torch.Tensor     # int32 [npair], sorted
+    m: torch.Tensor          # int32 [npair]
+    rows: torch.Tensor       # f64 [npair, naux]
+
+
+def pairs_from_blocks(iaP, miP, abP, nocc, nvir, naux, targets, device=0) -> Pairs:
+    """Pair rows of the tensor assembled as quasix/src/evgw.rs:161-205 does.
+
+    X[:, :nocc] = miP, X[:nocc, nocc:] = iaP, X[nocc:, nocc:] = abP, then
+    B_nm = (X[n, m] + X[m, n]) / 2.  Only the rows of the targets are formed.
+    """
+    dev = _dev(device)
+    nmo = nocc + nvir
+    ia = _f64(iaP, dev).reshape(nocc, nvir, naux)
+    mi = _f64(miP, dev).reshape(nmo, nocc, naux)
+    ab = _f64(abP, dev).reshape(nvir, nvir, naux)
+
+    def xrow(n):                       # X[n, :, :]  [nmo, naux]
+        r = torch.zeros((nmo, naux), dtype=torch.float64, device=dev)
+        r[:nocc] = mi[n]
+        if n < nocc:
+            r[nocc:] = ia[n]
+        else:
+            r[nocc:] = ab[n - nocc]
+        return r
+
+    def xcol(n):                       # X[:, n, :]  [nmo, naux]
+        if n < nocc:
+            return mi[:, n, :].clone()
+        c = torch.zeros((nmo, naux), dtype=torch.float64, device=dev)
+        c[:nocc] = ia[:, n - nocc, :]
+        c[nocc:] = ab[:, n - nocc, :]
+        return c
+
+    ts, ms, rows = [], [], []
+    for t, n in enumerate(np.asarray(targets).tolist()):
+        row = 0.5 * (xrow(n) + xcol(n))
+        nz = torch.nonzero(torch.any(row != 0.0, dim=1)).flatten()
+        ts.append(torch.full((nz.numel(),), t, dtype=torch.int32, device=dev))
+        ms.append(nz.to(torch.int32))
+        rows.append(row[nz])
+    return Pairs(torch.cat(ts), torch.cat(ms), torch.cat(rows).contiguous())
+
+
+def symmetrized_ov(iaP, miP, nocc, nvir, naux, device=0):
+    """Pi's ov block of the symmetrised tensor: (iaP[i,a] + miP[nocc+a, i]) / 2."""
+    dev = _dev(device)
+    ia = _f64(iaP, dev).reshape(nocc, nvir, naux)
+    mi = _f64(miP, dev).reshape(nocc + nvir, nocc, naux)
+    return (0.5 * (ia + mi[nocc:].permute(1, 0, 2))).reshape(nocc * nvir, naux).contiguous()
+
+
+def g0w0(mo_energy, B_ov, nocc, targets=None, *, nfreq=48, x0=0.5, eta=0.01, precision="fp64",
+         static=None, pairs: Pairs | None = None, mo_energy_ref=None, guess=None, device=0,
+         newton=None, stats=False):
+    """One G0W0 solve on the GPU.
+
+    mo_energy [nmo]; B_ov [nocc*nvir, naux] (device tensor or array); targets: orbital indices
+    (default HOMO-5..LUMO+5); static: Sigma_x - V_xc per target (default 0); pairs: explicit
+    target-partner rows (default: the ov-only pairs gathered on the device).
+    Returns dict(qp_energies, z_factors, sigma_c (complex), state, targets[, stats]).
+    """
+    dev = _dev(device)
+    h = _lib.handle(device)
+    e = _f64(mo_energy, dev)
+    nmo = e.numel()
+    B = _f64(B_ov, dev)
+    naux = B.shape[1]
+    nvir = nmo - nocc
+    if B.shape[0] != nocc * nvir:
+        raise ValueError(f"B_ov has {B.shape[0]} rows, expected nocc*nvir={nocc * nvir}")
+    tg = frontier_targets(nocc, nmo) if targets is None else np.asarray(targets, dtype=np.int32)
+    t_dev = _i32(tg, dev)
+    nt = len(tg)
+    d = make_desc(nocc, nvir, naux, nfreq, x0, eta, precision, nt, 0 if pairs is None
+                  else pairs.rows.shape[0], **(newton or {}))
+    st = None if static is None else _f64(static, dev)
+    er = None if mo_energy_ref is None else _f64(mo_energy_ref, dev)
+    gs = None if guess is None else _f64(guess, dev)
+    out = torch.empty((4, nt), dtype=torch.float64, device=dev)
+    state = torch.empty(nt, dtype=torch.int32, device=dev)
+    s = _lib.Stats()
+    torch.cuda.synchronize(dev)
+    if pairs is None:
+        rc = h.lib.qx_g0w0_cd_ov_dev(h.h, C.byref(d), _ptr(e), _ptr(er), _ptr(B), _ptr(t_dev),
+                                     _ptr(st), _ptr(gs), _ptr(out[0]), _ptr(out[1]),
+                                     _ptr(out[2]), _ptr(out[3]), _ptr(state), C.byref(s))
+    else:
+        rc = h.lib.qx_g0w0_cd_dev(h.h, C.byref(d), _ptr(e), _ptr(er), _ptr(B), _ptr(pairs.rows),
+                                  _ptr(pairs.target), _ptr(pairs.m), _ptr(t_dev), _ptr(st),
+                                  _ptr(gs), _ptr(out[0]), _ptr(out[1]), _ptr(out[2]),
+                                  _ptr(out[3]), _ptr(state), C.byref(s))
+    h.check(rc)
+    o = out.cpu().numpy()
+    res = dict(qp_energies=o[0], z_factors=o[1], sigma_c=o[2] + 1j * o[3],
+               state=state.cpu().numpy(), targets=tg)
+    if stats:
+        res["stats"] = {k: getattr(s, k) for k, _ in _lib.Stats._fields_}
+    return res
+
+
+def evgw_core(mo_energy, B_ov, nocc, targets=None, *, n_iter=8, damping=0.5, conv_tol=1e-4,
+              nfreq=64, x0=0.5, eta=0.01, precision="fp64", static=None, pairs=None,
+              device=0, newton=None):
+    """evGW on the CD self-energy: Pi, poles and Green's function from the previous QP
+    energies, QP equation anchored at the mean-field energies (quasix_core/src/qp/evgw.rs:
+    603-1027: iteration 0 direct, later iterations damped, Newton restarted from the previous
+    QP energy).  Orbitals outside ``targets`` keep their mean-field energies.  B stays resident
+    on the device across iterations."""
+    dev = _dev(device)
+    e0 = _f64(mo_energy, dev)
+    B = _f64(B_ov, dev)
+    nmo = e0.numel()
+    tg = frontier_targets(nocc, nmo) if targets is None else np.asarray(targets, dtype=np.int32)
+    tgt = torch.as_tensor(tg, dtype=torch.long, device=dev)
+    cur = e0.clone()
+    history = []
+    last = None
+    converged = False
+    for it in range(n_iter):
+        guess = None if it == 0 else cur[tgt]
+        last = g0w0(cur, B, nocc, tg, nfreq=nfreq, x0=x0, eta=eta, precision=precision,
+                    static=static, pairs=pairs, mo_energy_ref=e0, guess=guess, device=device,
+                    newton=newton)
+        new = cur.clone()
+        Et = torch.as_tensor(last["qp_energies"], device=dev)
+        new[tgt] = Et if it == 0 else (1.0 - damping) * cur[tgt] + damping * Et
+        dmax = float(torch.max(torch.abs(new[tgt] - cur[tgt])))
+        history.append(dict(cycle=it, qp_energies=new.cpu().numpy(), energy_change=dmax,
+                            z_factors=last["z_factors"]))
+        cur = new
+        if it > 0 and dmax < conv_tol:
+            converged = True
+            break
+    return dict(qp_energies=cur.cpu().numpy(), converged=converged, n_cycles=len(history),
+                history=history, last=last, targets=tg)
+
+
+def evgw(mo_energy, mo_occ, iaP, ijP, abP, chol_v, vxc_dft, max_cycle=12, conv_tol=1e-4,
+         conv_tol_z=1e-3, damping=0.5, damping_dynamic=False, diis=False, diis_space=6,
+         nfreq=48, eta=0.01, verbose=1, energy_window=2.0, frontier_convergence=True,
+         frontier_window=1.0, n_occ_active=None, n_vir_active=None, qp_solver="linearized",
+         newton_max_iterations=50, newton_energy_tolerance=1e-8, *, precision="fp64",
+         targets=None, sigma_x=None, device=0):
+    """quasix.quasix.evgw on the MI355X path (quasix/src/evgw.rs:22-453).
+
+    Same positional arguments and keyword defaults; the self-energy is the contour-deformation
+    one of this library.  ``sigma_x`` (per orbital) defaults to vxc_dft, i.e. Sigma_x - V_xc = 0
+    (the convention of quasix/quasix/g0w0.py:284-292).  Only the target orbitals (default: the
+    n_occ_active/n_vir_active window, else HOMO-5..LUMO+5) are updated.
+    """
+    if qp_solver not in ("linearized", "newton"):
+        raise ValueError(f"qp_solver must be 'linearized' or 'newton', got {qp_solver!r}")
+    e = np.ascontiguousarray(mo_energy, dtype=np.float64)
+    occ = np.asarray(mo_occ)
+    nocc = int(np.count_nonzero(occ > 0))
+    nmo = e.size
+    nvir = nmo - nocc
+    naux = int(np.asarray(chol_v).shape[0])
+    try:
+        ia = np.ascontiguousarray(iaP, dtype=np.float64).reshape(nocc * nvir, naux)
+        mi = np.ascontiguousarray(ijP, dtype=np.float64).reshape(nmo * nocc, naux)
+        ab = np.ascontiguousarray(abP, dtype=np.float64).reshape(nvir * nvir, naux)
+    except ValueError as exc:
+        raise ValueError(f"Failed to reshape DF tensors: {exc}") from exc
+    if targets is None:
+        no = 5 if n_occ_active is None else int(n_occ_active) - 1
+        nv = 5 if n_vir_active is None else int(n_vir_active) - 1
+        targets = frontier_targets(nocc, nmo, no, nv)
+    tg = np.asarray(targets, dtype=np.int32)
+    vxc = np.asarray(vxc_dft, dtype=np.float64)
+    sx = vxc if sigma_x is None else np.asarray(sigma_x, dtype=np.float64)
+    static = (sx - vxc)[tg]
+    B = symmetrized_ov(ia, mi, nocc, nvir, naux, device)
+    pairs = pairs_from_blocks(ia, mi, ab, nocc, nvir, naux, tg, device)
+    newton = dict(max_iterations=int(newton_max_iterations),
+                  energy_tolerance=float(newton_energy_tolerance))
+    r = evgw_core(e, B, nocc, tg, n_iter=int(max_cycle), damping=damping, conv_tol=conv_tol,
+                  nfreq=nfreq, eta=eta, precision=precision, static=static, pairs=pairs,
+                  device=device, newton=newton)
+    qp = r["qp_energies"]
+    z = np.ones(nmo)
+    z[tg] = r["last"]["z_factors"]
+    sig = np.zeros(nmo, dtype=np.complex128)
+    sig[tg] = r["last"]["sigma_c"]
+    hist = [dict(cycle=h["cycle"], qp_energies=h["qp_energies"], z_factors=z.copy(),
+                 energy_change=h["energy_change"], rms_change=h["energy_change"],
+                 damping_used=damping if h["cycle"] > 0 else 1.0,
+                 converged=(i == len(r["history"]) - 1) and r["converged"])
+            for i, h in enumerate(r["history"])]
+    homo, lumo = qp[nocc - 1], qp[nocc]
+    return dict(qp_energies=qp, z_factors=z, converged=r["converged"], n_cycles=r["n_cycles"],
+                final_error=r["history"][-1]["energy_change"], sigma_x=sx, sigma_c_re=sig.real,
+                sigma_c_im=sig.imag, iteration_history=hist, ip=-homo * HARTREE_TO_EV,
+                ea=-lumo * HARTREE_TO_EV, gap=(lumo - homo) * HARTREE_TO_EV)
+
+
+_RUN_EVGW_REQUIRED = ("mo_energy", "mo_occ", "iaP", "ijP", "chol_v", "vxc_dft")
+
+
+def run_evgw(config: dict):
+    """Dict front-end with the keys of quasix.quasix.run_evgw (quasix/src/evgw.rs:456-677);
+    a missing required key raises KeyError, eta is fixed at 0.01 as there."""
+    for k in _RUN_EVGW_REQUIRED:
+        if k not in config:
+            raise KeyError(k)
+    e = np.asarray(config["mo_energy"], dtype=np.float64)
+    occ = np.asarray(config["mo_occ"])
+    nocc = int(np.count_nonzero(occ > 0))
+    nvir = e.size - nocc
+    naux = int(np.asarray(config["chol_v"]).shape[0])
+    ab = config.get("abP")
+    if ab is None:
+        ab = np.zeros((nvir * nvir, naux))
+    return evgw(e, occ, config["iaP"], config["ijP"], ab, config["chol_v"], config["vxc_dft"],
+                max_cycle=config.get("max_cycle", 12), conv_tol=config.get("conv_tol", 1e-4),
+                damping=config.get("damping", 0.5), nfreq=config.get("nfreq", 48), eta=0.01,
+                diis=config.get("diis", True), diis_space=config.get("diis_space", 6),
+                frontier_window=config.get("frontier_window", 0.1),
+                n_occ_active=config.get("n_occ_active", 3),
+                n_vir_active=config.get("n_vir_active", 3),
+                qp_solver=config.get("qp_solver", "linearized"),
+                newton_max_iterations=config.get("newton_max_iterations", 50),
+                newton_energy_tolerance=config.get("newton_energy_tolerance", 1e-8))
+
+
+class G0W0Solver:
+    """Repeated G0W0 solves on device-resident inputs (no host round trips besides the
+    library's own per-solve status check).  Outputs stay in ``self.out`` (device)."""
+
+    def __init__(self, mo_energy, B_ov, nocc, targets=None, *, nfreq=48, x0=0.5, eta=0.01,
+                 precision="bf16x3", device=0, newton=None):
+        self.dev = _dev(device)
+        self.h = _lib.handle(device)
+        self.e = _f64(mo_energy, self.dev)
+        self.B = _f64(B_ov, self.dev)
+        nmo = self.e.numel()
+        self.nocc = nocc
+        self.targets = frontier_targets(nocc, nmo) if targets is None else np.asarray(targets,
+                                                                                      np.int32)
+        self.t_dev = _i32(self.targets, self.dev)
+        nt = len(self.targets)
+        self.desc = make_desc(nocc, nmo - nocc, self.B.shape[1], nfreq, x0, eta, precision, nt,
+                              0, **(newton or {}))
+        self.out = torch.empty((4, nt), dtype=torch.float64, device=self.dev)
+        self.state = torch.empty(nt, dtype=torch.int32, device=self.dev)
+        self.stats = _lib.Stats()
+        torch.cuda.synchronize(self.dev)
+
+    def run(self):
+        o = self.out
+        self.h.check(self.h.lib.qx_g0w0_cd_ov_dev(
+            self.h.h, C.byref(self.desc), _ptr(self.e), None, _ptr(self.B), _ptr(self.t_dev),
+            None, None, _ptr(o[0]), _ptr(o[1]), _ptr(o[2]), _ptr(o[3]), _ptr(self.state),
+            C.byref(self.stats)))
+
+    def set_profiling(self, on: bool):
+        self.h.check(self.h.lib.qx_set_profiling(self.h.h, 1 if on else 0))
+
+    def profile(self):
+        ms = (C.c_double * 8)()
+        n = (C.c_longlong * 8)()
+        self.h.check(self.h.lib.qx_get_profile(self.h.h, ms, n))
+        return {name: (ms[i], n[i]) for i, name in enumerate(_lib.PHASES)}
+
+    def result(self):
+        o = self.out.cpu().numpy()
+        return dict(qp_energies=o[0], z_factors=o[1], sigma_c=o[2] + 1j * o[3],
+                    state=self.state.cpu().numpy(), targets=self.targets)
