@@ -56,7 +56,7 @@ __device__ __forceinline__ void for_acc(const d4 (&acc)[4], int wave, int lane, 
     for (int bj = 0; bj < 4; ++bj)
         #pragma unroll
         for (int v = 0; v < 4; ++v)
-            fn(wave * 16 + 4 * (lane >> 4) + v, bj * 16 + (lane & 15), acc[bj][v]);
+            fn(wave * 16 + mfma_row_f64(lane, v), bj * 16 + (lane & 15), acc[bj][v]);
 }
 
 __global__ __launch_bounds__(256) void potrf_diag_kernel(double* __restrict__ a,

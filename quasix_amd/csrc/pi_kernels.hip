@@ -85,7 +85,9 @@ template <> struct Mfma<float> {
     }
 };
 
-template <typename T>
+// F64L: the accumulators come from v_mfma_f64_16x16x4f64 (row = 4 v + lane/16); otherwise
+// from an FP32-output MFMA (row = 4 (lane/16) + v).
+template <typename T, bool F64L>
 __device__ __forceinline__ void store_tile(const typename Mfma<T>::acc_t (&acc)[4][4],
                                            int I0, int J0, int wr, int wc, int lane,
                                            int npad, int naux, int f, PiOut mode,
@@ -96,15 +98,18 @@ __device__ __forceinline__ void store_tile(const typename Mfma<T>::acc_t (&acc)[
         for (int bj = 0; bj < 4; ++bj)
             #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                const int row = I0 + wr * 64 + bi * 16 + 4 * (lane >> 4) + v;
+                const int row = I0 + wr * 64 + bi * 16 +
+                                (F64L ? mfma_row_f64(lane, v) : mfma_row_f32(lane, v));
                 const int col = J0 + wc * 64 + bj * 16 + (lane & 15);
                 const double p = (double)acc[bi][bj][v] * sign_scale;
                 if (mode == PiOut::EpsLower) {
                     if (row < npad && col < npad)
                         out[((size_t)f * npad + row) * npad + col] = (row == col ? 1.0 : 0.0) - p;
                 } else {
-                    // full symmetric Pi with leading dimension npad
-                    if (row < naux && col < naux) {
+                    // full symmetric Pi with leading dimension npad; on diagonal tiles only
+                    // the lower triangle is mirrored so the result is exactly symmetric
+                    // (the reference symmetrises explicitly, polarizability.rs:264-272)
+                    if (row < naux && col < naux && (I0 != J0 || row >= col)) {
                         out[((size_t)f * npad + row) * npad + col] = p;
                         out[((size_t)f * npad + col) * npad + row] = p;
                     }
@@ -163,7 +168,7 @@ __global__ __launch_bounds__(256) void pi_syrk_kernel(
         }
         __syncthreads();
     }
-    store_tile<T>(acc, I0, J0, wr, wc, lane, npad, naux, f, mode, out, 1.0);
+    store_tile<T, sizeof(T) == 8>(acc, I0, J0, wr, wc, lane, npad, naux, f, mode, out, 1.0);
 }
 
 // ---- BF16x3 kernel ----------------------------------------------------------------------
@@ -291,7 +296,7 @@ __global__ __launch_bounds__(256) void pi_syrk_bf16x3_kernel(
         }
         __syncthreads();
     }
-    store_tile<double>(acc64, I0, J0, wr, wc, lane, npad, naux, f, mode, out, 1.0);
+    store_tile<double, false>(acc64, I0, J0, wr, wc, lane, npad, naux, f, mode, out, 1.0);
 }
 
 }  // namespace

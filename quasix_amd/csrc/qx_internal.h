@@ -22,6 +22,13 @@ constexpr int kMaxTargetsPerLaunch = 1 << 20;
 inline int round_up(int x, int m) { return (x + m - 1) / m * m; }
 inline int ceil_div(int x, int m) { return (x + m - 1) / m; }
 
+// Row of accumulator element v held by `lane` in a 16x16 MFMA output block (column is
+// lane % 16 for all three).  Measured on gfx950 with an in-bounds probe:
+//   v_mfma_f32_16x16x4f32, v_mfma_f32_16x16x32_bf16:  row = 4 * (lane / 16) + v
+//   v_mfma_f64_16x16x4f64:                            row = 4 * v + lane / 16
+__device__ __forceinline__ int mfma_row_f32(int lane, int v) { return 4 * (lane >> 4) + v; }
+__device__ __forceinline__ int mfma_row_f64(int lane, int v) { return 4 * v + (lane >> 4); }
+
 // ---- Polarizability (pi_kernels.hip) ---------------------------------------------------
 // Pi is built from the frequency weights s_f(r) = 4 e_r / (w_f^2 + e_r^2),
 // e_r = eps_i - eps_a for row r = i*nvir + a.

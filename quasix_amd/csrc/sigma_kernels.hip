@@ -74,7 +74,7 @@ __global__ __launch_bounds__(256) void wnm_linv_kernel(const double* __restrict_
     for (int bj = 0; bj < 4; ++bj)
         #pragma unroll
         for (int v = 0; v < 4; ++v) {
-            const int r = wave * 16 + 4 * (lane >> 4) + v, c = bj * 16 + (lane & 15);
+            const int r = wave * 16 + mfma_row_f64(lane, v), c = bj * 16 + (lane & 15);
             const double y = acc[bj][v];
             rs[v] += 2.0 * Bs[r][c] * y + y * y;
         }
@@ -84,7 +84,7 @@ __global__ __launch_bounds__(256) void wnm_linv_kernel(const double* __restrict_
     if ((lane & 15) == 0)
         #pragma unroll
         for (int v = 0; v < 4; ++v) {
-            const int j = j0 + wave * 16 + 4 * (lane >> 4) + v;
+            const int j = j0 + wave * 16 + mfma_row_f64(lane, v);
             if (j < npair) wpart[((size_t)f * nblk + pb) * npair + j] = rs[v];
         }
 }
@@ -139,7 +139,7 @@ __global__ __launch_bounds__(256) void wnm_wc_kernel(const double* __restrict__ 
     for (int bj = 0; bj < 4; ++bj)
         #pragma unroll
         for (int v = 0; v < 4; ++v) {
-            const int r = wave * 16 + 4 * (lane >> 4) + v, c = bj * 16 + (lane & 15);
+            const int r = wave * 16 + mfma_row_f64(lane, v), c = bj * 16 + (lane & 15);
             rs[v] += Bs[r][c] * acc[bj][v];
         }
     #pragma unroll
@@ -148,7 +148,7 @@ __global__ __launch_bounds__(256) void wnm_wc_kernel(const double* __restrict__ 
     if ((lane & 15) == 0)
         #pragma unroll
         for (int v = 0; v < 4; ++v) {
-            const int j = j0 + wave * 16 + 4 * (lane >> 4) + v;
+            const int j = j0 + wave * 16 + mfma_row_f64(lane, v);
             if (j < npair) wpart[((size_t)f * nblk + pb) * npair + j] = rs[v];
         }
 }

@@ -20,6 +20,14 @@ def _c0():
     return np.load(os.path.join(GOLD, "c0_g0w0.npz"))
 
 
+def _close(name, got, ref, tol, scale=None):
+    """max |got - ref| <= tol * scale (scale defaults to 1); prints the observed error."""
+    err = float(np.max(np.abs(np.asarray(got) - np.asarray(ref))))
+    lim = tol * (1.0 if scale is None else float(scale))
+    print(f"{name}: max abs err {err:.3e} (limit {lim:.3e})")
+    assert err <= lim, f"{name}: {err:.3e} > {lim:.3e}"
+
+
 def test_pi_fp64_matches_golden(gpu):
     from quasix_amd import core
     g = _c0()
@@ -29,8 +37,8 @@ def test_pi_fp64_matches_golden(gpu):
     p0 = core.compute_polarizability_p0(B.reshape(nocc, -1, naux), e, occ, g["freqs"])
     ref = g["pi"]
     got = p0[g["pi_idx"]]
-    assert np.max(np.abs(got - ref)) <= 1e-12 * np.max(np.abs(ref))
-    assert np.max(np.abs(p0 - p0.transpose(0, 2, 1))) == 0.0
+    _close("pi fp64", got, ref, 1e-12, np.max(np.abs(ref)))
+    _close("pi symmetry", p0, p0.transpose(0, 2, 1), 0.0)
 
 
 @pytest.mark.parametrize("prec,tol", [("fp64", 1e-12), ("fp32", 3e-5), ("bf16x3", 3e-5)])
@@ -43,7 +51,7 @@ def test_pi_ragged_shape(gpu, prec, tol):
     p0 = core.compute_polarizability_p0(B.reshape(nocc, nvir, naux), e, occ, fr, precision=prec)
     for k, w in enumerate(fr):
         ref = O.get_rho_response(w, e, B, nocc)
-        assert np.max(np.abs(p0[k] - ref)) <= tol * np.max(np.abs(ref)), (prec, k)
+        _close(f"pi {prec} k={k}", p0[k], ref, tol, np.max(np.abs(ref)))
 
 
 def test_screened_wc(gpu):
@@ -54,7 +62,7 @@ def test_screened_wc(gpu):
     assert nfb == 0
     for k in range(pis.shape[0]):
         ref = O.screened_wc(pis[k])
-        assert np.max(np.abs(wc[k] - ref)) <= 1e-11 * max(1.0, np.max(np.abs(ref)))
+        _close(f"wc k={k}", wc[k], ref, 1e-11, max(1.0, np.max(np.abs(ref))))
 
 
 def test_screened_wc_lu_fallback(gpu):
@@ -69,7 +77,7 @@ def test_screened_wc_lu_fallback(gpu):
     wc, nfb = core.compute_screening_wc(pi[None])
     assert nfb == 1
     ref = O.screened_wc(pi)
-    assert np.max(np.abs(wc[0] - ref)) <= 1e-10 * np.max(np.abs(ref))
+    _close("wc lu", wc[0], ref, 1e-10, np.max(np.abs(ref)))
 
 
 def test_wnm_matches_golden(gpu):
@@ -80,7 +88,7 @@ def test_wnm_matches_golden(gpu):
     pt, pm, Bp = O.pair_list(mn, g["targets"])
     wnm = core.compute_wmn_imag(e, B, 5, Bp, g["freqs"])
     ref = g["wnm"]
-    assert np.max(np.abs(wnm - ref)) <= 1e-12 * np.max(np.abs(ref))
+    _close("wnm", wnm, ref, 1e-12, np.max(np.abs(ref)))
 
 
 def test_g0w0_c0_fp64_matches_golden(gpu):
@@ -88,10 +96,10 @@ def test_g0w0_c0_fp64_matches_golden(gpu):
     g = _c0()
     r = gw.g0w0(g["mo_energy"], g["B_ov"], 5, g["targets"], nfreq=32, stats=True)
     assert np.all(r["state"] == 0)
-    assert np.max(np.abs(r["qp_energies"] - g["E"])) < 1e-6
-    assert np.max(np.abs(r["qp_energies"] - g["E"])) < 1e-9      # observed headroom
-    assert np.max(np.abs(r["z_factors"] - g["Z"])) < 1e-6
-    assert np.max(np.abs(r["sigma_c"] - g["sigma"])) < 1e-8
+    _close("E_QP", r["qp_energies"], g["E"], 1e-6)
+    _close("E_QP headroom", r["qp_energies"], g["E"], 1e-9)
+    _close("Z", r["z_factors"], g["Z"], 1e-6)
+    _close("Sigma", r["sigma_c"], g["sigma"], 1e-8)
     assert r["stats"]["residue_targets"] == 0
 
 
@@ -112,9 +120,10 @@ def test_g0w0_full_blocks_with_residues(gpu):
     abP = full[nocc:, nocc:].reshape(-1, naux)
     pairs = gw.pairs_from_blocks(iaP, miP, abP, nocc, nvir, naux, targets)
     r = gw.g0w0(e, B, nocc, targets, nfreq=32, eta=0.01, pairs=pairs, stats=True)
+    print("stats", r["stats"])
     assert r["stats"]["residue_targets"] > 0 and r["stats"]["residue_jobs"] > 0
-    assert np.max(np.abs(r["qp_energies"] - ref["E"])) < 1e-6
-    assert np.max(np.abs(r["z_factors"] - ref["Z"])) < 1e-5
+    _close("E_QP", r["qp_energies"], ref["E"], 1e-6)
+    _close("Z", r["z_factors"], ref["Z"], 1e-5)
 
 
 def test_g0w0_c3_fp64_and_mixed(gpu):
@@ -126,7 +135,7 @@ def test_g0w0_c3_fp64_and_mixed(gpu):
     fr, wt = O.get_scaled_legendre_roots(32, 0.5)
     ref = O.g0w0_cd(e, B, nocc, targets, fr, wt)
     r64 = gw.g0w0(e, B, nocc, targets, nfreq=32)
-    assert np.max(np.abs(r64["qp_energies"] - ref["E"])) < 1e-6
+    _close("E_QP fp64", r64["qp_energies"], ref["E"], 1e-6)
     for prec in ("fp32", "bf16x3"):
         rm = gw.g0w0(e, B, nocc, targets, nfreq=32, precision=prec)
         dE = np.max(np.abs(rm["qp_energies"] - r64["qp_energies"]))
