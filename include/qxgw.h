@@ -69,7 +69,8 @@ typedef struct {
     int nfreq;                 /* Gauss-Legendre imaginary frequencies */
     double x0;                 /* grid scale, w = x0 (1+t)/(1-t)  (correlation_ac.rs:94-114) */
     double eta;                /* broadening of the CD Green's function and residues */
-    int precision;             /* qx_precision of the Pi build */
+    int precision;             /* qx_precision of the Pi build (BF16X3 also runs the Wc_nm
+                                  product on BF16 MFMA; factorisation stays FP64) */
     int ntarget;               /* number of target orbitals */
     int npair;                 /* number of (target, partner) rows of B_pair */
     qx_newton_config newton;
@@ -82,6 +83,7 @@ typedef struct {
     int residue_jobs;          /* real-axis Wc evaluations performed */
     int newton_failures;       /* targets that fell back to the linearised solution */
     double seconds;            /* wall time of the call */
+    int pi_variant;            /* Pi kernel instance chosen by the autotuner */
 } qx_stats;
 
 /* Per-target state reported in `state` arrays. */
@@ -105,8 +107,8 @@ QXGW_API void qx_g0w0_desc_default(qx_g0w0_desc* d);
 /* Phases timed with HIP events on the handle's stream while profiling is enabled. */
 #define QX_PHASE_PREP 0      /* frequency weights, mixed-precision operand split */
 #define QX_PHASE_PI 1        /* Pi build (one launch per solve, all frequencies) */
-#define QX_PHASE_POTRF 2     /* blocked Cholesky of all eps_f */
-#define QX_PHASE_TRTRI 3     /* L^-1 */
+#define QX_PHASE_POTRF 2     /* blocked Cholesky of all eps_f, with L^-1 in the same loop */
+#define QX_PHASE_TRTRI 3     /* reserved (L^-1 is formed inside the Cholesky step loop) */
 #define QX_PHASE_WNM 4       /* Wc_nm contraction */
 #define QX_PHASE_NEWTON 5    /* device Newton kernel */
 #define QX_PHASE_RESIDUE 6   /* residue (real-axis Wc) jobs */
@@ -187,6 +189,18 @@ QXGW_API qx_status qx_g0w0_cd_ov_dev(qx_handle h, const qx_g0w0_desc* d,
                                      const double* static_terms, const double* guess,
                                      double* qp_energy, double* z_factor, double* sigma_re,
                                      double* sigma_im, int* state, qx_stats* stats);
+/* ---- Adjoint (no reference counterpart: the reference has no derivatives) -------------- */
+/* grad_B[N_ov][naux] = dL/dB_ov and grad_eps[nmo] = dL/d mo_energy for
+ * L = sum_t grad_qp[t] * qp_energy[t], where qp_energy comes from qx_g0w0_cd_ov_dev (FP64) with
+ * the same inputs and Sigma_x - V_xc held fixed.  Implicit differentiation of the Newton QP
+ * equation, reverse mode through the quadrature, Wc = eps^-1 - I and Pi; re-forms the FP64
+ * factorisation and keeps X_k = L_k^-1 of every frequency as the checkpoint.  Requires that
+ * no residue pole is active at the solutions (ov-only tensors, frontier targets). */
+QXGW_API qx_status qx_g0w0_adjoint_ov_dev(qx_handle h, const qx_g0w0_desc* d,
+                                          const double* mo_energy, const double* B_ov,
+                                          const int* targets, const double* qp_energy,
+                                          const double* grad_qp, double* grad_B,
+                                          double* grad_eps);
 /* Host-pointer variant of qx_g0w0_cd_dev. */
 QXGW_API qx_status qx_g0w0_cd(qx_handle h, const qx_g0w0_desc* d, const double* mo_energy,
                               const double* mo_energy_ref, const double* B_ov,

@@ -690,3 +690,76 @@ def linearized_qp_ac(mo_energy, sigma_x, vxc, coeff, ofit, orbs, nocc, de=1e-6):
         z = 1.0 / (1.0 - (sp - s) / de)
         out[i] = e[p] + z * (sigma_x[p] + s - vxc[p])
     return out
+
+
+# --------------------------------------------------------------------------------------
+# Adjoint of the G0W0 solve (d E_QP / d B, d E_QP / d eps) -- no reference counterpart
+# --------------------------------------------------------------------------------------
+
+def _g_and_dg(E, em, sign_eta, freqs, wts):
+    """Green's-function weights g[k, j] = W_k emo / (emo^2 + w_k^2) and dg/dE (complex)."""
+    emo = (E - em)[None, :] - 1j * sign_eta[None, :]
+    w2 = (freqs ** 2)[:, None]
+    den = emo ** 2 + w2
+    g = wts[:, None] * emo / den
+    dg = wts[:, None] * (w2 - emo ** 2) / den ** 2
+    return g, dg
+
+
+def g0w0_adjoint(mo_energy, B_ov, nocc, targets, freqs, wts, E, grad_E, *, eta=0.01):
+    """Gradient of L = sum_t grad_E[t] * E_QP[t] w.r.t. B_ov and mo_energy (ov-only pairs,
+    no residue poles at the solution; Sigma_x - V_xc held fixed).
+
+    Implicit differentiation of f(E) = E - eps_n - s - Re Sigma_n(E) = 0:
+      dE/dtheta = Z (d eps_n/dtheta + d Re Sigma_n/dtheta),  Z = 1 / (1 - d Re Sigma_n / dE);
+    Sigma_n = -(1/pi) sum_k sum_m w_nm(k) g(E - eps_m, w_k);  w_nm(k) = b^T Wc_k b;
+    dWc = eps^-1 dPi eps^-1;  Pi_k = sum_r s_k(r) b_r b_r^T,  s_k = 4 e / (w_k^2 + e^2).
+    Returns (grad_B [N_ov, naux], grad_eps [nmo]).
+    """
+    e = np.asarray(mo_energy, dtype=np.float64)
+    B = np.asarray(B_ov, dtype=np.float64)
+    nmo = e.size
+    nvir = nmo - nocc
+    naux = B.shape[1]
+    targets = np.asarray(targets)
+    eia = (e[:nocc, None] - e[None, nocc:]).reshape(-1)
+    nf = len(freqs)
+    # forward pieces with checkpoints
+    einv, wc = [], []
+    for w in freqs:
+        pi = get_rho_response(w, e, B, nocc)
+        inv = np.linalg.inv(np.eye(naux) - pi)
+        einv.append(inv)
+        wc.append(inv - np.eye(naux))
+    gB = np.zeros_like(B)
+    geps = np.zeros(nmo)
+    pibar = np.zeros((nf, naux, naux))
+    for t, n in enumerate(targets):
+        if n < nocc:
+            rows = n * nvir + np.arange(nvir)
+            m = nocc + np.arange(nvir)
+        else:
+            rows = np.arange(nocc) * nvir + (n - nocc)
+            m = np.arange(nocc)
+        bp = B[rows]                                              # [np, naux]
+        wnm = np.stack([np.einsum("jp,jp->j", bp @ wc[k], bp) for k in range(nf)])
+        sign = np.where(e[n] > e[m], 1.0, -1.0) * eta
+        g, dg = _g_and_dg(E[t], e[m], sign, freqs, wts)
+        dsig = -np.sum(wnm * dg.real) / math.pi
+        zc = grad_E[t] / (1.0 - dsig)                             # c_t * Z_raw
+        geps[n] += zc                                              # anchor eps_n
+        geps[m] += zc * (np.sum(wnm * dg.real, axis=0) / math.pi)  # d/d eps_m = -d/dE
+        abar = zc * (-1.0 / math.pi) * g.real                     # [nf, np]
+        for k in range(nf):
+            wb = (bp * abar[k][:, None]).T @ bp                   # W-bar_k
+            pibar[k] += einv[k] @ wb @ einv[k]
+            np.add.at(gB, rows, 2.0 * abar[k][:, None] * (bp @ wc[k]))
+    for k, w in enumerate(freqs):
+        s = 4.0 * eia / (w * w + eia * eia)
+        ds = 4.0 * (w * w - eia * eia) / (w * w + eia * eia) ** 2
+        Y = B @ pibar[k]
+        gB += 2.0 * s[:, None] * Y
+        ebar = ds * np.einsum("rp,rp->r", Y, B)
+        geps[:nocc] += ebar.reshape(nocc, nvir).sum(axis=1)
+        geps[nocc:] -= ebar.reshape(nocc, nvir).sum(axis=0)
+    return gB, geps

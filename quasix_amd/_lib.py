@@ -26,7 +26,7 @@ EXPORTED = [
     "qx_release_workspace", "qx_newton_config_default", "qx_g0w0_desc_default",
     "qx_gauss_legendre_grid", "qx_scaled_legendre_roots", "qx_rho_response_dev",
     "qx_screened_wc_dev", "qx_wmn_imag_dev", "qx_sigma_cd_dev", "qx_g0w0_cd_dev", "qx_g0w0_cd_ov_dev", "qx_g0w0_cd",
-    "qx_set_profiling", "qx_get_profile",
+    "qx_set_profiling", "qx_get_profile", "qx_g0w0_adjoint_ov_dev",
 ]
 
 
@@ -53,7 +53,7 @@ class Stats(C.Structure):
     _fields_ = [
         ("chol_fallbacks", C.c_int), ("singular_fallbacks", C.c_int),
         ("residue_targets", C.c_int), ("residue_jobs", C.c_int),
-        ("newton_failures", C.c_int), ("seconds", C.c_double),
+        ("newton_failures", C.c_int), ("seconds", C.c_double), ("pi_variant", C.c_int),
     ]
 
 
@@ -105,6 +105,7 @@ def load():
         lib.qx_g0w0_cd.argtypes = lib.qx_g0w0_cd_dev.argtypes
         lib.qx_g0w0_cd_ov_dev.argtypes = [vp, C.POINTER(G0W0Desc), dp, dp, dp, ip, dp, dp, dp, dp,
                                           dp, dp, ip, C.POINTER(Stats)]
+        lib.qx_g0w0_adjoint_ov_dev.argtypes = [vp, C.POINTER(G0W0Desc), dp, dp, ip, dp, dp, dp, dp]
         lib.qx_set_profiling.argtypes = [vp, C.c_int]
         lib.qx_get_profile.argtypes = [vp, dp, dp]
         non_status = ("qx_destroy", "qx_last_error", "qx_newton_config_default",

@@ -3,10 +3,13 @@
 // Reference: quasix_core/src/selfenergy/correlation_ac.rs:376-401 inverts eps with
 // LAPACK DPOTRF+DPOTRI (``invc``), falls back to LU (``inv``), then to the identity.
 // Here the same chain runs on the GPU for all frequencies at once:
-//   * right-looking blocked Cholesky, block NB = 64: a one-workgroup diagonal factor that
-//     also inverts the diagonal block, a panel product with that inverse, and a batched
-//     MFMA rank-64 update of the trailing lower triangle;
-//   * X = L^-1 by block forward substitution (one launch per block row, MFMA tiles);
+//   * left-looking blocked Cholesky, block NB = 64, with X = L^-1 formed in the same step
+//     loop.  Step j: one MFMA launch updates block column j of A from all finished columns
+//     (A_ij -= L_i,<j L_j,<j^T) and forms the block-row sums S_jk = L_j,k:j X_k:j,k of the
+//     inverse (long K, pipelined); a one-wavefront kernel factors and inverts A_jj; one
+//     launch forms the panel L_ij = A_ij Linv_jj^T and X_jk = -Linv_jj S_jk.  Only block
+//     column j of A and block row j of X are written per step, so the HBM traffic is the
+//     left-looking reads, not a rewrite of the trailing matrix every step;
 //   * Gauss-Jordan with partial pivoting (one workgroup per matrix) for any frequency
 //     whose Cholesky hit a non-positive pivot; a singular matrix gives Wc = 0.
 // The downstream contraction needs only X (Wc_nm = |X b|^2 - |b|^2), so DPOTRI's
@@ -59,130 +62,174 @@ __device__ __forceinline__ void for_acc(const d4 (&acc)[4], int wave, int lane, 
             fn(wave * 16 + mfma_row_f64(lane, v), bj * 16 + (lane & 15), acc[bj][v]);
 }
 
-__global__ __launch_bounds__(256) void potrf_diag_kernel(double* __restrict__ a,
-                                                         double* __restrict__ linv_diag,
-                                                         int npad, int j, int* __restrict__ fail) {
-    __shared__ double L[NB][P];
-    __shared__ double Li[NB][P];
-    __shared__ int bad;
-    const int f = blockIdx.x, tid = threadIdx.x;
+// Diagonal block j: L_jj = chol(A_jj) and Linv_jj, one wavefront per frequency.
+// Factorisation: lane r keeps row r in registers; at step k the raw column k is broadcast
+// through LDS once (all lanes read the same address, so the reads are broadcasts) and every
+// lane updates r[c] -= (r[k] / d) * col[c].  Inverse: L is staged in LDS and lane c forms
+// column c of Linv by forward substitution with broadcast reads of the rows of L (two
+// partial sums break the FMA dependency chain).  Loops are fully unrolled, so the register
+// arrays are statically indexed.  Linv_jj is written to linv_diag and to X_jj.
+__global__ __launch_bounds__(64) void potrf_diag_kernel(double* __restrict__ a,
+                                                        double* __restrict__ x,
+                                                        double* __restrict__ linv_diag,
+                                                        int npad, int j, int* __restrict__ fail) {
+    __shared__ double col[NB];
+    __shared__ double Ls[NB][P];
+    const int f = blockIdx.x, lane = threadIdx.x;
     const int nblk = npad / NB;
-    double* A = a + (size_t)f * npad * npad + (size_t)j * NB * npad + (size_t)j * NB;
-    if (tid == 0) bad = 0;
-    for (int idx = tid; idx < NB * NB; idx += 256) {
-        const int r = idx / NB, c = idx % NB;
-        L[r][c] = c <= r ? A[(size_t)r * npad + c] : 0.0;
-    }
-    __syncthreads();
-    for (int k = 0; k < NB; ++k) {
-        if (tid == 0) {
-            double d = L[k][k];
-            if (!(d > 0.0)) { bad = 1; d = 1.0; }
-            L[k][k] = sqrt(d);
+    const size_t off = (size_t)f * npad * npad + (size_t)j * NB * npad + (size_t)j * NB;
+    double* A = a + off;
+    double* X = x + off;
+    bool bad = false;
+    {
+        double r[NB];
+        #pragma unroll
+        for (int c = 0; c < NB; ++c) r[c] = A[(size_t)lane * npad + c];
+        #pragma unroll
+        for (int k = 0; k < NB; ++k) {
+            col[lane] = r[k];                      // raw column k (rows >= k meaningful)
+            __syncthreads();
+            double d = col[k];
+            if (!(d > 0.0)) { bad = true; d = 1.0; }
+            const double sd = sqrt(d);
+            const double t = r[k] / d;             // L[lane][k] / L[k][k]
+            #pragma unroll
+            for (int c = k + 1; c < NB; ++c) r[c] -= t * col[c];
+            r[k] = lane == k ? sd : r[k] / sd;
+            __syncthreads();
         }
-        __syncthreads();
-        if (tid > k && tid < NB) L[tid][k] /= L[k][k];
-        __syncthreads();
-        for (int idx = tid; idx < NB * NB; idx += 256) {
-            const int r = idx / NB, c = idx % NB;
-            if (c > k && c <= r) L[r][c] -= L[r][k] * L[c][k];
-        }
-        __syncthreads();
-    }
-    // Inverse of the lower-triangular block: thread c owns column c.
-    if (tid < NB) {
-        const int c = tid;
-        for (int r = 0; r < c; ++r) Li[r][c] = 0.0;
-        Li[c][c] = 1.0 / L[c][c];
-        for (int r = c + 1; r < NB; ++r) {
-            double sum = 0.0;
-            for (int k = c; k < r; ++k) sum += L[r][k] * Li[k][c];
-            Li[r][c] = -sum / L[r][r];
-        }
+        #pragma unroll
+        for (int c = 0; c < NB; ++c) Ls[lane][c] = c <= lane ? r[c] : 0.0;
     }
     __syncthreads();
     double* D = linv_diag + ((size_t)f * nblk + j) * NB * NB;
-    for (int idx = tid; idx < NB * NB; idx += 256) {
-        const int r = idx / NB, c = idx % NB;
-        A[(size_t)r * npad + c] = c <= r ? L[r][c] : 0.0;
-        D[idx] = Li[r][c];
+    for (int r = 0; r < NB; ++r) A[(size_t)r * npad + lane] = Ls[r][lane];
+    {
+        double li[NB];
+        #pragma unroll
+        for (int q = 0; q < NB; ++q) {
+            double s0 = lane == q ? 1.0 : 0.0, s1 = 0.0;
+            #pragma unroll
+            for (int k = 0; k < q; k += 2) {
+                s0 -= Ls[q][k] * li[k];
+                if (k + 1 < q) s1 -= Ls[q][k + 1] * li[k + 1];
+            }
+            li[q] = (s0 + s1) / Ls[q][q];
+        }
+        #pragma unroll
+        for (int q = 0; q < NB; ++q) {         // li[q] = Linv[q][lane]
+            X[(size_t)q * npad + lane] = li[q];
+            D[q * NB + lane] = li[q];
+        }
     }
-    if (tid == 0 && bad) fail[f] = 1;
+    if (__any(bad) && lane == 0) fail[f] = 1;
 }
 
-// L[i][j] = A[i][j] * Linv_jj^T for the block rows below the diagonal block j.
-__global__ __launch_bounds__(256) void potrf_panel_kernel(double* __restrict__ a,
-                                                          const double* __restrict__ linv_diag,
-                                                          int npad, int j) {
+// After the diagonal factor of step j:
+//   blockIdx.x <  j : finish block (j, bx) of X = L^-1:  X_j,bx = -Linv_jj S_j,bx
+//   blockIdx.x >= j : panel block i = j+1+(bx-j):         L_ij  =  A_ij Linv_jj^T
+__global__ __launch_bounds__(256) void potrf_rowpanel_kernel(double* __restrict__ a,
+                                                             double* __restrict__ x,
+                                                             const double* __restrict__ linv_diag,
+                                                             int npad, int j) {
     __shared__ double As[NB][P];
     __shared__ double Ls[NB][P];
-    const int i = j + 1 + blockIdx.x, f = blockIdx.y;
+    const int bx = blockIdx.x, f = blockIdx.y;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int nblk = npad / NB;
-    double* A = a + (size_t)f * npad * npad + (size_t)i * NB * npad + (size_t)j * NB;
-    load_block(As, A, npad, tid);
-    load_block(Ls, linv_diag + ((size_t)f * nblk + j) * NB * NB, NB, tid);
-    __syncthreads();
+    const double* Djj = linv_diag + ((size_t)f * nblk + j) * NB * NB;
+    load_block(Ls, Djj, NB, tid);
     d4 acc[4] = {d4{0, 0, 0, 0}, d4{0, 0, 0, 0}, d4{0, 0, 0, 0}, d4{0, 0, 0, 0}};
-    mma_64x64<true>(acc, As, Ls, wave, lane);
-    for_acc(acc, wave, lane, [&](int r, int c, double v) { A[(size_t)r * npad + c] = v; });
+    if (bx < j) {
+        double* S = x + (size_t)f * npad * npad + (size_t)j * NB * npad + (size_t)bx * NB;
+        load_block(As, S, npad, tid);
+        __syncthreads();
+        mma_64x64<false>(acc, Ls, As, wave, lane);          // Linv_jj * S
+        for_acc(acc, wave, lane, [&](int r, int c, double v) { S[(size_t)r * npad + c] = -v; });
+    } else {
+        const int i = j + 1 + (bx - j);
+        double* A = a + (size_t)f * npad * npad + (size_t)i * NB * npad + (size_t)j * NB;
+        load_block(As, A, npad, tid);
+        __syncthreads();
+        mma_64x64<true>(acc, As, Ls, wave, lane);            // A_ij * Linv_jj^T
+        for_acc(acc, wave, lane, [&](int r, int c, double v) { A[(size_t)r * npad + c] = v; });
+    }
 }
 
-// A[i][k] -= L[i][j] L[k][j]^T for all trailing lower-triangle blocks j < k <= i.
-__global__ __launch_bounds__(256) void potrf_update_kernel(double* __restrict__ a, int npad,
-                                                           int j) {
-    __shared__ double Pi_[NB][P];
-    __shared__ double Pk[NB][P];
-    const int f = blockIdx.y, t = blockIdx.x;
-    int ii = (int)((sqrtf(8.0f * (float)t + 1.0f) - 1.0f) * 0.5f);
-    while ((ii + 1) * (ii + 2) / 2 <= t) ++ii;
-    while (ii * (ii + 1) / 2 > t) --ii;
-    const int kk = t - ii * (ii + 1) / 2;
-    const int i = j + 1 + ii, k = j + 1 + kk;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    double* base = a + (size_t)f * npad * npad;
-    load_block(Pi_, base + (size_t)i * NB * npad + (size_t)j * NB, npad, tid);
-    load_block(Pk, base + (size_t)k * NB * npad + (size_t)j * NB, npad, tid);
-    __syncthreads();
-    d4 acc[4] = {d4{0, 0, 0, 0}, d4{0, 0, 0, 0}, d4{0, 0, 0, 0}, d4{0, 0, 0, 0}};
-    mma_64x64<true>(acc, Pi_, Pk, wave, lane);
-    double* C = base + (size_t)i * NB * npad + (size_t)k * NB;
-    for_acc(acc, wave, lane, [&](int r, int c, double v) { C[(size_t)r * npad + c] -= v; });
-}
-
-// Block row i of X = L^-1:  X_ii = Linv_ii,  X_ij = -Linv_ii * sum_{j<=k<i} L_ik X_kj.
-__global__ __launch_bounds__(256) void trtri_row_kernel(const double* __restrict__ l,
-                                                        const double* __restrict__ linv_diag,
-                                                        double* __restrict__ x, int npad, int i) {
-    __shared__ double As[NB][P];
-    __shared__ double Bs[NB][P];
-    const int jb = blockIdx.x, f = blockIdx.y;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+// Left-looking update of step j, one 64x64 output tile per workgroup, K streamed through
+// LDS in chunks of 32 with the next chunk's global loads in flight during the MFMAs:
+//   bx <  nblk - j : Cholesky tile i = j + bx:   A_ij -= sum_{m<j} L_im L_jm^T   (K = 64 j)
+//   bx >= nblk - j : inverse sums k = bx - (nblk - j) < j:
+//                    X_jk := S_jk = sum_{k<=m<j} L_jm X_mk                    (K = 64 (j - k))
+constexpr int GK = 32;
+__global__ __launch_bounds__(256) void potrf_colupdate_kernel(double* __restrict__ a,
+                                                              double* __restrict__ x, int npad,
+                                                              int j) {
+    __shared__ double As[NB][GK + 1];        // [row][k]
+    __shared__ double Bs[NB][GK + 1];        // NT: [col][k];  NN: [k][col] (first GK rows)
+    const int bx = blockIdx.x, f = blockIdx.y;
     const int nblk = npad / NB;
-    const double* L = l + (size_t)f * npad * npad;
-    double* X = x + (size_t)f * npad * npad;
-    const double* Dii = linv_diag + ((size_t)f * nblk + i) * NB * NB;
-    double* out = X + (size_t)i * NB * npad + (size_t)jb * NB;
-    if (jb == i) {
-        for (int idx = tid; idx < NB * NB; idx += 256)
-            out[(size_t)(idx / NB) * npad + idx % NB] = Dii[idx];
-        return;
-    }
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    double* Af = a + (size_t)f * npad * npad;
+    double* Xf = x + (size_t)f * npad * npad;
+    const bool chol = bx < nblk - j;
+    const int i = chol ? j + bx : j;                 // output block row
+    const int k = chol ? j : bx - (nblk - j);        // output block column
+    const int kbeg = chol ? 0 : k * NB, kend = j * NB;
+    // A operand rows: L[i-block][kbeg:kend]; B operand: NT -> L[j-block][kbeg:kend] (rows),
+    // NN -> X[kbeg:kend][k-block].
+    const double* arow = Af + (size_t)i * NB * npad;
+    const double* brow = chol ? Af + (size_t)j * NB * npad : Xf + (size_t)k * NB;
+    // staging: 64 rows x 32 k per operand = 2048 doubles, 8 per thread
+    double ra[8], rb[8];
+    auto fetch = [&](int k0) {
+        #pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int idx = tid + 256 * q;
+            const int rr = idx / GK, kk = idx % GK;      // rows of A / NT-B
+            ra[q] = arow[(size_t)rr * npad + k0 + kk];
+            if (chol) rb[q] = brow[(size_t)rr * npad + k0 + kk];
+            else {
+                const int kr = idx / NB, cc = idx % NB;  // NN-B: row k0+kr, column cc
+                rb[q] = brow[(size_t)(k0 + kr) * npad + cc];
+            }
+        }
+    };
     d4 acc[4] = {d4{0, 0, 0, 0}, d4{0, 0, 0, 0}, d4{0, 0, 0, 0}, d4{0, 0, 0, 0}};
-    for (int k = jb; k < i; ++k) {
-        load_block(As, L + (size_t)i * NB * npad + (size_t)k * NB, npad, tid);
-        load_block(Bs, X + (size_t)k * NB * npad + (size_t)jb * NB, npad, tid);
+    if (kbeg < kend) fetch(kbeg);
+    for (int k0 = kbeg; k0 < kend; k0 += GK) {
+        #pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int idx = tid + 256 * q;
+            As[idx / GK][idx % GK] = ra[q];
+            if (chol) Bs[idx / GK][idx % GK] = rb[q];
+            else reinterpret_cast<double*>(Bs)[(idx / NB) * (NB + 1) + idx % NB] = rb[q];
+        }
         __syncthreads();
-        mma_64x64<false>(acc, As, Bs, wave, lane);
+        if (k0 + GK < kend) fetch(k0 + GK);
+        const int rr = wave * 16 + (lane & 15);
+        const double* BsNN = reinterpret_cast<const double*>(Bs);
+        #pragma unroll
+        for (int kk = 0; kk < GK; kk += 4) {
+            const int q = kk + (lane >> 4);
+            const double av = As[rr][q];
+            #pragma unroll
+            for (int bj = 0; bj < 4; ++bj) {
+                const int c = bj * 16 + (lane & 15);
+                const double bv = chol ? Bs[c][q] : BsNN[q * (NB + 1) + c];
+                acc[bj] = mfma64(av, bv, acc[bj]);
+            }
+        }
         __syncthreads();
     }
-    // S -> Bs, Linv_ii -> As, then X_ij = -Linv_ii S
-    for_acc(acc, wave, lane, [&](int r, int c, double v) { Bs[r][c] = v; });
-    load_block(As, Dii, NB, tid);
-    __syncthreads();
-    d4 acc2[4] = {d4{0, 0, 0, 0}, d4{0, 0, 0, 0}, d4{0, 0, 0, 0}, d4{0, 0, 0, 0}};
-    mma_64x64<false>(acc2, As, Bs, wave, lane);
-    for_acc(acc2, wave, lane, [&](int r, int c, double v) { out[(size_t)r * npad + c] = -v; });
+    if (chol) {
+        if (j == 0) return;
+        double* C = Af + (size_t)i * NB * npad + (size_t)j * NB;
+        for_acc(acc, wave, lane, [&](int r, int c, double v) { C[(size_t)r * npad + c] -= v; });
+    } else {
+        double* C = Xf + (size_t)j * NB * npad + (size_t)k * NB;
+        for_acc(acc, wave, lane, [&](int r, int c, double v) { C[(size_t)r * npad + c] = v; });
+    }
 }
 
 // Wc = X^T X - I over the first naux rows/cols, full symmetric output [nf][naux][naux].
@@ -301,29 +348,20 @@ __global__ void gj_finish_kernel(const double* __restrict__ work, double* __rest
 
 }  // namespace
 
-hipError_t launch_potrf_batched(double* a, double* linv_diag, int npad, int nf, int* fail,
-                                hipStream_t st) {
+hipError_t launch_potrf_batched(double* a, double* x, double* linv_diag, int npad, int nf,
+                                int* fail, hipStream_t st) {
     const int nblk = npad / NB;
     for (int j = 0; j < nblk; ++j) {
-        hipLaunchKernelGGL(potrf_diag_kernel, dim3(nf), dim3(256), 0, st, a, linv_diag, npad, j,
-                           fail);
-        const int rem = nblk - j - 1;
-        if (rem > 0) {
-            hipLaunchKernelGGL(potrf_panel_kernel, dim3(rem, nf), dim3(256), 0, st, a, linv_diag,
+        if (j > 0)
+            hipLaunchKernelGGL(potrf_colupdate_kernel, dim3(nblk, nf), dim3(256), 0, st, a, x,
                                npad, j);
-            hipLaunchKernelGGL(potrf_update_kernel, dim3(rem * (rem + 1) / 2, nf), dim3(256), 0,
-                               st, a, npad, j);
-        }
+        hipLaunchKernelGGL(potrf_diag_kernel, dim3(nf), dim3(64), 0, st, a, x, linv_diag, npad, j,
+                           fail);
+        const int T = nblk - j - 1;
+        if (j + T > 0)
+            hipLaunchKernelGGL(potrf_rowpanel_kernel, dim3(j + T, nf), dim3(256), 0, st, a, x,
+                               linv_diag, npad, j);
     }
-    return hipGetLastError();
-}
-
-hipError_t launch_trtri_batched(const double* l, const double* linv_diag, double* x, int npad,
-                                int nf, hipStream_t st) {
-    const int nblk = npad / NB;
-    for (int i = 0; i < nblk; ++i)
-        hipLaunchKernelGGL(trtri_row_kernel, dim3(i + 1, nf), dim3(256), 0, st, l, linv_diag, x,
-                           npad, i);
     return hipGetLastError();
 }
 

@@ -13,7 +13,9 @@
 //
 // Precisions: FP64 (v_mfma_f64_16x16x4f64), FP32 (v_mfma_f32_16x16x4f32, FP32 accumulate)
 // and BF16x3 (B = hi + lo, S B = hi + lo, products hi*hi + hi*lo + lo*hi on
-// v_mfma_f32_16x16x32_bf16 with FP32 accumulation, flushed to FP64 every kFlush rows).
+// v_mfma_f32_16x16x32_bf16 with FP32 accumulation).  Each K-chunk's global loads are issued
+// into registers before the MFMA phase of the previous chunk, and 16x16 blocks outside naux
+// or above the diagonal are skipped.
 #include "qx_internal.h"
 
 namespace qx {
@@ -55,7 +57,7 @@ __device__ __forceinline__ float bf16_to_f32(unsigned short h) {
 // Weights s[f][r] = 4 e_r / (w_f^2 + e_r^2); rows r >= nov (padding) get 0.
 __global__ void pi_weights_kernel(const double* __restrict__ eps, int nocc, int nvir, int nf,
                                   const double* __restrict__ freqs, int ld,
-                                  double* __restrict__ s) {
+                                  double* __restrict__ s, float* __restrict__ s32) {
     const int nov = nocc * nvir;
     const int f = blockIdx.y;
     const double w = freqs[f];
@@ -67,6 +69,7 @@ __global__ void pi_weights_kernel(const double* __restrict__ eps, int nocc, int 
             v = 4.0 * e / (w * w + e * e);
         }
         s[(size_t)f * ld + r] = v;
+        if (s32) s32[(size_t)f * ld + r] = (float)v;
     }
 }
 
@@ -117,8 +120,15 @@ __device__ __forceinline__ void store_tile(const typename Mfma<T>::acc_t (&acc)[
             }
 }
 
+// Which 16x16 MFMA blocks of this wave hold entries of the lower triangle inside naux.
+__device__ __forceinline__ bool block_live(int I0, int J0, int wr, int wc, int bi, int bj,
+                                           int naux) {
+    const int r = I0 + wr * 64 + bi * 16, c = J0 + wc * 64 + bj * 16;
+    return r < naux && c < naux && (I0 != J0 || r >= c);
+}
+
 template <typename T, int KC>
-__global__ __launch_bounds__(256) void pi_syrk_kernel(
+__global__ __launch_bounds__(256, 2) void pi_syrk_kernel(
     const double* __restrict__ B, const double* __restrict__ s, int nov, int naux, int npad,
     int nf, int ntiles, int sld, double* __restrict__ out, PiOut mode) {
     __shared__ T As[KC][TILE + 2];
@@ -128,9 +138,30 @@ __global__ __launch_bounds__(256) void pi_syrk_kernel(
     int I, J;
     tri_index(tile, I, J);
     const int I0 = I * TILE, J0 = J * TILE;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);   // wave-uniform (scalar)
     const int wr = wave >> 1, wc = wave & 1;
     const double* sf = s + (size_t)f * sld;
+
+    // Staging: thread owns column c of the tile and rows kr0 + 2q of each chunk.
+    constexpr int PER = KC * TILE / 256;
+    const int c = tid % TILE, kr0 = tid / TILE;
+    const bool ca = I0 + c < naux, cb = J0 + c < naux;
+    double ra[PER], rb[PER];
+    auto fetch = [&](int r0) {
+        #pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            const int r = r0 + kr0 + 2 * q;
+            double a = 0.0, b = 0.0;
+            if (r < nov) {
+                const double* row = B + (size_t)r * naux;
+                if (ca) a = row[I0 + c];
+                if (cb) b = row[J0 + c] * sf[r];
+            }
+            ra[q] = a;
+            rb[q] = b;
+        }
+    };
 
     typename Mfma<T>::acc_t acc[4][4];
     #pragma unroll
@@ -138,20 +169,20 @@ __global__ __launch_bounds__(256) void pi_syrk_kernel(
         #pragma unroll
         for (int bj = 0; bj < 4; ++bj) acc[bi][bj] = typename Mfma<T>::acc_t{0, 0, 0, 0};
 
+    bool all_live = true;
+    #pragma unroll
+    for (int bi = 0; bi < 4; ++bi)
+        #pragma unroll
+        for (int bj = 0; bj < 4; ++bj) all_live &= block_live(I0, J0, wr, wc, bi, bj, naux);
+    fetch(0);
     for (int r0 = 0; r0 < nov; r0 += KC) {
-        for (int idx = tid; idx < KC * TILE; idx += 256) {
-            const int kr = idx / TILE, c = idx % TILE;
-            const int r = r0 + kr;
-            double a = 0.0, b = 0.0;
-            if (r < nov) {
-                const double* row = B + (size_t)r * naux;
-                if (I0 + c < naux) a = row[I0 + c];
-                if (J0 + c < naux) b = row[J0 + c] * sf[r];
-            }
-            As[kr][c] = (T)a;
-            Bs[kr][c] = (T)b;
+        #pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            As[kr0 + 2 * q][c] = (T)ra[q];
+            Bs[kr0 + 2 * q][c] = (T)rb[q];
         }
         __syncthreads();
+        if (r0 + KC < nov) fetch(r0 + KC);      // in flight during the MFMA phase
         #pragma unroll
         for (int kk = 0; kk < KC; kk += 4) {
             const int k = kk + (lane >> 4);
@@ -161,10 +192,20 @@ __global__ __launch_bounds__(256) void pi_syrk_kernel(
                 av[q] = As[k][wr * 64 + q * 16 + (lane & 15)];
                 bv[q] = Bs[k][wc * 64 + q * 16 + (lane & 15)];
             }
-            #pragma unroll
-            for (int bi = 0; bi < 4; ++bi)
+            if (all_live) {
                 #pragma unroll
-                for (int bj = 0; bj < 4; ++bj) acc[bi][bj] = Mfma<T>::run(av[bi], bv[bj], acc[bi][bj]);
+                for (int bi = 0; bi < 4; ++bi)
+                    #pragma unroll
+                    for (int bj = 0; bj < 4; ++bj)
+                        acc[bi][bj] = Mfma<T>::run(av[bi], bv[bj], acc[bi][bj]);
+            } else {
+                #pragma unroll
+                for (int bi = 0; bi < 4; ++bi)
+                    #pragma unroll
+                    for (int bj = 0; bj < 4; ++bj)
+                        if (block_live(I0, J0, wr, wc, bi, bj, naux))
+                            acc[bi][bj] = Mfma<T>::run(av[bi], bv[bj], acc[bi][bj]);
+            }
         }
         __syncthreads();
     }
@@ -174,9 +215,33 @@ __global__ __launch_bounds__(256) void pi_syrk_kernel(
 // ---- BF16x3 kernel ----------------------------------------------------------------------
 // Operands are staged K-contiguous: LDS rows are auxiliary indices, 32 N_ov rows per
 // chunk, so each lane fetches its 8 consecutive k values with one 16-byte read.
-constexpr int KB = 32;
-constexpr int KBP = 40;          // padded LDS pitch (bf16 elements): conflict-free rows
-constexpr int kFlush = 2048;     // FP32 partial sums are folded into FP64 every kFlush rows
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+
+// The three split products over a wave's 4x4 blocks, product-outer so that 16 independent
+// MFMAs separate dependent accumulations.  Guarded: skip blocks outside the triangle/naux.
+template <bool Guarded>
+__device__ __forceinline__ void mfma_bf16x3(f4 (&acc)[4][4], const bf16x8 (&ah)[4],
+                                            const bf16x8 (&al)[4], const bf16x8 (&ch)[4],
+                                            const bf16x8 (&cl)[4], const bool (&live)[4][4]) {
+    #pragma unroll
+    for (int bi = 0; bi < 4; ++bi)
+        #pragma unroll
+        for (int bj = 0; bj < 4; ++bj)
+            if (!Guarded || live[bi][bj])
+                acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[bi], ch[bj], acc[bi][bj], 0, 0, 0);
+    #pragma unroll
+    for (int bi = 0; bi < 4; ++bi)
+        #pragma unroll
+        for (int bj = 0; bj < 4; ++bj)
+            if (!Guarded || live[bi][bj])
+                acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[bi], cl[bj], acc[bi][bj], 0, 0, 0);
+    #pragma unroll
+    for (int bi = 0; bi < 4; ++bi)
+        #pragma unroll
+        for (int bj = 0; bj < 4; ++bj)
+            if (!Guarded || live[bi][bj])
+                acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[bi], ch[bj], acc[bi][bj], 0, 0, 0);
+}
 
 __global__ void pi_prepare_mixed_kernel(const double* __restrict__ B, int nov, int naux,
                                         int npad, int kpad, uint16_t* __restrict__ hi,
@@ -205,66 +270,89 @@ __global__ void pi_prepare_mixed_kernel(const double* __restrict__ B, int nov, i
     }
 }
 
-__global__ __launch_bounds__(256) void pi_syrk_bf16x3_kernel(
+// KB: N_ov rows per LDS chunk (32 or 64).  Staging map: each thread owns R = KB/16 tile
+// rows and 8 consecutive k of each; a row's KB values are read by KB/8 consecutive threads
+// (coalesced), and a thread needs only its 8 frequency weights per chunk.
+template <int KB>
+__global__ __launch_bounds__(256, 2) void pi_syrk_bf16x3_kernel(
     const uint16_t* __restrict__ bt_hi, const uint16_t* __restrict__ bt_lo,
-    const float* __restrict__ bt_f32, const double* __restrict__ s, int kpad, int naux,
-    int npad, int nf, int ntiles, int sld, double* __restrict__ out, PiOut mode) {
-    __shared__ __attribute__((aligned(16))) unsigned short Ah[TILE][KBP];
-    __shared__ __attribute__((aligned(16))) unsigned short Al[TILE][KBP];
-    __shared__ __attribute__((aligned(16))) unsigned short Ch[TILE][KBP];
-    __shared__ __attribute__((aligned(16))) unsigned short Cl[TILE][KBP];
-    __shared__ float sk[KB];
+    const float* __restrict__ bt_f32, const float* __restrict__ s32, int kpad, int naux,
+    int npad, int nf, int ntiles, double* __restrict__ out, PiOut mode) {
+    constexpr int KBP = KB + 8;                  // padded LDS pitch: conflict-free rows
+    constexpr int TPR = KB / 8;                  // threads per row
+    constexpr int RSTEP = 256 / TPR;             // row stride between a thread's rows
+    constexpr int R = TILE / RSTEP;              // rows per thread
+    __shared__ __attribute__((aligned(16))) __bf16 Ah[TILE][KBP];
+    __shared__ __attribute__((aligned(16))) __bf16 Al[TILE][KBP];
+    __shared__ __attribute__((aligned(16))) __bf16 Ch[TILE][KBP];
+    __shared__ __attribute__((aligned(16))) __bf16 Cl[TILE][KBP];
     const int work = xcd_remap(blockIdx.x, ntiles * nf);
     const int tile = work / nf, f = work % nf;
     int I, J;
     tri_index(tile, I, J);
     const int I0 = I * TILE, J0 = J * TILE;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);   // wave-uniform (scalar)
     const int wr = wave >> 1, wc = wave & 1;
-    const double* sf = s + (size_t)f * sld;
+    const float* sf = s32 + (size_t)f * kpad;
+
+    const int row0 = tid / TPR, kq0 = (tid % TPR) * 8;
+    u16x8 rh[R], rl[R];
+    f4 rc[2 * R], rs[2];
+    auto fetch = [&](int k0) {
+        #pragma unroll
+        for (int q = 0; q < R; ++q) {
+            const int row = row0 + q * RSTEP;
+            const bool va = I0 + row < npad, vc = J0 + row < npad;
+            const size_t oa = (size_t)(I0 + row) * kpad + k0 + kq0;
+            const size_t oc = (size_t)(J0 + row) * kpad + k0 + kq0;
+            rh[q] = va ? *reinterpret_cast<const u16x8*>(bt_hi + oa) : u16x8{0, 0, 0, 0, 0, 0, 0, 0};
+            rl[q] = va ? *reinterpret_cast<const u16x8*>(bt_lo + oa) : u16x8{0, 0, 0, 0, 0, 0, 0, 0};
+            rc[2 * q] = vc ? *reinterpret_cast<const f4*>(bt_f32 + oc) : f4{0.f, 0.f, 0.f, 0.f};
+            rc[2 * q + 1] = vc ? *reinterpret_cast<const f4*>(bt_f32 + oc + 4) : f4{0.f, 0.f, 0.f, 0.f};
+        }
+        rs[0] = *reinterpret_cast<const f4*>(sf + k0 + kq0);
+        rs[1] = *reinterpret_cast<const f4*>(sf + k0 + kq0 + 4);
+    };
 
     f4 acc[4][4];
-    d4 acc64[4][4];
     #pragma unroll
     for (int bi = 0; bi < 4; ++bi)
         #pragma unroll
-        for (int bj = 0; bj < 4; ++bj) {
-            acc[bi][bj] = f4{0.f, 0.f, 0.f, 0.f};
-            acc64[bi][bj] = d4{0.0, 0.0, 0.0, 0.0};
-        }
+        for (int bj = 0; bj < 4; ++bj) acc[bi][bj] = f4{0.f, 0.f, 0.f, 0.f};
+    bool live[4][4];
+    #pragma unroll
+    for (int bi = 0; bi < 4; ++bi)
+        #pragma unroll
+        for (int bj = 0; bj < 4; ++bj) live[bi][bj] = block_live(I0, J0, wr, wc, bi, bj, naux);
+    bool all_live = true;
+    #pragma unroll
+    for (int bi = 0; bi < 4; ++bi)
+        #pragma unroll
+        for (int bj = 0; bj < 4; ++bj) all_live &= live[bi][bj];
 
+    fetch(0);
     for (int k0 = 0; k0 < kpad; k0 += KB) {
-        if (tid < KB) sk[tid] = (float)sf[k0 + tid];
-        // A operand (unscaled, pre-split): 128 rows x 32 k, 8 bf16 per thread per part.
-        for (int idx = tid; idx < TILE * (KB / 8); idx += 256) {
-            const int row = idx / (KB / 8), kq = (idx % (KB / 8)) * 8;
-            const int c = I0 + row;
-            u16x8 h = u16x8{0, 0, 0, 0, 0, 0, 0, 0}, l = h;
-            if (c < npad) {
-                h = *reinterpret_cast<const u16x8*>(bt_hi + (size_t)c * kpad + k0 + kq);
-                l = *reinterpret_cast<const u16x8*>(bt_lo + (size_t)c * kpad + k0 + kq);
-            }
-            *reinterpret_cast<u16x8*>(&Ah[row][kq]) = h;
-            *reinterpret_cast<u16x8*>(&Al[row][kq]) = l;
-        }
-        __syncthreads();   // sk visible
-        // C operand = s * B, split on the fly.
-        for (int idx = tid; idx < TILE * (KB / 4); idx += 256) {
-            const int row = idx / (KB / 4), kq = (idx % (KB / 4)) * 4;
-            const int c = J0 + row;
-            float4 x = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (c < npad) x = *reinterpret_cast<const float4*>(bt_f32 + (size_t)c * kpad + k0 + kq);
-            const float v[4] = {x.x * sk[kq], x.y * sk[kq + 1], x.z * sk[kq + 2], x.w * sk[kq + 3]};
+        #pragma unroll
+        for (int q = 0; q < R; ++q) {
+            const int row = row0 + q * RSTEP;
+            *reinterpret_cast<u16x8*>(&Ah[row][kq0]) = rh[q];
+            *reinterpret_cast<u16x8*>(&Al[row][kq0]) = rl[q];
+            // C = s * B split into hi + lo on the fly (packed hardware conversions)
             #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const unsigned short h = f32_to_bf16_rne(v[q]);
-                Ch[row][kq + q] = h;
-                Cl[row][kq + q] = f32_to_bf16_rne(v[q] - bf16_to_f32(h));
+            for (int hq = 0; hq < 2; ++hq) {
+                const f4 v = rc[2 * q + hq] * rs[hq];
+                const bf16x4 h = __builtin_convertvector(v, bf16x4);
+                const bf16x4 l = __builtin_convertvector(v - __builtin_convertvector(h, f4), bf16x4);
+                *reinterpret_cast<bf16x4*>(&Ch[row][kq0 + 4 * hq]) = h;
+                *reinterpret_cast<bf16x4*>(&Cl[row][kq0 + 4 * hq]) = l;
             }
         }
         __syncthreads();
-        {
-            const int kq = (lane >> 4) * 8, r16 = lane & 15;
+        if (k0 + KB < kpad) fetch(k0 + KB);      // in flight during the MFMA phase
+        #pragma unroll
+        for (int ks = 0; ks < KB; ks += 32) {
+            const int kq = ks + (lane >> 4) * 8, r16 = lane & 15;
             bf16x8 ah[4], al[4], ch[4], cl[4];
             #pragma unroll
             for (int q = 0; q < 4; ++q) {
@@ -273,40 +361,161 @@ __global__ __launch_bounds__(256) void pi_syrk_bf16x3_kernel(
                 ch[q] = *reinterpret_cast<const bf16x8*>(&Ch[wc * 64 + q * 16 + r16][kq]);
                 cl[q] = *reinterpret_cast<const bf16x8*>(&Cl[wc * 64 + q * 16 + r16][kq]);
             }
-            #pragma unroll
-            for (int bi = 0; bi < 4; ++bi)
-                #pragma unroll
-                for (int bj = 0; bj < 4; ++bj) {
-                    f4 c = acc[bi][bj];
-                    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[bi], ch[bj], c, 0, 0, 0);
-                    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[bi], cl[bj], c, 0, 0, 0);
-                    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[bi], ch[bj], c, 0, 0, 0);
-                    acc[bi][bj] = c;
-                }
-        }
-        if (((k0 + KB) % kFlush) == 0 || k0 + KB >= kpad) {
-            #pragma unroll
-            for (int bi = 0; bi < 4; ++bi)
-                #pragma unroll
-                for (int bj = 0; bj < 4; ++bj) {
-                    #pragma unroll
-                    for (int v = 0; v < 4; ++v) acc64[bi][bj][v] += (double)acc[bi][bj][v];
-                    acc[bi][bj] = f4{0.f, 0.f, 0.f, 0.f};
-                }
+            // product-outer order: 16 independent MFMAs between dependent accumulations
+            if (all_live) mfma_bf16x3<false>(acc, ah, al, ch, cl, live);
+            else mfma_bf16x3<true>(acc, ah, al, ch, cl, live);
         }
         __syncthreads();
     }
-    store_tile<double, false>(acc64, I0, J0, wr, wc, lane, npad, naux, f, mode, out, 1.0);
+    store_tile<float, false>(acc, I0, J0, wr, wc, lane, npad, naux, f, mode, out, 1.0);
+}
+
+// Wide variant: 256 (A, unscaled) x 128 (C = s B, split on the fly) tile per 512-thread
+// workgroup, 8 waves of 64x64.  Converting only the narrow side halves the split work per
+// MFMA.  LDS rows are 64 bf16 (8 chunks of 16 B) with the chunk index XOR-swizzled by
+// (row / 2) % 8, which keeps both the staging writes and the MFMA fragment reads free of
+// bank conflicts.  Operand rows are allocated to a multiple of 256 (zero padding), so the
+// loads need no bounds selects.
+constexpr int WTI = 256, WTJ = 128, WKB = 64;
+
+__device__ __forceinline__ int swz(int row, int chunk) { return chunk ^ ((row >> 1) & 7); }
+
+__global__ __launch_bounds__(512, 1) void pi_syrk_bf16x3_wide_kernel(
+    const uint16_t* __restrict__ bt_hi, const uint16_t* __restrict__ bt_lo,
+    const float* __restrict__ bt_f32, const float* __restrict__ s32, int kpad, int naux,
+    int npad, int nf, int ntiles, int TJ, double* __restrict__ out, PiOut mode) {
+    __shared__ __attribute__((aligned(16))) __bf16 Ah[WTI * WKB];
+    __shared__ __attribute__((aligned(16))) __bf16 Al[WTI * WKB];
+    __shared__ __attribute__((aligned(16))) __bf16 Ch[WTJ * WKB];
+    __shared__ __attribute__((aligned(16))) __bf16 Cl[WTJ * WKB];
+    const int work = xcd_remap(blockIdx.x, ntiles * nf);
+    int tile = work / nf;
+    const int f = work % nf;
+    int a = 0;                                   // tiles of block row a: b = 0..min(2a+1, TJ-1)
+    for (;;) {
+        const int cnt = 2 * a + 2 < TJ ? 2 * a + 2 : TJ;
+        if (tile < cnt) break;
+        tile -= cnt;
+        ++a;
+    }
+    const int I0 = a * WTI, J0 = tile * WTJ;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);   // wave-uniform (scalar)
+    const int wr = wave >> 1, wc = wave & 1;
+    const float* sf = s32 + (size_t)f * kpad;
+
+    const int row0 = tid >> 3, ch0 = tid & 7;    // staging: 8 threads per row, 16 B each
+    const uint16_t* ph = bt_hi + (size_t)(I0 + row0) * kpad + ch0 * 8;
+    const uint16_t* pl = bt_lo + (size_t)(I0 + row0) * kpad + ch0 * 8;
+    const float* pc = bt_f32 + (size_t)(J0 + row0) * kpad + ch0 * 8;
+    const size_t rstep = (size_t)64 * kpad;
+    u16x8 rh[4], rl[4];
+    f4 rc[4], rs[2];
+    auto fetch = [&](int k0) {
+        #pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            rh[q] = *reinterpret_cast<const u16x8*>(ph + q * rstep + k0);
+            rl[q] = *reinterpret_cast<const u16x8*>(pl + q * rstep + k0);
+        }
+        #pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            rc[2 * q] = *reinterpret_cast<const f4*>(pc + q * rstep + k0);
+            rc[2 * q + 1] = *reinterpret_cast<const f4*>(pc + q * rstep + k0 + 4);
+        }
+        rs[0] = *reinterpret_cast<const f4*>(sf + k0 + ch0 * 8);
+        rs[1] = *reinterpret_cast<const f4*>(sf + k0 + ch0 * 8 + 4);
+    };
+
+    f4 acc[4][4];
+    bool live[4][4];
+    #pragma unroll
+    for (int bi = 0; bi < 4; ++bi)
+        #pragma unroll
+        for (int bj = 0; bj < 4; ++bj) {
+            acc[bi][bj] = f4{0.f, 0.f, 0.f, 0.f};
+            const int r = I0 + wr * 64 + bi * 16, c = J0 + wc * 64 + bj * 16;
+            live[bi][bj] = r < naux && c < naux && r >= c;
+        }
+    bool any = false, all_live = true;
+    #pragma unroll
+    for (int bi = 0; bi < 4; ++bi)
+        #pragma unroll
+        for (int bj = 0; bj < 4; ++bj) {
+            any |= live[bi][bj];
+            all_live &= live[bi][bj];
+        }
+
+    fetch(0);
+    for (int k0 = 0; k0 < kpad; k0 += WKB) {
+        #pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int R = row0 + 64 * q;
+            const int o = R * WKB + 8 * swz(R, ch0);
+            *reinterpret_cast<u16x8*>(&Ah[o]) = rh[q];
+            *reinterpret_cast<u16x8*>(&Al[o]) = rl[q];
+        }
+        #pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const int R = row0 + 64 * q;
+            const int o = R * WKB + 8 * swz(R, ch0);
+            #pragma unroll
+            for (int hq = 0; hq < 2; ++hq) {
+                const f4 v = rc[2 * q + hq] * rs[hq];
+                const bf16x4 h = __builtin_convertvector(v, bf16x4);
+                const bf16x4 l = __builtin_convertvector(v - __builtin_convertvector(h, f4), bf16x4);
+                *reinterpret_cast<bf16x4*>(&Ch[o + 4 * hq]) = h;
+                *reinterpret_cast<bf16x4*>(&Cl[o + 4 * hq]) = l;
+            }
+        }
+        __syncthreads();
+        if (k0 + WKB < kpad) fetch(k0 + WKB);
+        if (any) {
+            #pragma unroll
+            for (int ks = 0; ks < WKB; ks += 32) {
+                const int cq = ks / 8 + (lane >> 4), r16 = lane & 15;
+                bf16x8 ah[4], al[4], chv[4], clv[4];
+                #pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int Ra = wr * 64 + q * 16 + r16, Rc = wc * 64 + q * 16 + r16;
+                    const int oa = Ra * WKB + 8 * swz(Ra, cq), oc = Rc * WKB + 8 * swz(Rc, cq);
+                    ah[q] = *reinterpret_cast<const bf16x8*>(&Ah[oa]);
+                    al[q] = *reinterpret_cast<const bf16x8*>(&Al[oa]);
+                    chv[q] = *reinterpret_cast<const bf16x8*>(&Ch[oc]);
+                    clv[q] = *reinterpret_cast<const bf16x8*>(&Cl[oc]);
+                }
+                if (all_live) mfma_bf16x3<false>(acc, ah, al, chv, clv, live);
+                else mfma_bf16x3<true>(acc, ah, al, chv, clv, live);
+            }
+        }
+        __syncthreads();
+    }
+    #pragma unroll
+    for (int bi = 0; bi < 4; ++bi)
+        #pragma unroll
+        for (int bj = 0; bj < 4; ++bj)
+            #pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int row = I0 + wr * 64 + bi * 16 + mfma_row_f32(lane, v);
+                const int col = J0 + wc * 64 + bj * 16 + (lane & 15);
+                const double p = (double)acc[bi][bj][v];
+                if (mode == PiOut::EpsLower) {
+                    if (row < npad && col < npad)
+                        out[((size_t)f * npad + row) * npad + col] = (row == col ? 1.0 : 0.0) - p;
+                } else if (row < naux && col < naux && row >= col) {
+                    out[((size_t)f * npad + row) * npad + col] = p;
+                    out[((size_t)f * npad + col) * npad + row] = p;
+                }
+            }
 }
 
 }  // namespace
 
 hipError_t launch_pi_weights(const double* eps, int nocc, int nvir, int nf,
-                             const double* freqs, double* s, hipStream_t st) {
-    const int ld = round_up(nocc * nvir, 32);
+                             const double* freqs, double* s, float* s32, hipStream_t st) {
+    const int ld = round_up(nocc * nvir, KPAD);
     dim3 grid(ceil_div(ld, 256) < 1024 ? ceil_div(ld, 256) : 1024, nf);
     hipLaunchKernelGGL(pi_weights_kernel, grid, dim3(256), 0, st, eps, nocc, nvir, nf, freqs,
-                       ld, s);
+                       ld, s, s32);
     return hipGetLastError();
 }
 
@@ -321,10 +530,10 @@ hipError_t launch_pi_prepare_mixed(const double* B, int nov, int naux, int npad,
 
 hipError_t launch_pi_syrk(int precision, const double* B, const PiOperands& ops,
                           const double* s, int nov, int naux, int npad, int nf, double* out,
-                          PiOut mode, hipStream_t st) {
+                          PiOut mode, hipStream_t st, int variant) {
     const int T = ceil_div(mode == PiOut::EpsLower ? npad : naux, TILE);
     const int ntiles = T * (T + 1) / 2;
-    const int sld = round_up(nov, 32);
+    const int sld = round_up(nov, KPAD);
     dim3 grid(ntiles * nf), block(256);
     if (precision == 0) {
         hipLaunchKernelGGL((pi_syrk_kernel<double, 16>), grid, block, 0, st, B, s, nov, naux,
@@ -333,11 +542,26 @@ hipError_t launch_pi_syrk(int precision, const double* B, const PiOperands& ops,
         hipLaunchKernelGGL((pi_syrk_kernel<float, 32>), grid, block, 0, st, B, s, nov, naux,
                            npad, nf, ntiles, sld, out, mode);
     } else {
-        if (!ops.bt_hi || ops.kpad != sld) return hipErrorInvalidValue;
-        hipLaunchKernelGGL(pi_syrk_bf16x3_kernel, grid, block, 0, st, ops.bt_hi, ops.bt_lo,
-                           ops.bt_f32, s, ops.kpad, naux, npad, nf, ntiles, sld, out, mode);
+        if (!ops.bt_hi || !ops.s32 || ops.kpad != sld) return hipErrorInvalidValue;
+        if (variant == 2) {
+            if (ops.rows < round_up(npad > naux ? npad : naux, WTI)) return hipErrorInvalidValue;
+            const int TJ = ceil_div(mode == PiOut::EpsLower ? npad : naux, WTJ);
+            const int TIw = ceil_div(mode == PiOut::EpsLower ? npad : naux, WTI);
+            int nt = 0;
+            for (int a = 0; a < TIw; ++a) nt += 2 * a + 2 < TJ ? 2 * a + 2 : TJ;
+            hipLaunchKernelGGL(pi_syrk_bf16x3_wide_kernel, dim3(nt * nf), dim3(512), 0, st,
+                               ops.bt_hi, ops.bt_lo, ops.bt_f32, ops.s32, ops.kpad, naux, npad, nf,
+                               nt, TJ, out, mode);
+        } else if (variant == 1)
+            hipLaunchKernelGGL(pi_syrk_bf16x3_kernel<64>, grid, block, 0, st, ops.bt_hi, ops.bt_lo,
+                               ops.bt_f32, ops.s32, ops.kpad, naux, npad, nf, ntiles, out, mode);
+        else
+            hipLaunchKernelGGL(pi_syrk_bf16x3_kernel<32>, grid, block, 0, st, ops.bt_hi, ops.bt_lo,
+                               ops.bt_f32, ops.s32, ops.kpad, naux, npad, nf, ntiles, out, mode);
     }
     return hipGetLastError();
 }
+
+int pi_variant_count(int precision) { return precision == 2 ? 3 : 1; }
 
 }  // namespace qx

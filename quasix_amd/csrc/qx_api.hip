@@ -15,6 +15,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <string>
@@ -34,6 +35,9 @@ struct qx_context {
     long long phase_n[QX_NPHASE] = {0};
     std::vector<hipEvent_t> ev_free;
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_pending;
+    // autotuner: best kernel variant per (kind, precision, shape) key
+    std::map<std::string, int> tuned;
+    bool tune_loaded = false;
 };
 
 namespace {
@@ -110,6 +114,42 @@ void resolve_profile(qx_handle h) {
         h->ev_free.push_back(p.second.second);
     }
     h->ev_pending.clear();
+}
+
+// ---- autotuner ----------------------------------------------------------------------------
+// Kernel variants are timed once per shape key on the live operands (HIP events on the
+// handle's stream; the output of a timing launch is overwritten by the real launch) and
+// the fastest is cached in the handle.  QXGW_AUTOTUNE=0 disables timing (variant 0);
+// QXGW_PI_VARIANT forces a variant; QXGW_AUTOTUNE_DB names a JSON-lines file that seeds
+// and records the choices ({"key": ..., "variant": ..., "ms": [...]}).
+const char* tune_db_path() { return std::getenv("QXGW_AUTOTUNE_DB"); }
+
+void tune_load(qx_handle h) {
+    if (h->tune_loaded) return;
+    h->tune_loaded = true;
+    const char* path = tune_db_path();
+    if (!path) return;
+    FILE* fp = std::fopen(path, "r");
+    if (!fp) return;
+    char line[1024];
+    while (std::fgets(line, sizeof line, fp)) {
+        char key[512];
+        int v = -1;
+        if (std::sscanf(line, "{\"key\": \"%511[^\"]\", \"variant\": %d", key, &v) == 2 && v >= 0)
+            h->tuned[key] = v;
+    }
+    std::fclose(fp);
+}
+
+void tune_store(const std::string& key, int v, const std::vector<float>& ms) {
+    const char* path = tune_db_path();
+    if (!path) return;
+    FILE* fp = std::fopen(path, "a");
+    if (!fp) return;
+    std::fprintf(fp, "{\"key\": \"%s\", \"variant\": %d, \"ms\": [", key.c_str(), v);
+    for (size_t i = 0; i < ms.size(); ++i) std::fprintf(fp, i ? ", %.4f" : "%.4f", ms[i]);
+    std::fprintf(fp, "]}\n");
+    std::fclose(fp);
 }
 
 template <typename T>
@@ -294,14 +334,49 @@ NewtonParams to_params(const qx_newton_config& c) {
 struct WnmResult {
     int chol_fallbacks = 0;
     int singular = 0;
+    int pi_variant = 0;
 };
+
+// Returns the variant to use for `kind` at this shape, timing all candidates on first use.
+template <class Launch>
+int choose_variant(qx_handle h, const char* kind, int precision, int nvariants,
+                   std::initializer_list<int> shape, Launch launch) {
+    if (nvariants <= 1) return 0;
+    if (const char* f = std::getenv("QXGW_PI_VARIANT")) {
+        const int v = std::atoi(f);
+        if (v >= 0 && v < nvariants) return v;
+    }
+    const char* at = std::getenv("QXGW_AUTOTUNE");
+    if (at && std::atoi(at) == 0) return 0;
+    std::string key = std::string(kind) + "/p" + std::to_string(precision);
+    for (int x : shape) key += "/" + std::to_string(x);
+    tune_load(h);
+    auto it = h->tuned.find(key);
+    if (it != h->tuned.end() && it->second < nvariants) return it->second;
+    std::vector<float> ms(nvariants, 0.f);
+    hipEvent_t a = take_event(h), b = take_event(h);
+    for (int v = 0; v < nvariants; ++v) {
+        launch(v);                                   // warm-up (code object load, caches)
+        HIPCK(h, hipEventRecord(a, h->stream));
+        launch(v);
+        HIPCK(h, hipEventRecord(b, h->stream));
+        HIPCK(h, hipEventSynchronize(b));
+        HIPCK(h, hipEventElapsedTime(&ms[v], a, b));
+    }
+    h->ev_free.push_back(a);
+    h->ev_free.push_back(b);
+    const int best = (int)(std::min_element(ms.begin(), ms.end()) - ms.begin());
+    h->tuned[key] = best;
+    tune_store(key, best, ms);
+    return best;
+}
 
 WnmResult build_wnm(qx_handle h, int precision, int nocc, int nvir, int naux,
                     const double* eps_dev, const double* B_ov, int nf, const double* freqs_dev,
                     int npair, const double* B_pair, double* wnm_out) {
     WnmResult r;
     const int nov = nocc * nvir;
-    const int npad = round_up(naux, NB), nblk = npad / NB, kpad = round_up(nov, 32);
+    const int npad = round_up(naux, NB), nblk = npad / NB, kpad = round_up(nov, KPAD);
     hipStream_t st = h->stream;
     double* s = ws<double>(h, "pi_weights", (size_t)nf * kpad);
     double* eps_mat = ws<double>(h, "eps_mat", (size_t)nf * npad * npad);
@@ -310,27 +385,39 @@ WnmResult build_wnm(qx_handle h, int precision, int nocc, int nvir, int naux,
     PiOperands ops;
     {
         Phase ph(h, QX_PHASE_PREP);
-        HIPCK(h, launch_pi_weights(eps_dev, nocc, nvir, nf, freqs_dev, s, st));
+        float* s32 = precision == QX_PREC_BF16X3 ? ws<float>(h, "pi_weights32", (size_t)nf * kpad)
+                                                 : nullptr;
+        HIPCK(h, launch_pi_weights(eps_dev, nocc, nvir, nf, freqs_dev, s, s32, st));
+        ops.s32 = s32;
         if (precision == QX_PREC_BF16X3) {
-            uint16_t* hi = ws<uint16_t>(h, "bt_hi", (size_t)npad * kpad);
-            uint16_t* lo = ws<uint16_t>(h, "bt_lo", (size_t)npad * kpad);
-            float* f32 = ws<float>(h, "bt_f32", (size_t)npad * kpad);
-            HIPCK(h, launch_pi_prepare_mixed(B_ov, nov, naux, npad, kpad, hi, lo, f32, st));
+            const int brows = round_up(npad, 256);
+            uint16_t* hi = ws<uint16_t>(h, "bt_hi", (size_t)brows * kpad);
+            uint16_t* lo = ws<uint16_t>(h, "bt_lo", (size_t)brows * kpad);
+            float* f32 = ws<float>(h, "bt_f32", (size_t)brows * kpad);
+            HIPCK(h, launch_pi_prepare_mixed(B_ov, nov, naux, brows, kpad, hi, lo, f32, st));
             ops.bt_hi = hi;
             ops.bt_lo = lo;
             ops.bt_f32 = f32;
             ops.kpad = kpad;
+            ops.rows = brows;
         }
         fill_i32(h, fail, 0, nf);
     }
+    const int pv = choose_variant(h, "pi", precision, pi_variant_count(precision),
+                                  {naux, nov, nf}, [&](int v) {
+        HIPCK(h, launch_pi_syrk(precision, B_ov, ops, s, nov, naux, npad, nf, eps_mat,
+                                PiOut::EpsLower, st, v));
+    });
+    r.pi_variant = pv;
     {
         Phase ph(h, QX_PHASE_PI);
         HIPCK(h, launch_pi_syrk(precision, B_ov, ops, s, nov, naux, npad, nf, eps_mat,
-                                PiOut::EpsLower, st));
+                                PiOut::EpsLower, st, pv));
     }
+    double* x = ws<double>(h, "xinv", (size_t)nf * npad * npad);
     {
         Phase ph(h, QX_PHASE_POTRF);
-        HIPCK(h, launch_potrf_batched(eps_mat, linv, npad, nf, fail, st));
+        HIPCK(h, launch_potrf_batched(eps_mat, x, linv, npad, nf, fail, st));
     }
     std::vector<int> fail_h(nf);
     d2h(h, fail_h.data(), fail, nf);
@@ -344,14 +431,17 @@ WnmResult build_wnm(qx_handle h, int precision, int nocc, int nvir, int naux,
     }
     double* wpart = ws<double>(h, "wpart", (size_t)nf * nblk * std::max(npair, 1));
     if (r.chol_fallbacks == 0) {
-        double* x = ws<double>(h, "xinv", (size_t)nf * npad * npad);
-        {
-            Phase ph(h, QX_PHASE_TRTRI);
-            HIPCK(h, launch_trtri_batched(eps_mat, linv, x, npad, nf, st));
-        }
         if (npair > 0) {
+            WnmMixed mx{};
+            if (precision == QX_PREC_BF16X3) {
+                const size_t npp = (size_t)round_up(npair, 128);
+                mx.bh = ws<__bf16>(h, "wnm_bh", npp * npad);
+                mx.bl = ws<__bf16>(h, "wnm_bl", npp * npad);
+                mx.mh = ws<__bf16>(h, "wnm_mh", (size_t)nf * npad * npad);
+                mx.ml = ws<__bf16>(h, "wnm_ml", (size_t)nf * npad * npad);
+            }
             Phase ph(h, QX_PHASE_WNM);
-            HIPCK(h, launch_wnm(precision, bp, x, npair, npad, nf, wpart, wnm_out, st));
+            HIPCK(h, launch_wnm(precision, bp, x, npair, npad, nf, wpart, wnm_out, &mx, st));
         }
     } else {
         // Reference chain: Cholesky failed somewhere -> LU inverse (here Gauss-Jordan with
@@ -636,6 +726,7 @@ qx_status run_g0w0(qx_handle h, const qx_g0w0_desc* d, const double* mo_energy,
         stats->residue_targets = residue_targets;
         stats->residue_jobs = cd.residue_jobs;
         stats->newton_failures = newton_fail;
+        stats->pi_variant = wr.pi_variant;
         stats->seconds =
             std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     }
@@ -799,22 +890,32 @@ qx_status qx_rho_response_dev(qx_handle h, int precision, int nocc, int nvir, in
                 "dimensions must be positive");
         REQUIRE(h, precision >= 0 && precision <= 2, QX_ERR_INVALID_INPUT, "unknown precision");
         REQUIRE(h, mo_energy && B_ov && freqs && pi_out, QX_ERR_INVALID_INPUT, "NULL pointer");
-        const int nov = nocc * nvir, kpad = round_up(nov, 32), npad = round_up(naux, NB);
+        const int nov = nocc * nvir, kpad = round_up(nov, KPAD), npad = round_up(naux, NB);
         double* s = ws<double>(h, "pi_weights", (size_t)nfreq * kpad);
-        HIPCK(h, launch_pi_weights(mo_energy, nocc, nvir, nfreq, freqs, s, h->stream));
         PiOperands ops;
+        float* s32 = precision == QX_PREC_BF16X3 ? ws<float>(h, "pi_weights32", (size_t)nfreq * kpad)
+                                                 : nullptr;
+        HIPCK(h, launch_pi_weights(mo_energy, nocc, nvir, nfreq, freqs, s, s32, h->stream));
+        ops.s32 = s32;
         if (precision == QX_PREC_BF16X3) {
-            uint16_t* hi = ws<uint16_t>(h, "bt_hi", (size_t)npad * kpad);
-            uint16_t* lo = ws<uint16_t>(h, "bt_lo", (size_t)npad * kpad);
-            float* f32 = ws<float>(h, "bt_f32", (size_t)npad * kpad);
-            HIPCK(h, launch_pi_prepare_mixed(B_ov, nov, naux, npad, kpad, hi, lo, f32, h->stream));
+            const int brows = round_up(npad, 256);
+            uint16_t* hi = ws<uint16_t>(h, "bt_hi", (size_t)brows * kpad);
+            uint16_t* lo = ws<uint16_t>(h, "bt_lo", (size_t)brows * kpad);
+            float* f32 = ws<float>(h, "bt_f32", (size_t)brows * kpad);
+            HIPCK(h, launch_pi_prepare_mixed(B_ov, nov, naux, brows, kpad, hi, lo, f32, h->stream));
             ops.bt_hi = hi;
             ops.bt_lo = lo;
             ops.bt_f32 = f32;
             ops.kpad = kpad;
+            ops.rows = brows;
         }
+        const int pv = choose_variant(h, "pi_full", precision, pi_variant_count(precision),
+                                      {naux, nov, nfreq}, [&](int v) {
+            HIPCK(h, launch_pi_syrk(precision, B_ov, ops, s, nov, naux, naux, nfreq, pi_out,
+                                    PiOut::PiFull, h->stream, v));
+        });
         HIPCK(h, launch_pi_syrk(precision, B_ov, ops, s, nov, naux, naux, nfreq, pi_out,
-                                PiOut::PiFull, h->stream));
+                                PiOut::PiFull, h->stream, pv));
         HIPCK(h, hipStreamSynchronize(h->stream));
         return QX_OK;
     });
@@ -833,14 +934,13 @@ qx_status qx_screened_wc_dev(qx_handle h, int naux, int nfreq, const double* pi,
         double* linv = ws<double>(h, "linv_diag", (size_t)nfreq * nblk * NB * NB);
         int* fail = ws<int>(h, "chol_fail", nfreq);
         fill_i32(h, fail, 0, nfreq);
-        HIPCK(h, launch_potrf_batched(eps, linv, npad, nfreq, fail, h->stream));
+        double* x = ws<double>(h, "xinv", (size_t)nfreq * npad * npad);
+        HIPCK(h, launch_potrf_batched(eps, x, linv, npad, nfreq, fail, h->stream));
         std::vector<int> fh(nfreq);
         d2h(h, fh.data(), fail, nfreq);
         int nfail = 0;
         for (int v : fh) nfail += v != 0;
         if (nfail == 0) {
-            double* x = ws<double>(h, "xinv", (size_t)nfreq * npad * npad);
-            HIPCK(h, launch_trtri_batched(eps, linv, x, npad, nfreq, h->stream));
             HIPCK(h, launch_wc_from_linv(x, wc_out, npad, naux, nfreq, h->stream));
         } else {
             double* work = ws<double>(h, "gj_work", (size_t)nfreq * naux * 2 * naux + nfreq);
@@ -944,6 +1044,95 @@ qx_status qx_g0w0_cd_ov_dev(qx_handle h, const qx_g0w0_desc* d, const double* mo
         return run_g0w0(h, &dd, mo_energy, mo_energy_ref, B_ov, rows, pt, pm, targets,
                         static_terms, guess, qp_energy, z_factor, sigma_re, sigma_im, state,
                         stats);
+    });
+}
+
+qx_status qx_g0w0_adjoint_ov_dev(qx_handle h, const qx_g0w0_desc* d, const double* mo_energy,
+                                 const double* B_ov, const int* targets,
+                                 const double* qp_energy, const double* grad_qp,
+                                 double* grad_B, double* grad_eps) {
+    return guarded(h, [&] {
+        validate_desc(h, d);
+        REQUIRE(h, mo_energy && B_ov && targets && qp_energy && grad_qp && grad_B && grad_eps,
+                QX_ERR_INVALID_INPUT, "NULL pointer");
+        const int nocc = d->nocc, nvir = d->nvir, naux = d->naux, nf = d->nfreq;
+        const int nt = d->ntarget, nmo = nocc + nvir, nov = nocc * nvir;
+        const int npad = round_up(naux, NB);
+        hipStream_t st = h->stream;
+        // ov pairs (as the forward)
+        std::vector<int> tg(nt), off(nt + 1, 0);
+        d2h(h, tg.data(), targets, nt);
+        for (int t = 0; t < nt; ++t) {
+            REQUIRE(h, tg[t] >= 0 && tg[t] < nmo, QX_ERR_INVALID_INPUT, "target out of range");
+            off[t + 1] = off[t] + (tg[t] < nocc ? nvir : nocc);
+        }
+        const int npair = off[nt];
+        int* off_dev = ws<int>(h, "adj_off", nt + 1);
+        h2d(h, off_dev, off.data(), nt + 1);
+        int* pt = ws<int>(h, "adj_pt", npair);
+        int* pm = ws<int>(h, "adj_pm", npair);
+        double* rows = ws<double>(h, "adj_rows", (size_t)npair * naux);
+        hipLaunchKernelGGL(ov_pairs_kernel, dim3(std::min(std::max(nocc, nvir), 1024), nt),
+                           dim3(256), 0, st, B_ov, nocc, nvir, naux, targets, off_dev, nt, pt, pm,
+                           rows);
+        HIPCK(h, hipGetLastError());
+        // forward checkpoints in FP64: X_k = L_k^-1 for every frequency, Wc_nm
+        std::vector<double> fr, wt;
+        scaled_roots(nf, d->x0, fr, wt);
+        double* fr_dev = ws<double>(h, "freqs", nf);
+        double* wt_dev = ws<double>(h, "wts", nf);
+        h2d(h, fr_dev, fr.data(), nf);
+        h2d(h, wt_dev, wt.data(), nf);
+        double* wnm = ws<double>(h, "wnm", (size_t)nf * npair);
+        WnmResult wr = build_wnm(h, QX_PREC_FP64, nocc, nvir, naux, mo_energy, B_ov, nf, fr_dev,
+                                 npair, rows, wnm);
+        REQUIRE(h, wr.chol_fallbacks == 0, QX_ERR_NOT_IMPLEMENTED,
+                "adjoint needs SPD dielectric matrices (Cholesky failed at %d frequencies)",
+                wr.chol_fallbacks);
+        double* X = ws<double>(h, "xinv", (size_t)nf * npad * npad);
+        HIPCK(h, launch_zero_upper_blocks(X, npad, nf, st));
+        fill_f64(h, grad_B, 0.0, (size_t)nov * naux);
+        fill_f64(h, grad_eps, 0.0, nmo);
+        // adjoint of the quadrature + Newton
+        const int sldp = round_up(npair, KPAD);
+        double* abar = ws<double>(h, "adj_abar", (size_t)nf * sldp);
+        fill_f64(h, abar, 0.0, (size_t)nf * sldp);
+        std::vector<double> eps_h(nmo);
+        d2h(h, eps_h.data(), mo_energy, nmo);
+        HIPCK(h, launch_adjoint_quad(targets, off_dev, pm, wnm, nt, npair, nf, mo_energy, fr_dev,
+                                     wt_dev, d->eta, qp_energy, grad_qp, sldp, abar, grad_eps, st));
+        // Wbar_k = sum_j abar_jk b_j b_j^T ;  Pibar_k = eps_k^-1 Wbar_k eps_k^-1
+        const size_t mat = (size_t)nf * npad * npad;
+        double* wbar = ws<double>(h, "adj_wbar", mat);
+        double* E = ws<double>(h, "adj_einv", mat);
+        double* T = ws<double>(h, "adj_tmp", mat);
+        double* pibar = ws<double>(h, "adj_pibar", mat);
+        fill_f64(h, wbar, 0.0, mat);
+        PiOperands none;
+        HIPCK(h, launch_pi_syrk(QX_PREC_FP64, rows, none, abar, npair, naux, npad, nf, wbar,
+                                PiOut::PiFull, st));
+        HIPCK(h, launch_batched_gemm(true, false, X, X, E, npad, nf, st));        // X^T X
+        HIPCK(h, launch_batched_gemm(false, false, E, wbar, T, npad, nf, st));
+        HIPCK(h, launch_batched_gemm(false, false, T, E, pibar, npad, nf, st));
+        // dL/dB through Pi and dL/de_ia
+        const int sldb = round_up(nov, KPAD);
+        double* s = ws<double>(h, "adj_s", (size_t)nf * sldb);
+        double* ds = ws<double>(h, "adj_ds", (size_t)nf * sldb);
+        HIPCK(h, launch_adjoint_weights(mo_energy, nocc, nvir, nf, fr_dev, sldb, s, ds, st));
+        const int nparts = ceil_div(naux, 64);
+        double* part = ws<double>(h, "adj_part", (size_t)nparts * nov);
+        HIPCK(h, launch_scaled_gemm(B_ov, nov, naux, s, ds, sldb, pibar, npad, nf, 2.0, grad_B,
+                                    part, st));
+        HIPCK(h, launch_adjoint_eps_from_ebar(part, nparts, nocc, nvir, grad_eps, st));
+        // dL/db_j through w_j = b_j^T Wc_k b_j (pair rows are rows of B_ov)
+        double* gbp = ws<double>(h, "adj_gbp", (size_t)npair * naux);
+        fill_f64(h, gbp, 0.0, (size_t)npair * naux);
+        HIPCK(h, launch_scaled_gemm(rows, npair, naux, abar, nullptr, sldp, E, npad, nf, 2.0, gbp,
+                                    nullptr, st));
+        HIPCK(h, launch_adjoint_pair_scatter(gbp, rows, abar, sldp, nf, targets, pt, pm, npair,
+                                             nocc, nvir, naux, grad_B, st));
+        HIPCK(h, hipStreamSynchronize(st));
+        return QX_OK;
     });
 }
 

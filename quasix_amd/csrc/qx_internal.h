@@ -16,7 +16,8 @@
 
 namespace qx {
 
-constexpr int NB = 64;          // Cholesky / trtri block size
+constexpr int NB = 64;          // Cholesky block size
+constexpr int KPAD = 64;        // N_ov is padded to a multiple of KPAD for the Pi operands
 constexpr int kMaxTargetsPerLaunch = 1 << 20;
 
 inline int round_up(int x, int m) { return (x + m - 1) / m * m; }
@@ -38,26 +39,30 @@ struct PiOperands {              // per-solve operand copies for the mixed kerne
     const uint16_t* bt_hi = nullptr;   // [npad][kpad] bf16 high part of B^T
     const uint16_t* bt_lo = nullptr;   // [npad][kpad] bf16 low part of B^T
     const float*    bt_f32 = nullptr;  // [npad][kpad] B^T in FP32
-    int kpad = 0;                      // N_ov rounded up to 32
+    const float*    s32 = nullptr;     // [nf][kpad] frequency weights in FP32
+    int kpad = 0;                      // N_ov rounded up to KPAD
+    int rows = 0;                      // allocated rows of bt_* (zero padded, multiple of 256)
 };
 
-// s[f][r] table; eia from eps.  s has layout [nf][nov].
+// s[f][r] table (FP64, and FP32 into s32 if non-null); eia from eps.  Layout [nf][kpad].
 hipError_t launch_pi_weights(const double* eps, int nocc, int nvir, int nf,
-                             const double* freqs, double* s, hipStream_t st);
-// B^T split copies for the mixed kernels.
-hipError_t launch_pi_prepare_mixed(const double* B, int nov, int naux, int npad, int kpad,
+                             const double* freqs, double* s, float* s32, hipStream_t st);
+// B^T split copies for the mixed kernels; rows [naux, nrows) are zero.
+hipError_t launch_pi_prepare_mixed(const double* B, int nov, int naux, int nrows, int kpad,
                                    uint16_t* bt_hi, uint16_t* bt_lo, float* bt_f32,
                                    hipStream_t st);
-// precision: 0 = FP64 MFMA, 1 = FP32 MFMA, 2 = BF16x3 MFMA (FP32 accumulation)
+// precision: 0 = FP64 MFMA, 1 = FP32 MFMA, 2 = BF16x3 MFMA (FP32 accumulation).
+// variant selects the kernel instance: BF16x3 has 128x128 tiles with K chunks 32 (0) and
+// 64 (1), and 256x128 tiles with a swizzled LDS layout (2); FP64/FP32 have one instance.  pi_variant_count() tells how many exist per precision.
 hipError_t launch_pi_syrk(int precision, const double* B, const PiOperands& ops,
                           const double* s, int nov, int naux, int npad, int nf,
-                          double* out, PiOut mode, hipStream_t st);
+                          double* out, PiOut mode, hipStream_t st, int variant = 0);
+int pi_variant_count(int precision);
 
 // ---- Factorisation (chol_kernels.hip) --------------------------------------------------
-hipError_t launch_potrf_batched(double* a, double* linv_diag, int npad, int nf,
+// Cholesky eps = L L^T (L into a, lower blocks) and X = L^-1 (lower blocks of x) together.
+hipError_t launch_potrf_batched(double* a, double* x, double* linv_diag, int npad, int nf,
                                 int* fail, hipStream_t st);
-hipError_t launch_trtri_batched(const double* l, const double* linv_diag, double* x,
-                                int npad, int nf, hipStream_t st);
 // eps^-1 - I from X = L^-1 :  W = X^T X - I  (full symmetric)
 hipError_t launch_wc_from_linv(const double* x, double* wc, int npad, int naux, int nf,
                                hipStream_t st);
@@ -69,9 +74,16 @@ hipError_t launch_lu_fallback_wc(const double* pi_full_or_eps, int is_eps, doubl
                                  hipStream_t st);
 
 // ---- Contractions and self-energy (sigma_kernels.hip) ----------------------------------
-// wnm[f][j] = 2 b_j.(M b_j) + |M b_j|^2,  M = X - I  (precision as for Pi)
+// Workspace of the BF16x3 contraction: hi/lo splits of the pair rows ([round_up(npair,128)]
+// [npad]) and of M_f = X_f - I ([nf][npad][npad]).
+struct WnmMixed {
+    __bf16 *bh, *bl, *mh, *ml;
+};
+// wnm[f][j] = 2 b_j.(M b_j) + |M b_j|^2,  M = X - I.  precision 2 runs the product on BF16
+// MFMA (hi/lo split, FP32 accumulation), otherwise FP64 MFMA.
 hipError_t launch_wnm(int precision, const double* bpairs, const double* x, int npair,
-                      int npad, int nf, double* wpart, double* wnm, hipStream_t st);
+                      int npad, int nf, double* wpart, double* wnm, const WnmMixed* mixed,
+                      hipStream_t st);
 // wnm from an explicit Wc: wnm[f][j] = b_j^T Wc_f b_j  (FP64)
 hipError_t launch_wnm_from_wc(const double* bpairs, const double* wc, int npair, int npad,
                               int nf, double* wpart, double* wnm, hipStream_t st);
@@ -115,13 +127,38 @@ hipError_t launch_sigma_imag_eval(const SigmaProblem& p, const double* energies,
                                   double* out, hipStream_t st);
 
 // Residues: for job q, W = b^T[(I - Pi(z_q))^-1 - I] b, z_q = omega_q + i eta.
-// Builds complex Pi(z_q) (FP64), solves with a one-workgroup complex LU with partial
-// pivoting, and also returns dW/domega.
+// Builds complex Pi(z_q) (FP64) and solves with a one-workgroup complex LU with partial
+// pivoting.
 hipError_t launch_residue_jobs(const double* B, const double* eps, int nocc, int nvir,
                                int naux, int njobs, const double* omegas, double eta,
                                const double* brows,      // [njobs][naux]
                                double* work,             // [njobs][naux][2*naux+2] complex
                                double* w_out,            // [njobs][2] re, im
                                hipStream_t st);
+
+// ---- Adjoint (adjoint_kernels.hip) -----------------------------------------------------
+hipError_t launch_adjoint_quad(const int* targets, const int* pair_off, const int* pair_m,
+                               const double* wnm, int nt, int npair, int nf, const double* eps,
+                               const double* freqs, const double* wts, double eta,
+                               const double* E, const double* gE, int sld, double* abar,
+                               double* geps, hipStream_t st);
+// C_b = op(A_b) op(B_b), square n x n (n % 64 == 0), batched.
+hipError_t launch_batched_gemm(bool ta, bool tb, const double* A, const double* B, double* C,
+                               int n, int batch, hipStream_t st);
+// out[r][P] += alpha sum_k sum_Q s1_k(r) A[r][Q] M_k[Q][P]; part[pt][r] = row dots with s2.
+hipError_t launch_scaled_gemm(const double* A, int nrow, int ncol, const double* s1,
+                              const double* s2, int sld, const double* M, int npad, int nf,
+                              double alpha, double* out, double* part, hipStream_t st);
+hipError_t launch_adjoint_eps_from_ebar(const double* part, int nparts, int nocc, int nvir,
+                                        double* geps, hipStream_t st);
+hipError_t launch_adjoint_weights(const double* eps, int nocc, int nvir, int nf,
+                                  const double* freqs, int sld, double* s, double* ds,
+                                  hipStream_t st);
+hipError_t launch_adjoint_pair_scatter(const double* gbp, const double* bp, const double* abar,
+                                       int sld, int nf, const int* targets, const int* pair_t,
+                                       const int* pair_m, int npair, int nocc, int nvir,
+                                       int naux, double* gB, hipStream_t st);
+// Zero the strictly-upper NB-blocks of every [npad][npad] matrix of the batch.
+hipError_t launch_zero_upper_blocks(double* x, int npad, int nf, hipStream_t st);
 
 }  // namespace qx

@@ -153,6 +153,156 @@ __global__ __launch_bounds__(256) void wnm_wc_kernel(const double* __restrict__ 
         }
 }
 
+// ---- Wc_nm contraction, BF16x3 --------------------------------------------------------------
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+typedef double dv4 __attribute__((ext_vector_type(4)));
+constexpr int WT = 128;          // tile (pairs x auxiliary columns)
+constexpr int WK = 32;           // K chunk
+constexpr int WKP = 40;          // padded LDS pitch (bf16)
+
+__device__ __forceinline__ void split4(const dv4 x, bf16x4& h, bf16x4& l) {
+    const f4 xf = __builtin_convertvector(x, f4);
+    h = __builtin_convertvector(xf, bf16x4);
+    const dv4 r = x - __builtin_convertvector(__builtin_convertvector(h, f4), dv4);
+    l = __builtin_convertvector(__builtin_convertvector(r, f4), bf16x4);
+}
+
+// hi/lo split of M_f = X_f - I over the lower 64-blocks (zero above them).
+__global__ void split_m_kernel(const double* __restrict__ x, int npad, int nf,
+                               __bf16* __restrict__ mh, __bf16* __restrict__ ml) {
+    const size_t total = (size_t)nf * npad * npad / 4;
+    for (size_t i4 = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i4 < total;
+         i4 += (size_t)gridDim.x * blockDim.x) {
+        const size_t i = i4 * 4;
+        const size_t rc = i % ((size_t)npad * npad);
+        const int P = rc / npad, Q = rc % npad;
+        dv4 v = dv4{0.0, 0.0, 0.0, 0.0};
+        if (Q / NB <= P / NB) {
+            v = *reinterpret_cast<const dv4*>(x + i);
+            if (P >= Q && P < Q + 4) v[P - Q] -= 1.0;
+        }
+        bf16x4 h, l;
+        split4(v, h, l);
+        *reinterpret_cast<bf16x4*>(mh + i) = h;
+        *reinterpret_cast<bf16x4*>(ml + i) = l;
+    }
+}
+
+// hi/lo split of the pair rows, rows padded with zeros to npair_pad.
+__global__ void split_rows_kernel(const double* __restrict__ bp, int npair, int npair_pad,
+                                  int npad, __bf16* __restrict__ bh, __bf16* __restrict__ bl) {
+    const size_t total = (size_t)npair_pad * npad / 4;
+    for (size_t i4 = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i4 < total;
+         i4 += (size_t)gridDim.x * blockDim.x) {
+        const size_t i = i4 * 4;
+        const int r = i / npad;
+        const dv4 v = r < npair ? *reinterpret_cast<const dv4*>(bp + i) : dv4{0.0, 0.0, 0.0, 0.0};
+        bf16x4 h, l;
+        split4(v, h, l);
+        *reinterpret_cast<bf16x4*>(bh + i) = h;
+        *reinterpret_cast<bf16x4*>(bl + i) = l;
+    }
+}
+
+// Y = Bp M^T on BF16 MFMA (hi*hi + hi*lo + lo*hi, FP32 accumulation) over Q < P0 + 128
+// (M is lower block-triangular); epilogue 2 b.y + y.y reduced per 64-column block.
+__global__ __launch_bounds__(256, 2) void wnm_bf16x3_kernel(
+    const __bf16* __restrict__ bh, const __bf16* __restrict__ bl, const __bf16* __restrict__ mh,
+    const __bf16* __restrict__ ml, const double* __restrict__ bp, int npair, int npad,
+    double* __restrict__ wpart) {
+    __shared__ __attribute__((aligned(16))) __bf16 Ah[WT][WKP];
+    __shared__ __attribute__((aligned(16))) __bf16 Al[WT][WKP];
+    __shared__ __attribute__((aligned(16))) __bf16 Mh[WT][WKP];
+    __shared__ __attribute__((aligned(16))) __bf16 Ml[WT][WKP];
+    const int j0 = blockIdx.x * WT, P0 = blockIdx.y * WT, f = blockIdx.z;
+    const int nblk = npad / NB;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wr = wave >> 1, wc = wave & 1;
+    const int row = tid >> 1, kq0 = (tid & 1) * 16;
+    const bool vm = P0 + row < npad;
+    const __bf16* pah = bh + (size_t)(j0 + row) * npad + kq0;
+    const __bf16* pal = bl + (size_t)(j0 + row) * npad + kq0;
+    const size_t moff = (size_t)f * npad * npad + (size_t)(P0 + row) * npad + kq0;
+    const __bf16* pmh = mh + moff;
+    const __bf16* pml = ml + moff;
+    bf16x8 rah[2], ral[2], rmh[2], rml[2];
+    const bf16x8 z8 = bf16x8{0, 0, 0, 0, 0, 0, 0, 0};
+    auto fetch = [&](int k0) {
+        #pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            rah[q] = *reinterpret_cast<const bf16x8*>(pah + k0 + 8 * q);
+            ral[q] = *reinterpret_cast<const bf16x8*>(pal + k0 + 8 * q);
+            rmh[q] = vm ? *reinterpret_cast<const bf16x8*>(pmh + k0 + 8 * q) : z8;
+            rml[q] = vm ? *reinterpret_cast<const bf16x8*>(pml + k0 + 8 * q) : z8;
+        }
+    };
+    f4 acc[4][4];
+    #pragma unroll
+    for (int bi = 0; bi < 4; ++bi)
+        #pragma unroll
+        for (int bj = 0; bj < 4; ++bj) acc[bi][bj] = f4{0.f, 0.f, 0.f, 0.f};
+    const int kend = P0 + WT < npad ? P0 + WT : npad;
+    fetch(0);
+    for (int k0 = 0; k0 < kend; k0 += WK) {
+        #pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            *reinterpret_cast<bf16x8*>(&Ah[row][kq0 + 8 * q]) = rah[q];
+            *reinterpret_cast<bf16x8*>(&Al[row][kq0 + 8 * q]) = ral[q];
+            *reinterpret_cast<bf16x8*>(&Mh[row][kq0 + 8 * q]) = rmh[q];
+            *reinterpret_cast<bf16x8*>(&Ml[row][kq0 + 8 * q]) = rml[q];
+        }
+        __syncthreads();
+        if (k0 + WK < kend) fetch(k0 + WK);
+        const int kq = (lane >> 4) * 8, r16 = lane & 15;
+        bf16x8 ah[4], al[4], xh[4], xl[4];
+        #pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            ah[q] = *reinterpret_cast<const bf16x8*>(&Ah[wr * 64 + q * 16 + r16][kq]);
+            al[q] = *reinterpret_cast<const bf16x8*>(&Al[wr * 64 + q * 16 + r16][kq]);
+            xh[q] = *reinterpret_cast<const bf16x8*>(&Mh[wc * 64 + q * 16 + r16][kq]);
+            xl[q] = *reinterpret_cast<const bf16x8*>(&Ml[wc * 64 + q * 16 + r16][kq]);
+        }
+        #pragma unroll
+        for (int bi = 0; bi < 4; ++bi)
+            #pragma unroll
+            for (int bj = 0; bj < 4; ++bj) {
+                f4 c = acc[bi][bj];
+                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[bi], xh[bj], c, 0, 0, 0);
+                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[bi], xl[bj], c, 0, 0, 0);
+                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[bi], xh[bj], c, 0, 0, 0);
+                acc[bi][bj] = c;
+            }
+        __syncthreads();
+    }
+    #pragma unroll
+    for (int bi = 0; bi < 4; ++bi) {
+        double rs[4] = {0, 0, 0, 0};
+        #pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int j = j0 + wr * 64 + bi * 16 + mfma_row_f32(lane, v);
+            #pragma unroll
+            for (int bj = 0; bj < 4; ++bj) {
+                const int Pc = P0 + wc * 64 + bj * 16 + (lane & 15);
+                const double y = (double)acc[bi][bj][v];
+                const double b = (j < npair && Pc < npad) ? bp[(size_t)j * npad + Pc] : 0.0;
+                rs[v] += 2.0 * b * y + y * y;
+            }
+        }
+        #pragma unroll
+        for (int v = 0; v < 4; ++v)
+            for (int off = 1; off < 16; off <<= 1) rs[v] += __shfl_xor(rs[v], off, 64);
+        const int pb = P0 / NB + wc;
+        if ((lane & 15) == 0 && pb < nblk)
+            #pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int j = j0 + wr * 64 + bi * 16 + mfma_row_f32(lane, v);
+                if (j < npair) wpart[((size_t)f * nblk + pb) * npair + j] = rs[v];
+            }
+    }
+}
+
 // ---- CD quadrature ------------------------------------------------------------------------
 
 __device__ __forceinline__ double wave_sum(double v) {
@@ -425,11 +575,22 @@ __global__ __launch_bounds__(1024) void residue_solve_kernel(int naux,
 }  // namespace
 
 hipError_t launch_wnm(int precision, const double* bpairs, const double* x, int npair, int npad,
-                      int nf, double* wpart, double* wnm, hipStream_t st) {
-    (void)precision;
+                      int nf, double* wpart, double* wnm, const WnmMixed* mixed, hipStream_t st) {
     const int nblk = npad / NB;
-    dim3 grid(ceil_div(npair, NB), nblk, nf);
-    hipLaunchKernelGGL(wnm_linv_kernel, grid, dim3(256), 0, st, bpairs, x, npair, npad, wpart);
+    if (precision == 2) {
+        if (!mixed) return hipErrorInvalidValue;
+        const int npp = round_up(npair, WT);
+        hipLaunchKernelGGL(split_rows_kernel, dim3(2048), dim3(256), 0, st, bpairs, npair, npp,
+                           npad, mixed->bh, mixed->bl);
+        hipLaunchKernelGGL(split_m_kernel, dim3(8192), dim3(256), 0, st, x, npad, nf, mixed->mh,
+                           mixed->ml);
+        dim3 grid(npp / WT, ceil_div(npad, WT), nf);
+        hipLaunchKernelGGL(wnm_bf16x3_kernel, grid, dim3(256), 0, st, mixed->bh, mixed->bl,
+                           mixed->mh, mixed->ml, bpairs, npair, npad, wpart);
+    } else {
+        dim3 grid(ceil_div(npair, NB), nblk, nf);
+        hipLaunchKernelGGL(wnm_linv_kernel, grid, dim3(256), 0, st, bpairs, x, npair, npad, wpart);
+    }
     dim3 g2(ceil_div(npair, 256), nf);
     hipLaunchKernelGGL(wnm_reduce_kernel, g2, dim3(256), 0, st, wpart, npair, nblk, wnm);
     return hipGetLastError();

@@ -52,9 +52,9 @@ def test_struct_layouts_match_header(lib):
 #include <stddef.h>
 #include "qxgw.h"
 int main(void) {
-  printf("%zu %zu %zu %zu %zu %zu\n", sizeof(qx_newton_config), sizeof(qx_g0w0_desc),
+  printf("%zu %zu %zu %zu %zu %zu %zu\n", sizeof(qx_newton_config), sizeof(qx_g0w0_desc),
          sizeof(qx_stats), offsetof(qx_g0w0_desc, newton), offsetof(qx_g0w0_desc, npair),
-         offsetof(qx_stats, seconds));
+         offsetof(qx_stats, seconds), offsetof(qx_stats, pi_variant));
   return 0;
 }
 """
@@ -67,7 +67,8 @@ int main(void) {
     vals = list(map(int, subprocess.run([exe], capture_output=True, text=True,
                                         check=True).stdout.split()))
     assert vals == [C.sizeof(L.NewtonConfig), C.sizeof(L.G0W0Desc), C.sizeof(L.Stats),
-                    L.G0W0Desc.newton.offset, L.G0W0Desc.npair.offset, L.Stats.seconds.offset]
+                    L.G0W0Desc.newton.offset, L.G0W0Desc.npair.offset, L.Stats.seconds.offset,
+                    L.Stats.pi_variant.offset]
 
 
 @pytest.mark.parametrize("n", [1, 2, 7, 32, 48, 64, 100])

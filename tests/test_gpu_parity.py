@@ -141,3 +141,48 @@ def test_g0w0_c3_fp64_and_mixed(gpu):
         dE = np.max(np.abs(rm["qp_energies"] - r64["qp_energies"]))
         print(f"{prec}: max|dE_QP| vs FP64 = {dE / MEV:.3e} meV")
         assert dE <= 1.0 * MEV
+
+
+@pytest.mark.parametrize("prec,tol", [("fp32", 2e-5), ("bf16x3", 2e-5)])
+def test_wnm_mixed_precision(gpu, prec, tol):
+    """Wc_nm through the mixed kernels against the FP64 oracle (ragged naux = 150)."""
+    from quasix_amd import core
+    nocc, nvir, naux = 6, 40, 150
+    e, B = O.make_inputs(nocc, nvir, naux, seed=5)
+    targets = O.frontier_targets(nocc, nocc + nvir, 2, 2)
+    mn = O.assemble_mnP(B, *O.ov_only_blocks(B, nocc, nvir), nocc, nvir, naux)
+    pt, pm, Bp = O.pair_list(mn, targets)
+    fr, _ = O.get_scaled_legendre_roots(8, 0.5)
+    ref = O.wmn_imag(e, B, nocc, Bp, fr)
+    got = core.compute_wmn_imag(e, B, nocc, Bp, fr, precision=prec)
+    _close(f"wnm {prec}", got, ref, tol, np.max(np.abs(ref)))
+
+
+@pytest.mark.parametrize("variant", [0, 1, 2])
+def test_pi_bf16x3_every_variant(gpu, variant, monkeypatch):
+    """Each BF16x3 Pi kernel instance the autotuner can pick, forced in turn."""
+    from quasix_amd import core
+    monkeypatch.setenv("QXGW_PI_VARIANT", str(variant))
+    nocc, nvir, naux = 9, 70, 300                # naux not a multiple of 64/128/256
+    e, B = O.make_inputs(nocc, nvir, naux, seed=13)
+    fr, _ = O.get_scaled_legendre_roots(5, 0.5)
+    occ = np.array([2.0] * nocc + [0.0] * nvir)
+    p0 = core.compute_polarizability_p0(B.reshape(nocc, nvir, naux), e, occ, fr,
+                                        precision="bf16x3")
+    _close(f"pi symmetry v{variant}", p0, p0.transpose(0, 2, 1), 0.0)
+    for k, w in enumerate(fr):
+        ref = O.get_rho_response(w, e, B, nocc)
+        _close(f"pi bf16x3 v{variant} k={k}", p0[k], ref, 3e-5, np.max(np.abs(ref)))
+
+
+@pytest.mark.parametrize("variant", [0, 1, 2])
+def test_g0w0_bf16x3_every_variant(gpu, variant, monkeypatch):
+    from quasix_amd import gw
+    monkeypatch.setenv("QXGW_PI_VARIANT", str(variant))
+    nocc, nvir, naux = 12, 108, 480
+    e, B = O.make_inputs(nocc, nvir, naux, seed=17)
+    targets = O.frontier_targets(nocc, nocc + nvir)
+    r64 = gw.g0w0(e, B, nocc, targets, nfreq=24)
+    rm = gw.g0w0(e, B, nocc, targets, nfreq=24, precision="bf16x3", stats=True)
+    assert rm["stats"]["pi_variant"] == variant
+    _close(f"E_QP bf16x3 v{variant}", rm["qp_energies"], r64["qp_energies"], MEV)
