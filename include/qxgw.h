@@ -84,6 +84,9 @@ typedef struct {
     int newton_failures;       /* targets that fell back to the linearised solution */
     double seconds;            /* wall time of the call */
     int pi_variant;            /* Pi kernel instance chosen by the autotuner */
+    int waves;                 /* batched solves: workspace waves */
+    int graph_replay;          /* batched solves: 1 if the call replayed a captured graph */
+    int resolved_instances;    /* batched solves: instances re-solved one by one */
 } qx_stats;
 
 /* Per-target state reported in `state` arrays. */
@@ -189,6 +192,35 @@ QXGW_API qx_status qx_g0w0_cd_ov_dev(qx_handle h, const qx_g0w0_desc* d,
                                      const double* static_terms, const double* guess,
                                      double* qp_energy, double* z_factor, double* sigma_re,
                                      double* sigma_im, int* state, qx_stats* stats);
+/* ---- Batched solves (no reference counterpart: the reference solves one system per call;
+ * this is for callers with many small systems, e.g. a molecule set or a screening loop) -- */
+/* One instance of a batch.  Device pointers except `targets`. */
+typedef struct {
+    int nocc, nvir, naux;
+    int ntarget;
+    const double* mo_energy;      /* [nocc + nvir] */
+    const double* B_ov;           /* [nocc * nvir][naux], ov-only tensor */
+    const int* targets;           /* [ntarget] orbital indices, HOST memory */
+    const double* static_terms;   /* [ntarget] Sigma_x - V_xc, or NULL for zero */
+} qx_batch_instance;
+
+/* qx_g0w0_cd_ov_dev for many instances of different sizes in grouped launches: one
+ * persistent Pi launch over every instance's tiles and frequencies, one launch per
+ * factorisation kernel and step over every instance that still has that block column, one
+ * Wc_nm launch pair and one Newton launch.  d supplies nfreq, x0, eta, precision (FP64 or
+ * BF16X3; the Wc_nm contraction is FP64) and newton; its sizes are ignored.  Outputs are
+ * device arrays of sum(ntarget) entries, instance after instance in the given order.  The
+ * first call with a batch (sizes, pointers, targets, options, outputs) runs the launches
+ * directly and tunes them; later calls with the same batch replay them as a captured HIP
+ * graph, so they follow the current contents of mo_energy, B_ov and static_terms.
+ * Instances whose Cholesky fails at a frequency, or with a target whose Newton iterates
+ * reach a residue pole, are re-solved by qx_g0w0_cd_ov_dev.  Instances run in waves that fit
+ * the workspace budget (environment QXGW_BATCH_GB, default 40% of device memory). */
+QXGW_API qx_status qx_g0w0_batch_ov_dev(qx_handle h, const qx_g0w0_desc* d, int ninst,
+                                        const qx_batch_instance* inst, double* qp_energy,
+                                        double* z_factor, double* sigma_re, double* sigma_im,
+                                        int* state, qx_stats* stats);
+
 /* ---- Adjoint (no reference counterpart: the reference has no derivatives) -------------- */
 /* grad_B[N_ov][naux] = dL/dB_ov and grad_eps[nmo] = dL/d mo_energy for
  * L = sum_t grad_qp[t] * qp_energy[t], where qp_energy comes from qx_g0w0_cd_ov_dev (FP64) with

@@ -26,7 +26,7 @@ EXPORTED = [
     "qx_release_workspace", "qx_newton_config_default", "qx_g0w0_desc_default",
     "qx_gauss_legendre_grid", "qx_scaled_legendre_roots", "qx_rho_response_dev",
     "qx_screened_wc_dev", "qx_wmn_imag_dev", "qx_sigma_cd_dev", "qx_g0w0_cd_dev", "qx_g0w0_cd_ov_dev", "qx_g0w0_cd",
-    "qx_set_profiling", "qx_get_profile", "qx_g0w0_adjoint_ov_dev",
+    "qx_set_profiling", "qx_get_profile", "qx_g0w0_adjoint_ov_dev", "qx_g0w0_batch_ov_dev",
 ]
 
 
@@ -54,6 +54,15 @@ class Stats(C.Structure):
         ("chol_fallbacks", C.c_int), ("singular_fallbacks", C.c_int),
         ("residue_targets", C.c_int), ("residue_jobs", C.c_int),
         ("newton_failures", C.c_int), ("seconds", C.c_double), ("pi_variant", C.c_int),
+        ("waves", C.c_int), ("graph_replay", C.c_int), ("resolved_instances", C.c_int),
+    ]
+
+
+class BatchInstance(C.Structure):
+    _fields_ = [
+        ("nocc", C.c_int), ("nvir", C.c_int), ("naux", C.c_int), ("ntarget", C.c_int),
+        ("mo_energy", C.c_void_p), ("B_ov", C.c_void_p), ("targets", C.c_void_p),
+        ("static_terms", C.c_void_p),
     ]
 
 
@@ -106,6 +115,9 @@ def load():
         lib.qx_g0w0_cd_ov_dev.argtypes = [vp, C.POINTER(G0W0Desc), dp, dp, dp, ip, dp, dp, dp, dp,
                                           dp, dp, ip, C.POINTER(Stats)]
         lib.qx_g0w0_adjoint_ov_dev.argtypes = [vp, C.POINTER(G0W0Desc), dp, dp, ip, dp, dp, dp, dp]
+        lib.qx_g0w0_batch_ov_dev.argtypes = [vp, C.POINTER(G0W0Desc), C.c_int,
+                                             C.POINTER(BatchInstance), dp, dp, dp, dp, ip,
+                                             C.POINTER(Stats)]
         lib.qx_set_profiling.argtypes = [vp, C.c_int]
         lib.qx_get_profile.argtypes = [vp, dp, dp]
         non_status = ("qx_destroy", "qx_last_error", "qx_newton_config_default",

@@ -16,6 +16,8 @@
 // explicit inverse is formed only for the stage API.
 #include "qx_internal.h"
 
+#include <algorithm>
+
 namespace qx {
 namespace {
 
@@ -63,78 +65,79 @@ __device__ __forceinline__ void for_acc(const d4 (&acc)[4], int wave, int lane, 
 }
 
 // Diagonal block j: L_jj = chol(A_jj) and Linv_jj, one wavefront per frequency.
-// Factorisation: lane r keeps row r in registers; at step k the raw column k is broadcast
-// through LDS once (all lanes read the same address, so the reads are broadcasts) and every
-// lane updates r[c] -= (r[k] / d) * col[c].  Inverse: L is staged in LDS and lane c forms
-// column c of Linv by forward substitution with broadcast reads of the rows of L (two
-// partial sums break the FMA dependency chain).  Loops are fully unrolled, so the register
-// arrays are statically indexed.  Linv_jj is written to linv_diag and to X_jj.
-__global__ __launch_bounds__(64) void potrf_diag_kernel(double* __restrict__ a,
-                                                        double* __restrict__ x,
-                                                        double* __restrict__ linv_diag,
-                                                        int npad, int j, int* __restrict__ fail) {
-    __shared__ double col[NB];
-    __shared__ double Ls[NB][P];
-    const int f = blockIdx.x, lane = threadIdx.x;
+// Both loops over the 64 pivots stay rolled (fully unrolled, the body is ~100 KB of code and
+// runs instruction-fetch bound).  Registers keep static indices by rotation: at pivot k,
+// r[c] holds row `lane`, column k + c of the partially eliminated block; each step applies
+// the rank-1 update and shifts the row down by one slot.  The LDS vectors are stored
+// pre-shifted the same way (entry c = row k + c), so every read has a static, 16-byte
+// aligned offset; slots past row 63 are written as zero.  Factorisation: column k is
+// broadcast through `col` (all lanes read the same addresses).  Inverse: lane c solves
+// L x = e_c by column-oriented forward substitution with the same rotation, reading
+// column k of L from Ls[k][.].
+// col: NB doubles of LDS, Ls: NB * NB doubles of LDS (both 16-byte aligned).
+__device__ __forceinline__ void diag_body(double* __restrict__ a, double* __restrict__ x,
+                                          double* __restrict__ linv_diag, int npad, int j,
+                                          int* __restrict__ fail, int f, double* col,
+                                          double* Ls) {
+    const int lane = threadIdx.x;
     const int nblk = npad / NB;
     const size_t off = (size_t)f * npad * npad + (size_t)j * NB * npad + (size_t)j * NB;
     double* A = a + off;
     double* X = x + off;
+    double* D = linv_diag + ((size_t)f * nblk + j) * NB * NB;
     bool bad = false;
     {
         double r[NB];
         #pragma unroll
         for (int c = 0; c < NB; ++c) r[c] = A[(size_t)lane * npad + c];
-        #pragma unroll
         for (int k = 0; k < NB; ++k) {
-            col[lane] = r[k];                      // raw column k (rows >= k meaningful)
+            const int slot = lane >= k ? lane - k : lane - k + NB;   // every slot written once
+            col[slot] = lane >= k ? r[0] : 0.0;
             __syncthreads();
-            double d = col[k];
+            double d = col[0];
             if (!(d > 0.0)) { bad = true; d = 1.0; }
-            const double sd = sqrt(d);
-            const double t = r[k] / d;             // L[lane][k] / L[k][k]
+            const double sd = sqrt(d), isd = 1.0 / sd;
+            const double l = lane > k ? r[0] * isd : sd;             // L[lane][k] for lane >= k
+            Ls[k * NB + slot] = lane >= k ? l : 0.0;
+            const double t = lane > k ? l * isd : 0.0;
             #pragma unroll
-            for (int c = k + 1; c < NB; ++c) r[c] -= t * col[c];
-            r[k] = lane == k ? sd : r[k] / sd;
+            for (int c = 1; c < NB; ++c) r[c - 1] = r[c] - t * col[c];
             __syncthreads();
         }
-        #pragma unroll
-        for (int c = 0; c < NB; ++c) Ls[lane][c] = c <= lane ? r[c] : 0.0;
     }
-    __syncthreads();
-    double* D = linv_diag + ((size_t)f * nblk + j) * NB * NB;
-    for (int r = 0; r < NB; ++r) A[(size_t)r * npad + lane] = Ls[r][lane];
+    for (int q = 0; q < NB; ++q) A[(size_t)q * npad + lane] = q >= lane ? Ls[lane * NB + q - lane] : 0.0;
     {
-        double li[NB];
+        double y[NB];
         #pragma unroll
-        for (int q = 0; q < NB; ++q) {
-            double s0 = lane == q ? 1.0 : 0.0, s1 = 0.0;
+        for (int c = 0; c < NB; ++c) y[c] = c == lane ? 1.0 : 0.0;
+        for (int k = 0; k < NB; ++k) {
+            const double* Lk = Ls + k * NB;      // Lk[c] = L[k + c][k]
+            const double yk = y[0] / Lk[0];      // Linv[k][lane]
+            X[(size_t)k * npad + lane] = yk;
+            D[k * NB + lane] = yk;
             #pragma unroll
-            for (int k = 0; k < q; k += 2) {
-                s0 -= Ls[q][k] * li[k];
-                if (k + 1 < q) s1 -= Ls[q][k + 1] * li[k + 1];
-            }
-            li[q] = (s0 + s1) / Ls[q][q];
-        }
-        #pragma unroll
-        for (int q = 0; q < NB; ++q) {         // li[q] = Linv[q][lane]
-            X[(size_t)q * npad + lane] = li[q];
-            D[q * NB + lane] = li[q];
+            for (int c = 1; c < NB; ++c) y[c - 1] = y[c] - Lk[c] * yk;
         }
     }
     if (__any(bad) && lane == 0) fail[f] = 1;
 }
 
+__global__ __launch_bounds__(64) void potrf_diag_kernel(double* __restrict__ a,
+                                                        double* __restrict__ x,
+                                                        double* __restrict__ linv_diag,
+                                                        int npad, int j, int* __restrict__ fail) {
+    __shared__ __attribute__((aligned(16))) double col[NB];
+    __shared__ __attribute__((aligned(16))) double Ls[NB * NB];
+    diag_body(a, x, linv_diag, npad, j, fail, blockIdx.x, col, Ls);
+}
+
 // After the diagonal factor of step j:
 //   blockIdx.x <  j : finish block (j, bx) of X = L^-1:  X_j,bx = -Linv_jj S_j,bx
 //   blockIdx.x >= j : panel block i = j+1+(bx-j):         L_ij  =  A_ij Linv_jj^T
-__global__ __launch_bounds__(256) void potrf_rowpanel_kernel(double* __restrict__ a,
-                                                             double* __restrict__ x,
-                                                             const double* __restrict__ linv_diag,
-                                                             int npad, int j) {
-    __shared__ double As[NB][P];
-    __shared__ double Ls[NB][P];
-    const int bx = blockIdx.x, f = blockIdx.y;
+__device__ __forceinline__ void rowpanel_body(double* __restrict__ a, double* __restrict__ x,
+                                              const double* __restrict__ linv_diag, int npad,
+                                              int j, int bx, int f, double (*As)[P],
+                                              double (*Ls)[P]) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int nblk = npad / NB;
     const double* Djj = linv_diag + ((size_t)f * nblk + j) * NB * NB;
@@ -156,18 +159,25 @@ __global__ __launch_bounds__(256) void potrf_rowpanel_kernel(double* __restrict_
     }
 }
 
+__global__ __launch_bounds__(256) void potrf_rowpanel_kernel(double* __restrict__ a,
+                                                             double* __restrict__ x,
+                                                             const double* __restrict__ linv_diag,
+                                                             int npad, int j) {
+    __shared__ double As[NB][P];
+    __shared__ double Ls[NB][P];
+    rowpanel_body(a, x, linv_diag, npad, j, blockIdx.x, blockIdx.y, As, Ls);
+}
+
 // Left-looking update of step j, one 64x64 output tile per workgroup, K streamed through
 // LDS in chunks of 32 with the next chunk's global loads in flight during the MFMAs:
 //   bx <  nblk - j : Cholesky tile i = j + bx:   A_ij -= sum_{m<j} L_im L_jm^T   (K = 64 j)
 //   bx >= nblk - j : inverse sums k = bx - (nblk - j) < j:
 //                    X_jk := S_jk = sum_{k<=m<j} L_jm X_mk                    (K = 64 (j - k))
 constexpr int GK = 32;
-__global__ __launch_bounds__(256) void potrf_colupdate_kernel(double* __restrict__ a,
-                                                              double* __restrict__ x, int npad,
-                                                              int j) {
-    __shared__ double As[NB][GK + 1];        // [row][k]
-    __shared__ double Bs[NB][GK + 1];        // NT: [col][k];  NN: [k][col] (first GK rows)
-    const int bx = blockIdx.x, f = blockIdx.y;
+// As: [row][k]; Bs: NT [col][k], NN [k][col] (first GK rows), both NB x (GK + 1) in LDS.
+__device__ __forceinline__ void colupdate_body(double* __restrict__ a, double* __restrict__ x,
+                                               int npad, int j, int bx, int f,
+                                               double (*As)[GK + 1], double (*Bs)[GK + 1]) {
     const int nblk = npad / NB;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     double* Af = a + (size_t)f * npad * npad;
@@ -230,6 +240,48 @@ __global__ __launch_bounds__(256) void potrf_colupdate_kernel(double* __restrict
         double* C = Xf + (size_t)j * NB * npad + (size_t)k * NB;
         for_acc(acc, wave, lane, [&](int r, int c, double v) { C[(size_t)r * npad + c] = v; });
     }
+}
+
+__global__ __launch_bounds__(256) void potrf_colupdate_kernel(double* __restrict__ a,
+                                                              double* __restrict__ x, int npad,
+                                                              int j) {
+    __shared__ double As[NB][GK + 1];
+    __shared__ double Bs[NB][GK + 1];
+    colupdate_body(a, x, npad, j, blockIdx.x, blockIdx.y, As, Bs);
+}
+
+// ---- Grouped forms: one launch per step covers every active instance of a batch --------
+// Instances are sorted by block count, descending, so at step j the active ones (nblk > j)
+// are a prefix of the table.  off[q] is the first work item of instance q in a launch
+// whose per-instance grid is (items per frequency) x nf.
+__global__ __launch_bounds__(256) void potrf_colupdate_group_kernel(
+    const PotrfInst* __restrict__ inst, const int* __restrict__ off, int nact, int j) {
+    __shared__ double As[NB][GK + 1];
+    __shared__ double Bs[NB][GK + 1];
+    const int w = blockIdx.x;
+    const int q = find_inst(off, nact, w);
+    const PotrfInst in = inst[q];
+    const int nblk = in.npad / NB, local = w - off[q];
+    colupdate_body(in.a, in.x, in.npad, j, local % nblk, local / nblk, As, Bs);
+}
+
+__global__ __launch_bounds__(64) void potrf_diag_group_kernel(const PotrfInst* __restrict__ inst,
+                                                              int nf, int j) {
+    __shared__ __attribute__((aligned(16))) double col[NB];
+    __shared__ __attribute__((aligned(16))) double Ls[NB * NB];
+    const PotrfInst in = inst[blockIdx.x / nf];
+    diag_body(in.a, in.x, in.linv, in.npad, j, in.fail, blockIdx.x % nf, col, Ls);
+}
+
+__global__ __launch_bounds__(256) void potrf_rowpanel_group_kernel(
+    const PotrfInst* __restrict__ inst, const int* __restrict__ off, int nact, int j) {
+    __shared__ double As[NB][P];
+    __shared__ double Ls[NB][P];
+    const int w = blockIdx.x;
+    const int q = find_inst(off, nact, w);
+    const PotrfInst in = inst[q];
+    const int nb1 = in.npad / NB - 1, local = w - off[q];
+    rowpanel_body(in.a, in.x, in.linv, in.npad, j, local % nb1, local / nb1, As, Ls);
 }
 
 // Wc = X^T X - I over the first naux rows/cols, full symmetric output [nf][naux][naux].
@@ -349,18 +401,61 @@ __global__ void gj_finish_kernel(const double* __restrict__ work, double* __rest
 }  // namespace
 
 hipError_t launch_potrf_batched(double* a, double* x, double* linv_diag, int npad, int nf,
-                                int* fail, hipStream_t st) {
+                                int* fail, hipStream_t st, const StreamFan* fan, int groups) {
     const int nblk = npad / NB;
+    int ng = fan ? std::min(groups, fan->n + 1) : 1;
+    ng = std::max(1, std::min(ng, nf));
+    hipStream_t sg[StreamFan::kMax + 1];
+    sg[0] = st;
+    if (ng > 1) {
+        hipError_t e = hipEventRecord(fan->fork, st);
+        for (int g = 1; g < ng && e == hipSuccess; ++g) {
+            sg[g] = fan->side[g - 1];
+            e = hipStreamWaitEvent(sg[g], fan->fork, 0);
+        }
+        if (e != hipSuccess) return e;
+    }
+    const size_t mat = (size_t)npad * npad, dblk = (size_t)nblk * NB * NB;
     for (int j = 0; j < nblk; ++j) {
-        if (j > 0)
-            hipLaunchKernelGGL(potrf_colupdate_kernel, dim3(nblk, nf), dim3(256), 0, st, a, x,
-                               npad, j);
-        hipLaunchKernelGGL(potrf_diag_kernel, dim3(nf), dim3(64), 0, st, a, x, linv_diag, npad, j,
-                           fail);
-        const int T = nblk - j - 1;
-        if (j + T > 0)
-            hipLaunchKernelGGL(potrf_rowpanel_kernel, dim3(j + T, nf), dim3(256), 0, st, a, x,
-                               linv_diag, npad, j);
+        for (int g = 0; g < ng; ++g) {
+            const int f0 = g * nf / ng, nfg = (g + 1) * nf / ng - f0;
+            double* ag = a + f0 * mat;
+            double* xg = x + f0 * mat;
+            double* dg = linv_diag + f0 * dblk;
+            if (j > 0)
+                hipLaunchKernelGGL(potrf_colupdate_kernel, dim3(nblk, nfg), dim3(256), 0, sg[g],
+                                   ag, xg, npad, j);
+            hipLaunchKernelGGL(potrf_diag_kernel, dim3(nfg), dim3(64), 0, sg[g], ag, xg, dg,
+                               npad, j, fail + f0);
+            if (nblk > 1)
+                hipLaunchKernelGGL(potrf_rowpanel_kernel, dim3(nblk - 1, nfg), dim3(256), 0,
+                                   sg[g], ag, xg, dg, npad, j);
+        }
+    }
+    for (int g = 1; g < ng; ++g) {
+        hipError_t e = hipEventRecord(fan->join[g - 1], sg[g]);
+        if (e == hipSuccess) e = hipStreamWaitEvent(st, fan->join[g - 1], 0);
+        if (e != hipSuccess) return e;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_potrf_group(const PotrfInst* inst_dev, const int* nblk_sorted, int ninst,
+                              int nf, const int* cu_off_dev, const int* rp_off_dev,
+                              const int* cu_off_host, const int* rp_off_host, hipStream_t st) {
+    if (ninst <= 0) return hipSuccess;
+    const int maxblk = nblk_sorted[0];
+    int nact = ninst;
+    for (int j = 0; j < maxblk; ++j) {
+        while (nact > 0 && nblk_sorted[nact - 1] <= j) --nact;
+        if (j > 0 && cu_off_host[nact] > 0)
+            hipLaunchKernelGGL(potrf_colupdate_group_kernel, dim3(cu_off_host[nact]), dim3(256), 0,
+                               st, inst_dev, cu_off_dev, nact, j);
+        hipLaunchKernelGGL(potrf_diag_group_kernel, dim3(nact * nf), dim3(64), 0, st, inst_dev,
+                           nf, j);
+        if (rp_off_host[nact] > 0)
+            hipLaunchKernelGGL(potrf_rowpanel_group_kernel, dim3(rp_off_host[nact]), dim3(256), 0,
+                               st, inst_dev, rp_off_dev, nact, j);
     }
     return hipGetLastError();
 }

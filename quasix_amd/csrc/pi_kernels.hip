@@ -18,6 +18,8 @@
 // or above the diagonal are skipped.
 #include "qx_internal.h"
 
+#include <algorithm>
+
 namespace qx {
 namespace {
 
@@ -127,17 +129,13 @@ __device__ __forceinline__ bool block_live(int I0, int J0, int wr, int wc, int b
     return r < naux && c < naux && (I0 != J0 || r >= c);
 }
 
+// One 128x128 lower-triangle tile (I0, J0) of frequency f.  As/Bs: KC x (TILE + 2) in LDS.
 template <typename T, int KC>
-__global__ __launch_bounds__(256, 2) void pi_syrk_kernel(
-    const double* __restrict__ B, const double* __restrict__ s, int nov, int naux, int npad,
-    int nf, int ntiles, int sld, double* __restrict__ out, PiOut mode) {
-    __shared__ T As[KC][TILE + 2];
-    __shared__ T Bs[KC][TILE + 2];
-    const int work = xcd_remap(blockIdx.x, ntiles * nf);
-    const int tile = work / nf, f = work % nf;
-    int I, J;
-    tri_index(tile, I, J);
-    const int I0 = I * TILE, J0 = J * TILE;
+__device__ __forceinline__ void pi_tile_body(const double* __restrict__ B,
+                                             const double* __restrict__ s, int nov, int naux,
+                                             int npad, int sld, double* __restrict__ out,
+                                             PiOut mode, int I0, int J0, int f,
+                                             T (*As)[TILE + 2], T (*Bs)[TILE + 2]) {
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);   // wave-uniform (scalar)
     const int wr = wave >> 1, wc = wave & 1;
@@ -169,11 +167,6 @@ __global__ __launch_bounds__(256, 2) void pi_syrk_kernel(
         #pragma unroll
         for (int bj = 0; bj < 4; ++bj) acc[bi][bj] = typename Mfma<T>::acc_t{0, 0, 0, 0};
 
-    bool all_live = true;
-    #pragma unroll
-    for (int bi = 0; bi < 4; ++bi)
-        #pragma unroll
-        for (int bj = 0; bj < 4; ++bj) all_live &= block_live(I0, J0, wr, wc, bi, bj, naux);
     fetch(0);
     for (int r0 = 0; r0 < nov; r0 += KC) {
         #pragma unroll
@@ -192,24 +185,29 @@ __global__ __launch_bounds__(256, 2) void pi_syrk_kernel(
                 av[q] = As[k][wr * 64 + q * 16 + (lane & 15)];
                 bv[q] = Bs[k][wc * 64 + q * 16 + (lane & 15)];
             }
-            if (all_live) {
+            #pragma unroll
+            for (int bi = 0; bi < 4; ++bi)
                 #pragma unroll
-                for (int bi = 0; bi < 4; ++bi)
-                    #pragma unroll
-                    for (int bj = 0; bj < 4; ++bj)
+                for (int bj = 0; bj < 4; ++bj)
+                    if (block_live(I0, J0, wr, wc, bi, bj, naux))
                         acc[bi][bj] = Mfma<T>::run(av[bi], bv[bj], acc[bi][bj]);
-            } else {
-                #pragma unroll
-                for (int bi = 0; bi < 4; ++bi)
-                    #pragma unroll
-                    for (int bj = 0; bj < 4; ++bj)
-                        if (block_live(I0, J0, wr, wc, bi, bj, naux))
-                            acc[bi][bj] = Mfma<T>::run(av[bi], bv[bj], acc[bi][bj]);
-            }
         }
         __syncthreads();
     }
     store_tile<T, sizeof(T) == 8>(acc, I0, J0, wr, wc, lane, npad, naux, f, mode, out, 1.0);
+}
+
+template <typename T, int KC>
+__global__ __launch_bounds__(256, 2) void pi_syrk_kernel(
+    const double* __restrict__ B, const double* __restrict__ s, int nov, int naux, int npad,
+    int nf, int ntiles, int sld, double* __restrict__ out, PiOut mode) {
+    __shared__ T As[KC][TILE + 2];
+    __shared__ T Bs[KC][TILE + 2];
+    const int work = xcd_remap(blockIdx.x, ntiles * nf);
+    const int tile = work / nf, f = work % nf;
+    int I, J;
+    tri_index(tile, I, J);
+    pi_tile_body<T, KC>(B, s, nov, naux, npad, sld, out, mode, I * TILE, J * TILE, f, As, Bs);
 }
 
 // ---- BF16x3 kernel ----------------------------------------------------------------------
@@ -218,8 +216,8 @@ __global__ __launch_bounds__(256, 2) void pi_syrk_kernel(
 typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 
 // The three split products over a wave's 4x4 blocks, product-outer so that 16 independent
-// MFMAs separate dependent accumulations.  Guarded: skip blocks outside the triangle/naux.
-template <bool Guarded>
+// MFMAs separate dependent accumulations.  Blocks outside the triangle/naux are skipped
+// (the wave index is scalar, so the skips are scalar branches, not exec masking).
 __device__ __forceinline__ void mfma_bf16x3(f4 (&acc)[4][4], const bf16x8 (&ah)[4],
                                             const bf16x8 (&al)[4], const bf16x8 (&ch)[4],
                                             const bf16x8 (&cl)[4], const bool (&live)[4][4]) {
@@ -227,19 +225,19 @@ __device__ __forceinline__ void mfma_bf16x3(f4 (&acc)[4][4], const bf16x8 (&ah)[
     for (int bi = 0; bi < 4; ++bi)
         #pragma unroll
         for (int bj = 0; bj < 4; ++bj)
-            if (!Guarded || live[bi][bj])
+            if (live[bi][bj])
                 acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[bi], ch[bj], acc[bi][bj], 0, 0, 0);
     #pragma unroll
     for (int bi = 0; bi < 4; ++bi)
         #pragma unroll
         for (int bj = 0; bj < 4; ++bj)
-            if (!Guarded || live[bi][bj])
+            if (live[bi][bj])
                 acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[bi], cl[bj], acc[bi][bj], 0, 0, 0);
     #pragma unroll
     for (int bi = 0; bi < 4; ++bi)
         #pragma unroll
         for (int bj = 0; bj < 4; ++bj)
-            if (!Guarded || live[bi][bj])
+            if (live[bi][bj])
                 acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[bi], ch[bj], acc[bi][bj], 0, 0, 0);
 }
 
@@ -274,23 +272,14 @@ __global__ void pi_prepare_mixed_kernel(const double* __restrict__ B, int nov, i
 // rows and 8 consecutive k of each; a row's KB values are read by KB/8 consecutive threads
 // (coalesced), and a thread needs only its 8 frequency weights per chunk.
 template <int KB>
-__global__ __launch_bounds__(256, 2) void pi_syrk_bf16x3_kernel(
+__device__ __forceinline__ void pi_tile_bf16x3_body(
     const uint16_t* __restrict__ bt_hi, const uint16_t* __restrict__ bt_lo,
     const float* __restrict__ bt_f32, const float* __restrict__ s32, int kpad, int naux,
-    int npad, int nf, int ntiles, double* __restrict__ out, PiOut mode) {
-    constexpr int KBP = KB + 8;                  // padded LDS pitch: conflict-free rows
+    int npad, double* __restrict__ out, PiOut mode, int I0, int J0, int f,
+    __bf16 (*Ah)[KB + 8], __bf16 (*Al)[KB + 8], __bf16 (*Ch)[KB + 8], __bf16 (*Cl)[KB + 8]) {
     constexpr int TPR = KB / 8;                  // threads per row
     constexpr int RSTEP = 256 / TPR;             // row stride between a thread's rows
     constexpr int R = TILE / RSTEP;              // rows per thread
-    __shared__ __attribute__((aligned(16))) __bf16 Ah[TILE][KBP];
-    __shared__ __attribute__((aligned(16))) __bf16 Al[TILE][KBP];
-    __shared__ __attribute__((aligned(16))) __bf16 Ch[TILE][KBP];
-    __shared__ __attribute__((aligned(16))) __bf16 Cl[TILE][KBP];
-    const int work = xcd_remap(blockIdx.x, ntiles * nf);
-    const int tile = work / nf, f = work % nf;
-    int I, J;
-    tri_index(tile, I, J);
-    const int I0 = I * TILE, J0 = J * TILE;
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);   // wave-uniform (scalar)
     const int wr = wave >> 1, wc = wave & 1;
@@ -325,11 +314,6 @@ __global__ __launch_bounds__(256, 2) void pi_syrk_bf16x3_kernel(
     for (int bi = 0; bi < 4; ++bi)
         #pragma unroll
         for (int bj = 0; bj < 4; ++bj) live[bi][bj] = block_live(I0, J0, wr, wc, bi, bj, naux);
-    bool all_live = true;
-    #pragma unroll
-    for (int bi = 0; bi < 4; ++bi)
-        #pragma unroll
-        for (int bj = 0; bj < 4; ++bj) all_live &= live[bi][bj];
 
     fetch(0);
     for (int k0 = 0; k0 < kpad; k0 += KB) {
@@ -362,12 +346,79 @@ __global__ __launch_bounds__(256, 2) void pi_syrk_bf16x3_kernel(
                 cl[q] = *reinterpret_cast<const bf16x8*>(&Cl[wc * 64 + q * 16 + r16][kq]);
             }
             // product-outer order: 16 independent MFMAs between dependent accumulations
-            if (all_live) mfma_bf16x3<false>(acc, ah, al, ch, cl, live);
-            else mfma_bf16x3<true>(acc, ah, al, ch, cl, live);
+            mfma_bf16x3(acc, ah, al, ch, cl, live);
         }
         __syncthreads();
     }
     store_tile<float, false>(acc, I0, J0, wr, wc, lane, npad, naux, f, mode, out, 1.0);
+}
+
+template <int KB>
+__global__ __launch_bounds__(256, 2) void pi_syrk_bf16x3_kernel(
+    const uint16_t* __restrict__ bt_hi, const uint16_t* __restrict__ bt_lo,
+    const float* __restrict__ bt_f32, const float* __restrict__ s32, int kpad, int naux,
+    int npad, int nf, int ntiles, double* __restrict__ out, PiOut mode) {
+    constexpr int KBP = KB + 8;                  // padded LDS pitch: conflict-free rows
+    __shared__ __attribute__((aligned(16))) __bf16 Ah[TILE][KBP];
+    __shared__ __attribute__((aligned(16))) __bf16 Al[TILE][KBP];
+    __shared__ __attribute__((aligned(16))) __bf16 Ch[TILE][KBP];
+    __shared__ __attribute__((aligned(16))) __bf16 Cl[TILE][KBP];
+    const int work = xcd_remap(blockIdx.x, ntiles * nf);
+    const int tile = work / nf, f = work % nf;
+    int I, J;
+    tri_index(tile, I, J);
+    pi_tile_bf16x3_body<KB>(bt_hi, bt_lo, bt_f32, s32, kpad, naux, npad, out, mode, I * TILE,
+                            J * TILE, f, Ah, Al, Ch, Cl);
+}
+
+// ---- Grouped (batched) kernels ------------------------------------------------------------
+// Persistent: gridDim.x (a multiple of 8) workgroups walk the concatenated work items of
+// all instances (instance-major, then tile, then frequency).  In every sweep the G/8
+// workgroups of one XCD take G/8 consecutive items, so the frequencies of one tile, which
+// stream the same operand rows, share that XCD's L2.  Every workgroup leaves the loop once
+// its next item is past the end.
+__device__ __forceinline__ bool group_item(const PiInst* __restrict__ inst,
+                                           const int* __restrict__ off, int ninst, int nf,
+                                           int total, int it, PiInst& in, int& I0, int& J0,
+                                           int& f) {
+    const int G = gridDim.x;
+    const int w = it * G + (blockIdx.x & 7) * (G >> 3) + (blockIdx.x >> 3);
+    if (w >= total) return false;
+    const int q = find_inst(off, ninst, w);
+    in = inst[q];
+    const int local = w - off[q];
+    f = local % nf;
+    int I, J;
+    tri_index(local / nf, I, J);
+    I0 = I * TILE;
+    J0 = J * TILE;
+    return true;
+}
+
+template <int KB>
+__global__ __launch_bounds__(256, 2) void pi_group_bf16x3_kernel(
+    const PiInst* __restrict__ inst, const int* __restrict__ off, int ninst, int nf, int total) {
+    constexpr int KBP = KB + 8;
+    __shared__ __attribute__((aligned(16))) __bf16 Ah[TILE][KBP];
+    __shared__ __attribute__((aligned(16))) __bf16 Al[TILE][KBP];
+    __shared__ __attribute__((aligned(16))) __bf16 Ch[TILE][KBP];
+    __shared__ __attribute__((aligned(16))) __bf16 Cl[TILE][KBP];
+    PiInst in;
+    int I0, J0, f;
+    for (int it = 0; group_item(inst, off, ninst, nf, total, it, in, I0, J0, f); ++it)
+        pi_tile_bf16x3_body<KB>(in.bt_hi, in.bt_lo, in.bt_f32, in.s32, in.kpad, in.naux, in.npad,
+                                in.out, PiOut::EpsLower, I0, J0, f, Ah, Al, Ch, Cl);
+}
+
+__global__ __launch_bounds__(256, 2) void pi_group_f64_kernel(
+    const PiInst* __restrict__ inst, const int* __restrict__ off, int ninst, int nf, int total) {
+    __shared__ double As[16][TILE + 2];
+    __shared__ double Bs[16][TILE + 2];
+    PiInst in;
+    int I0, J0, f;
+    for (int it = 0; group_item(inst, off, ninst, nf, total, it, in, I0, J0, f); ++it)
+        pi_tile_body<double, 16>(in.B, in.s, in.nov, in.naux, in.npad, in.kpad, in.out,
+                                 PiOut::EpsLower, I0, J0, f, As, Bs);
 }
 
 // Wide variant: 256 (A, unscaled) x 128 (C = s B, split on the fly) tile per 512-thread
@@ -436,14 +487,11 @@ __global__ __launch_bounds__(512, 1) void pi_syrk_bf16x3_wide_kernel(
             const int r = I0 + wr * 64 + bi * 16, c = J0 + wc * 64 + bj * 16;
             live[bi][bj] = r < naux && c < naux && r >= c;
         }
-    bool any = false, all_live = true;
+    bool any = false;
     #pragma unroll
     for (int bi = 0; bi < 4; ++bi)
         #pragma unroll
-        for (int bj = 0; bj < 4; ++bj) {
-            any |= live[bi][bj];
-            all_live &= live[bi][bj];
-        }
+        for (int bj = 0; bj < 4; ++bj) any |= live[bi][bj];
 
     fetch(0);
     for (int k0 = 0; k0 < kpad; k0 += WKB) {
@@ -483,8 +531,7 @@ __global__ __launch_bounds__(512, 1) void pi_syrk_bf16x3_wide_kernel(
                     chv[q] = *reinterpret_cast<const bf16x8*>(&Ch[oc]);
                     clv[q] = *reinterpret_cast<const bf16x8*>(&Cl[oc]);
                 }
-                if (all_live) mfma_bf16x3<false>(acc, ah, al, chv, clv, live);
-                else mfma_bf16x3<true>(acc, ah, al, chv, clv, live);
+                mfma_bf16x3(acc, ah, al, chv, clv, live);
             }
         }
         __syncthreads();
@@ -563,5 +610,37 @@ hipError_t launch_pi_syrk(int precision, const double* B, const PiOperands& ops,
 }
 
 int pi_variant_count(int precision) { return precision == 2 ? 3 : 1; }
+
+hipError_t launch_pi_group(int precision, const PiInst* inst_dev, const int* off_dev, int ninst,
+                           int nf, int total, int grid, hipStream_t st, int variant) {
+    if (ninst <= 0 || total <= 0) return hipSuccess;
+    grid = std::max(8, grid / 8 * 8);
+    if (precision == 0)
+        hipLaunchKernelGGL(pi_group_f64_kernel, dim3(grid), dim3(256), 0, st, inst_dev, off_dev,
+                           ninst, nf, total);
+    else if (precision == 2 && variant == 1)
+        hipLaunchKernelGGL(pi_group_bf16x3_kernel<64>, dim3(grid), dim3(256), 0, st, inst_dev,
+                           off_dev, ninst, nf, total);
+    else if (precision == 2)
+        hipLaunchKernelGGL(pi_group_bf16x3_kernel<32>, dim3(grid), dim3(256), 0, st, inst_dev,
+                           off_dev, ninst, nf, total);
+    else
+        return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+int pi_group_variant_count(int precision) { return precision == 2 ? 2 : 1; }
+
+int pi_group_occupancy(int precision, int variant) {
+    int n = 0;
+    hipError_t e;
+    if (precision == 0)
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, pi_group_f64_kernel, 256, 0);
+    else if (variant == 1)
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, pi_group_bf16x3_kernel<64>, 256, 0);
+    else
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, pi_group_bf16x3_kernel<32>, 256, 0);
+    return e == hipSuccess && n > 0 ? n : 1;
+}
 
 }  // namespace qx

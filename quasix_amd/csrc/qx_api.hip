@@ -18,10 +18,16 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <string>
 #include <vector>
 
 using namespace qx;
+
+namespace {
+struct BatchPlan;
+void delete_batch_plan(BatchPlan* p);
+}  // namespace
 
 struct qx_context {
     int device = 0;
@@ -38,6 +44,10 @@ struct qx_context {
     // autotuner: best kernel variant per (kind, precision, shape) key
     std::map<std::string, int> tuned;
     bool tune_loaded = false;
+    // side streams for the grouped factorisation (created on first use)
+    StreamFan fan;
+    // last batched solve: work tables, workspace layout and its captured graph
+    BatchPlan* batch = nullptr;
 };
 
 namespace {
@@ -371,6 +381,27 @@ int choose_variant(qx_handle h, const char* kind, int precision, int nvariants,
     return best;
 }
 
+// Side streams and the group count for the factorisation step loop (QXGW_POTRF_GROUPS,
+// default 3; 1 runs everything on the handle's stream).
+const StreamFan* potrf_fan(qx_handle h, int& groups) {
+    static const int env_groups = [] {
+        const char* v = getenv("QXGW_POTRF_GROUPS");
+        const int g = v ? atoi(v) : 3;
+        return std::max(1, std::min(g, StreamFan::kMax + 1));
+    }();
+    groups = env_groups;
+    if (groups > 1 && h->fan.n < groups - 1) {
+        if (!h->fan.fork) HIPCK(h, hipEventCreateWithFlags(&h->fan.fork, hipEventDisableTiming));
+        while (h->fan.n < groups - 1) {
+            const int g = h->fan.n;
+            HIPCK(h, hipStreamCreate(&h->fan.side[g]));
+            HIPCK(h, hipEventCreateWithFlags(&h->fan.join[g], hipEventDisableTiming));
+            h->fan.n = g + 1;
+        }
+    }
+    return &h->fan;
+}
+
 WnmResult build_wnm(qx_handle h, int precision, int nocc, int nvir, int naux,
                     const double* eps_dev, const double* B_ov, int nf, const double* freqs_dev,
                     int npair, const double* B_pair, double* wnm_out) {
@@ -417,7 +448,9 @@ WnmResult build_wnm(qx_handle h, int precision, int nocc, int nvir, int naux,
     double* x = ws<double>(h, "xinv", (size_t)nf * npad * npad);
     {
         Phase ph(h, QX_PHASE_POTRF);
-        HIPCK(h, launch_potrf_batched(eps_mat, x, linv, npad, nf, fail, st));
+        int groups = 1;
+        const StreamFan* fan = potrf_fan(h, groups);
+        HIPCK(h, launch_potrf_batched(eps_mat, x, linv, npad, nf, fail, st, fan, groups));
     }
     std::vector<int> fail_h(nf);
     d2h(h, fail_h.data(), fail, nf);
@@ -778,6 +811,13 @@ void qx_destroy(qx_handle h) {
         (void)hipEventDestroy(p.second.second);
     }
     for (hipEvent_t e : h->ev_free) (void)hipEventDestroy(e);
+    delete_batch_plan(h->batch);
+    for (int g = 0; g < h->fan.n; ++g) {
+        (void)hipStreamSynchronize(h->fan.side[g]);
+        (void)hipStreamDestroy(h->fan.side[g]);
+        (void)hipEventDestroy(h->fan.join[g]);
+    }
+    if (h->fan.fork) (void)hipEventDestroy(h->fan.fork);
     if (h->own_stream) (void)hipStreamDestroy(h->stream);
     delete h;
 }
@@ -832,6 +872,8 @@ qx_status qx_synchronize(qx_handle h) {
 qx_status qx_release_workspace(qx_handle h) {
     return guarded(h, [&] {
         HIPCK(h, hipStreamSynchronize(h->stream));
+        delete_batch_plan(h->batch);
+        h->batch = nullptr;
         for (auto& kv : h->bufs) HIPCK(h, hipFree(kv.second.first));
         h->bufs.clear();
         return QX_OK;
@@ -935,7 +977,9 @@ qx_status qx_screened_wc_dev(qx_handle h, int naux, int nfreq, const double* pi,
         int* fail = ws<int>(h, "chol_fail", nfreq);
         fill_i32(h, fail, 0, nfreq);
         double* x = ws<double>(h, "xinv", (size_t)nfreq * npad * npad);
-        HIPCK(h, launch_potrf_batched(eps, x, linv, npad, nfreq, fail, h->stream));
+        int groups = 1;
+        const StreamFan* fan = potrf_fan(h, groups);
+        HIPCK(h, launch_potrf_batched(eps, x, linv, npad, nfreq, fail, h->stream, fan, groups));
         std::vector<int> fh(nfreq);
         d2h(h, fh.data(), fail, nfreq);
         int nfail = 0;
@@ -1012,38 +1056,49 @@ qx_status qx_g0w0_cd_dev(qx_handle h, const qx_g0w0_desc* d, const double* mo_en
     });
 }
 
+namespace {
+// qx_g0w0_cd_ov_dev without the error guard: gathers the ov-only pair rows on the device.
+qx_status run_g0w0_ov(qx_handle h, const qx_g0w0_desc* d, const double* mo_energy,
+                      const double* mo_energy_ref, const double* B_ov, const int* targets,
+                      const double* static_terms, const double* guess, double* qp_energy,
+                      double* z_factor, double* sigma_re, double* sigma_im, int* state,
+                      qx_stats* stats) {
+    REQUIRE(h, d != nullptr && targets != nullptr, QX_ERR_INVALID_INPUT, "NULL argument");
+    const int nt = d->ntarget;
+    REQUIRE(h, nt > 0, QX_ERR_INVALID_INPUT, "ntarget must be positive");
+    std::vector<int> tg(nt), off(nt + 1, 0);
+    d2h(h, tg.data(), targets, nt);
+    for (int t = 0; t < nt; ++t) {
+        REQUIRE(h, tg[t] >= 0 && tg[t] < d->nocc + d->nvir, QX_ERR_INVALID_INPUT,
+                "target %d out of range", tg[t]);
+        off[t + 1] = off[t] + (tg[t] < d->nocc ? d->nvir : d->nocc);
+    }
+    const int npair = off[nt];
+    int* off_dev = ws<int>(h, "ov_off", nt + 1);
+    h2d(h, off_dev, off.data(), nt + 1);
+    int* pt = ws<int>(h, "ov_pt", npair);
+    int* pm = ws<int>(h, "ov_pm", npair);
+    double* rows = ws<double>(h, "ov_rows", (size_t)npair * d->naux);
+    const int maxcnt = std::max(d->nocc, d->nvir);
+    hipLaunchKernelGGL(ov_pairs_kernel, dim3(std::min(maxcnt, 1024), nt), dim3(256), 0,
+                       h->stream, B_ov, d->nocc, d->nvir, d->naux, targets, off_dev, nt, pt, pm,
+                       rows);
+    HIPCK(h, hipGetLastError());
+    qx_g0w0_desc dd = *d;
+    dd.npair = npair;
+    return run_g0w0(h, &dd, mo_energy, mo_energy_ref, B_ov, rows, pt, pm, targets, static_terms,
+                    guess, qp_energy, z_factor, sigma_re, sigma_im, state, stats);
+}
+}  // namespace
+
 qx_status qx_g0w0_cd_ov_dev(qx_handle h, const qx_g0w0_desc* d, const double* mo_energy,
                             const double* mo_energy_ref, const double* B_ov, const int* targets,
                             const double* static_terms, const double* guess, double* qp_energy,
                             double* z_factor, double* sigma_re, double* sigma_im, int* state,
                             qx_stats* stats) {
     return guarded(h, [&] {
-        REQUIRE(h, d != nullptr && targets != nullptr, QX_ERR_INVALID_INPUT, "NULL argument");
-        const int nt = d->ntarget;
-        REQUIRE(h, nt > 0, QX_ERR_INVALID_INPUT, "ntarget must be positive");
-        std::vector<int> tg(nt), off(nt + 1, 0);
-        d2h(h, tg.data(), targets, nt);
-        for (int t = 0; t < nt; ++t) {
-            REQUIRE(h, tg[t] >= 0 && tg[t] < d->nocc + d->nvir, QX_ERR_INVALID_INPUT,
-                    "target %d out of range", tg[t]);
-            off[t + 1] = off[t] + (tg[t] < d->nocc ? d->nvir : d->nocc);
-        }
-        const int npair = off[nt];
-        int* off_dev = ws<int>(h, "ov_off", nt + 1);
-        h2d(h, off_dev, off.data(), nt + 1);
-        int* pt = ws<int>(h, "ov_pt", npair);
-        int* pm = ws<int>(h, "ov_pm", npair);
-        double* rows = ws<double>(h, "ov_rows", (size_t)npair * d->naux);
-        const int maxcnt = std::max(d->nocc, d->nvir);
-        hipLaunchKernelGGL(ov_pairs_kernel, dim3(std::min(maxcnt, 1024), nt), dim3(256), 0,
-                           h->stream, B_ov, d->nocc, d->nvir, d->naux, targets, off_dev, nt, pt,
-                           pm, rows);
-        HIPCK(h, hipGetLastError());
-        qx_g0w0_desc dd = *d;
-        dd.npair = npair;
-        return run_g0w0(h, &dd, mo_energy, mo_energy_ref, B_ov, rows, pt, pm, targets,
-                        static_terms, guess, qp_energy, z_factor, sigma_re, sigma_im, state,
-                        stats);
+        return run_g0w0_ov(h, d, mo_energy, mo_energy_ref, B_ov, targets, static_terms, guess,
+                           qp_energy, z_factor, sigma_re, sigma_im, state, stats);
     });
 }
 
@@ -1203,3 +1258,522 @@ qx_status qx_g0w0_cd(qx_handle h, const qx_g0w0_desc* d, const double* mo_energy
 }
 
 }  // extern "C"
+
+// ---- Batched solves ----------------------------------------------------------------------
+// Many independent instances of different sizes share one frequency grid and one set of
+// options.  Instances are sorted by padded size (descending) and packed into waves whose
+// workspace fits a budget; each wave is a fixed sequence of launches: per-instance operand
+// preparation (weights, BF16 splits, pair rows), ONE persistent Pi launch over every tile
+// of every instance, one launch per factorisation kernel and step over the instances that
+// still have that block column, one Wc_nm launch pair and one Newton launch.  The sequence
+// reads its work tables from device memory, so it is captured once as a HIP graph and
+// replayed by later calls with the same batch.
+namespace {
+
+struct BatchInst {
+    int idx = 0;                          // position in the caller's array
+    int nocc = 0, nvir = 0, naux = 0, nov = 0, nt = 0, npair = 0;
+    int npad = 0, nblk = 0, kpad = 0, rows = 0, out_off = 0;
+    const double* mo_energy = nullptr;
+    const double* B_ov = nullptr;
+    const double* static_terms = nullptr; // caller's array or the zero block
+    std::vector<int> targets, pair_off;
+    // device memory (wave workspace / tables / persistent block)
+    double *s = nullptr, *eps = nullptr, *x = nullptr, *linv = nullptr, *rows_naux = nullptr,
+           *bp = nullptr, *wpart = nullptr, *wnm = nullptr;
+    float *s32 = nullptr, *f32 = nullptr;
+    uint16_t *hi = nullptr, *lo = nullptr;
+    int *pt = nullptr, *pm = nullptr, *fail = nullptr, *targets_dev = nullptr,
+        *pair_off_dev = nullptr;
+};
+
+struct BatchWave {
+    int b = 0, e = 0;                     // [b, e) in size order
+    int pi_total = 0, pi_variant = 0, pi_grid = 8, wn_total = 0, red_total = 0, t_total = 0;
+    PiInst* pi = nullptr;
+    int* pi_off = nullptr;
+    PotrfInst* po = nullptr;
+    int *cu_off = nullptr, *rp_off = nullptr;
+    std::vector<int> nblk, cu_h, rp_h;
+    WnmInst* wn = nullptr;
+    int *wn_off = nullptr, *red_off = nullptr;
+    NewtonInst* nw = nullptr;
+    int* t_off = nullptr;
+};
+
+struct BatchPlan {
+    std::vector<unsigned long long> key;
+    std::vector<BatchInst> inst;          // size order
+    std::vector<BatchWave> waves;
+    int nf = 0, precision = 0, ntot = 0, pi_grid = 0;
+    double eta = 0.0;
+    NewtonParams np{};
+    double *freqs = nullptr, *wts = nullptr;
+    int *state = nullptr, *iters = nullptr;
+    double *e_out = nullptr, *z_out = nullptr, *sre_out = nullptr, *sim_out = nullptr;
+    bool ran = false;                     // the direct (tuning) run has happened
+    hipGraphExec_t exec = nullptr;
+    ~BatchPlan() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+    }
+};
+
+void delete_batch_plan(BatchPlan* p) { delete p; }
+
+// Bump allocator over a device region; with base == nullptr it only measures.
+struct Bump {
+    char* base = nullptr;
+    size_t off = 0;
+    template <typename T>
+    T* take(size_t n) {
+        off = (off + 255) / 256 * 256;
+        T* p = base ? reinterpret_cast<T*>(base + off) : nullptr;
+        off += std::max<size_t>(n, 1) * sizeof(T);
+        return p;
+    }
+};
+
+void carve_instance(Bump& b, BatchInst& in, int nf, bool mixed) {
+    const size_t mat = (size_t)in.npad * in.npad;
+    in.s = b.take<double>((size_t)nf * in.kpad);
+    in.s32 = mixed ? b.take<float>((size_t)nf * in.kpad) : nullptr;
+    in.hi = mixed ? b.take<uint16_t>((size_t)in.rows * in.kpad) : nullptr;
+    in.lo = mixed ? b.take<uint16_t>((size_t)in.rows * in.kpad) : nullptr;
+    in.f32 = mixed ? b.take<float>((size_t)in.rows * in.kpad) : nullptr;
+    in.eps = b.take<double>((size_t)nf * mat);
+    in.x = b.take<double>((size_t)nf * mat);
+    in.linv = b.take<double>((size_t)nf * in.nblk * NB * NB);
+    in.rows_naux = b.take<double>((size_t)in.npair * in.naux);
+    in.bp = b.take<double>((size_t)in.npair * in.npad);
+    in.pt = b.take<int>(in.npair);
+    in.pm = b.take<int>(in.npair);
+    in.wpart = b.take<double>((size_t)nf * in.nblk * in.npair);
+    in.wnm = b.take<double>((size_t)nf * in.npair);
+}
+
+size_t batch_budget(qx_handle h) {
+    if (const char* v = std::getenv("QXGW_BATCH_GB")) {
+        const double gb = std::atof(v);
+        if (gb > 0) return (size_t)(gb * (1ull << 30));
+    }
+    size_t fr = 0, tot = 0;
+    HIPCK(h, hipMemGetInfo(&fr, &tot));
+    return tot / 10 * 4;
+}
+
+std::vector<unsigned long long> batch_key(const qx_g0w0_desc* d, int ninst,
+                                          const qx_batch_instance* inst, const void* const* outs,
+                                          int nouts) {
+    std::vector<unsigned long long> k;
+    auto put = [&](const void* p, size_t n) {
+        const unsigned char* c = static_cast<const unsigned char*>(p);
+        for (size_t i = 0; i < n; i += 8) {
+            unsigned long long v = 0;
+            std::memcpy(&v, c + i, std::min<size_t>(8, n - i));
+            k.push_back(v);
+        }
+    };
+    qx_g0w0_desc dd = *d;
+    dd.nocc = dd.nvir = dd.naux = dd.ntarget = dd.npair = 0;
+    put(&dd, sizeof dd);
+    k.push_back((unsigned long long)ninst);
+    for (int q = 0; q < ninst; ++q) {
+        const qx_batch_instance& in = inst[q];
+        const long long dims[4] = {in.nocc, in.nvir, in.naux, in.ntarget};
+        put(dims, sizeof dims);
+        const void* ptrs[3] = {in.mo_energy, in.B_ov, in.static_terms};
+        put(ptrs, sizeof ptrs);
+        put(in.targets, sizeof(int) * (size_t)in.ntarget);
+    }
+    put(outs, sizeof(void*) * (size_t)nouts);
+    return k;
+}
+
+BatchPlan* build_batch_plan(qx_handle h, const qx_g0w0_desc* d, int ninst,
+                            const qx_batch_instance* cin) {
+    const bool mixed = d->precision == QX_PREC_BF16X3;
+    const int nf = d->nfreq;
+    auto plan = std::make_unique<BatchPlan>();
+    plan->nf = nf;
+    plan->precision = d->precision;
+    plan->eta = d->eta;
+    plan->np = to_params(d->newton);
+    // instances, sorted by padded size then by N_ov (largest first)
+    std::vector<BatchInst>& v = plan->inst;
+    v.resize(ninst);
+    int off = 0;
+    for (int q = 0; q < ninst; ++q) {
+        const qx_batch_instance& c = cin[q];
+        BatchInst& in = v[q];
+        in.idx = q;
+        in.nocc = c.nocc;
+        in.nvir = c.nvir;
+        in.naux = c.naux;
+        in.nov = c.nocc * c.nvir;
+        in.nt = c.ntarget;
+        in.mo_energy = c.mo_energy;
+        in.B_ov = c.B_ov;
+        in.static_terms = c.static_terms;
+        in.targets.assign(c.targets, c.targets + c.ntarget);
+        in.pair_off.assign(in.nt + 1, 0);
+        for (int t = 0; t < in.nt; ++t)
+            in.pair_off[t + 1] = in.pair_off[t] + (in.targets[t] < in.nocc ? in.nvir : in.nocc);
+        in.npair = in.pair_off[in.nt];
+        in.npad = round_up(in.naux, NB);
+        in.nblk = in.npad / NB;
+        in.kpad = round_up(in.nov, KPAD);
+        in.rows = round_up(in.npad, 256);
+        in.out_off = off;
+        off += in.nt;
+    }
+    plan->ntot = off;
+    std::stable_sort(v.begin(), v.end(), [](const BatchInst& a, const BatchInst& b) {
+        return a.npad != b.npad ? a.npad > b.npad : a.nov > b.nov;
+    });
+    // waves under the workspace budget
+    const size_t budget = batch_budget(h);
+    size_t wave_max = 0;
+    {
+        int b = 0;
+        while (b < ninst) {
+            Bump m;
+            int e = b;
+            while (e < ninst) {
+                Bump t = m;
+                carve_instance(t, v[e], nf, mixed);
+                if (e > b && t.off > budget) break;
+                m = t;
+                ++e;
+            }
+            BatchWave w;
+            w.b = b;
+            w.e = e;
+            plan->waves.push_back(std::move(w));
+            wave_max = std::max(wave_max, m.off);
+            b = e;
+        }
+    }
+    // persistent block: grid, fail flags, zero statics, Newton state
+    {
+        Bump m;
+        m.take<double>(nf);
+        m.take<double>(nf);
+        m.take<int>((size_t)ninst * nf);
+        m.take<double>(plan->ntot);
+        m.take<int>(plan->ntot);
+        m.take<int>(plan->ntot);
+        Bump k{static_cast<char*>(static_cast<void*>(ws<char>(h, "batch_keep", m.off)))};
+        plan->freqs = k.take<double>(nf);
+        plan->wts = k.take<double>(nf);
+        int* fail = k.take<int>((size_t)ninst * nf);
+        double* zero = k.take<double>(plan->ntot);
+        plan->state = k.take<int>(plan->ntot);
+        plan->iters = k.take<int>(plan->ntot);
+        std::vector<double> fr, wt;
+        scaled_roots(nf, d->x0, fr, wt);
+        h2d(h, plan->freqs, fr.data(), nf);
+        h2d(h, plan->wts, wt.data(), nf);
+        fill_f64(h, zero, 0.0, plan->ntot);
+        for (int q = 0; q < ninst; ++q) {
+            v[q].fail = fail + (size_t)q * nf;
+            if (!v[q].static_terms) v[q].static_terms = zero + v[q].out_off;
+        }
+    }
+    // wave workspace (shared by all waves, which run one after another on the stream)
+    char* wsb = ws<char>(h, "batch_ws", wave_max);
+    for (BatchWave& w : plan->waves) {
+        Bump m{wsb};
+        for (int q = w.b; q < w.e; ++q) carve_instance(m, v[q], nf, mixed);
+    }
+    // work tables: measure, then fill a host image and copy it once
+    auto layout = [&](Bump& m) {
+        for (BatchInst& in : v) {
+            in.targets_dev = m.take<int>(in.nt);
+            in.pair_off_dev = m.take<int>(in.nt + 1);
+        }
+        for (BatchWave& w : plan->waves) {
+            const int n = w.e - w.b;
+            w.pi = m.take<PiInst>(n);
+            w.pi_off = m.take<int>(n + 1);
+            w.po = m.take<PotrfInst>(n);
+            w.cu_off = m.take<int>(n + 1);
+            w.rp_off = m.take<int>(n + 1);
+            w.wn = m.take<WnmInst>(n);
+            w.wn_off = m.take<int>(n + 1);
+            w.red_off = m.take<int>(n + 1);
+            w.nw = m.take<NewtonInst>(n);
+            w.t_off = m.take<int>(n + 1);
+        }
+    };
+    Bump meas;
+    layout(meas);
+    char* tab = ws<char>(h, "batch_tab", meas.off);
+    Bump real{tab};
+    layout(real);
+    std::vector<char> img(meas.off, 0);
+    auto host = [&](const void* dev) { return img.data() + (static_cast<const char*>(dev) - tab); };
+    auto put = [&](const void* dev, const void* src, size_t bytes) {
+        std::memcpy(host(dev), src, bytes);
+    };
+    for (BatchInst& in : v) {
+        put(in.targets_dev, in.targets.data(), sizeof(int) * in.nt);
+        put(in.pair_off_dev, in.pair_off.data(), sizeof(int) * (in.nt + 1));
+    }
+    for (BatchWave& w : plan->waves) {
+        const int n = w.e - w.b;
+        std::vector<PiInst> pi(n);
+        std::vector<PotrfInst> po(n);
+        std::vector<WnmInst> wn(n);
+        std::vector<NewtonInst> nw(n);
+        std::vector<int> pio(n + 1, 0), cu(n + 1, 0), rp(n + 1, 0), wno(n + 1, 0), red(n + 1, 0),
+            to(n + 1, 0);
+        w.nblk.resize(n);
+        for (int i = 0; i < n; ++i) {
+            const BatchInst& in = v[w.b + i];
+            const int T = ceil_div(in.npad, 128);
+            pi[i] = PiInst{in.hi, in.lo, in.f32, in.s32, in.B_ov, in.s, in.eps, in.nov,
+                           in.kpad, in.naux, in.npad};
+            pio[i + 1] = pio[i] + T * (T + 1) / 2 * nf;
+            po[i] = PotrfInst{in.eps, in.x, in.linv, in.fail, in.npad};
+            cu[i + 1] = cu[i] + in.nblk * nf;
+            rp[i + 1] = rp[i] + (in.nblk - 1) * nf;
+            w.nblk[i] = in.nblk;
+            wn[i] = WnmInst{in.bp, in.x, in.wpart, in.wnm, in.npair, in.npad};
+            wno[i + 1] = wno[i] + ceil_div(std::max(in.npair, 1), NB) * in.nblk * nf;
+            red[i + 1] = red[i] + nf * in.npair;
+            SigmaProblem& p = nw[i].p;
+            p.nt = in.nt;
+            p.npair = in.npair;
+            p.nf = nf;
+            p.nocc = in.nocc;
+            p.nmo = in.nocc + in.nvir;
+            p.targets = in.targets_dev;
+            p.pair_off = in.pair_off_dev;
+            p.pair_m = in.pm;
+            p.wnm = in.wnm;
+            p.eps_g = in.mo_energy;
+            p.eps_ref = in.mo_energy;
+            p.freqs = plan->freqs;
+            p.wts = plan->wts;
+            p.static_terms = in.static_terms;
+            p.eta = d->eta;
+            p.ef = 0.0;                      // recomputed on the device
+            nw[i].out_off = in.out_off;
+            to[i + 1] = to[i] + in.nt;
+        }
+        put(w.pi, pi.data(), sizeof(PiInst) * n);
+        put(w.pi_off, pio.data(), sizeof(int) * (n + 1));
+        put(w.po, po.data(), sizeof(PotrfInst) * n);
+        put(w.cu_off, cu.data(), sizeof(int) * (n + 1));
+        put(w.rp_off, rp.data(), sizeof(int) * (n + 1));
+        put(w.wn, wn.data(), sizeof(WnmInst) * n);
+        put(w.wn_off, wno.data(), sizeof(int) * (n + 1));
+        put(w.red_off, red.data(), sizeof(int) * (n + 1));
+        put(w.nw, nw.data(), sizeof(NewtonInst) * n);
+        put(w.t_off, to.data(), sizeof(int) * (n + 1));
+        w.pi_total = pio[n];
+        w.cu_h = cu;
+        w.rp_h = rp;
+        w.wn_total = wno[n];
+        w.red_total = red[n];
+        w.t_total = to[n];
+    }
+    h2d(h, tab, img.data(), img.size());
+    int dev = 0, ncu = 0;
+    HIPCK(h, hipGetDevice(&dev));
+    HIPCK(h, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    plan->pi_grid = ncu;                     // times the occupancy of the chosen variant
+    return plan.release();
+}
+
+// Enqueues the whole batch on the handle's stream.  direct: the first run, which tunes the
+// Pi variant of each wave and records phase times; otherwise the launches only (capture).
+void enqueue_batch(qx_handle h, BatchPlan& P, bool direct) {
+    hipStream_t st = h->stream;
+    const bool mixed = P.precision == QX_PREC_BF16X3;
+    const int nf = P.nf;
+    auto phase = [&](int id, auto&& fn) {
+        if (direct) {
+            Phase ph(h, id);
+            fn();
+        } else {
+            fn();
+        }
+    };
+    for (BatchWave& w : P.waves) {
+        const int n = w.e - w.b;
+        phase(QX_PHASE_PREP, [&] {
+            for (int q = w.b; q < w.e; ++q) {
+                BatchInst& in = P.inst[q];
+                HIPCK(h, launch_pi_weights(in.mo_energy, in.nocc, in.nvir, nf, P.freqs, in.s,
+                                           mixed ? in.s32 : nullptr, st));
+                if (mixed)
+                    HIPCK(h, launch_pi_prepare_mixed(in.B_ov, in.nov, in.naux, in.rows, in.kpad,
+                                                     in.hi, in.lo, in.f32, st));
+                hipLaunchKernelGGL(fill_i32_kernel, dim3(1), dim3(256), 0, st, in.fail, 0,
+                                   (size_t)nf);
+                hipLaunchKernelGGL(ov_pairs_kernel,
+                                   dim3(std::min(std::max(in.nocc, in.nvir), 1024), in.nt),
+                                   dim3(256), 0, st, in.B_ov, in.nocc, in.nvir, in.naux,
+                                   in.targets_dev, in.pair_off_dev, in.nt, in.pt, in.pm,
+                                   in.rows_naux);
+                hipLaunchKernelGGL(pad_rows_kernel, grid1d((size_t)in.npair * in.npad), dim3(256),
+                                   0, st, in.rows_naux, in.npair, in.naux, in.bp, in.npad);
+                HIPCK(h, hipGetLastError());
+            }
+        });
+        const int prec = mixed ? 2 : 0;
+        auto pi_launch = [&](int var) {
+            HIPCK(h, launch_pi_group(prec, w.pi, w.pi_off, n, nf, w.pi_total,
+                                     P.pi_grid * pi_group_occupancy(prec, var), st, var));
+        };
+        if (direct) {
+            w.pi_variant = choose_variant(h, "pi_group", prec, pi_group_variant_count(prec),
+                                          {n, w.pi_total, P.inst[w.b].kpad}, pi_launch);
+            w.pi_grid = P.pi_grid * pi_group_occupancy(prec, w.pi_variant);
+        }
+        phase(QX_PHASE_PI, [&] {
+            HIPCK(h, launch_pi_group(prec, w.pi, w.pi_off, n, nf, w.pi_total, w.pi_grid, st,
+                                     w.pi_variant));
+        });
+        phase(QX_PHASE_POTRF, [&] {
+            HIPCK(h, launch_potrf_group(w.po, w.nblk.data(), n, nf, w.cu_off, w.rp_off,
+                                        w.cu_h.data(), w.rp_h.data(), st));
+        });
+        phase(QX_PHASE_WNM, [&] {
+            HIPCK(h, launch_wnm_group(w.wn, w.wn_off, w.red_off, n, nf, w.wn_total, w.red_total,
+                                      st));
+        });
+        phase(QX_PHASE_NEWTON, [&] {
+            HIPCK(h, launch_newton_group(w.nw, w.t_off, n, w.t_total, P.np, P.e_out, P.z_out,
+                                         P.sre_out, P.sim_out, P.state, P.iters, st));
+        });
+    }
+}
+
+}  // namespace
+
+qx_status qx_g0w0_batch_ov_dev(qx_handle h, const qx_g0w0_desc* d, int ninst,
+                               const qx_batch_instance* inst, double* qp_energy,
+                               double* z_factor, double* sigma_re, double* sigma_im, int* state,
+                               qx_stats* stats) {
+    return guarded(h, [&] {
+        const auto t0 = std::chrono::steady_clock::now();
+        REQUIRE(h, d != nullptr && inst != nullptr, QX_ERR_INVALID_INPUT, "NULL argument");
+        REQUIRE(h, ninst > 0, QX_ERR_INVALID_INPUT, "ninst must be positive (got %d)", ninst);
+        REQUIRE(h, qp_energy && z_factor && sigma_re && sigma_im, QX_ERR_INVALID_INPUT,
+                "output pointer is NULL");
+        REQUIRE(h, d->precision == QX_PREC_FP64 || d->precision == QX_PREC_BF16X3,
+                QX_ERR_INVALID_INPUT, "batched solves support FP64 and BF16X3 (got %d)",
+                d->precision);
+        long long ntot = 0;
+        for (int q = 0; q < ninst; ++q) {
+            const qx_batch_instance& c = inst[q];
+            qx_g0w0_desc dq = *d;
+            dq.nocc = c.nocc;
+            dq.nvir = c.nvir;
+            dq.naux = c.naux;
+            dq.ntarget = c.ntarget;
+            dq.npair = 0;
+            validate_desc(h, &dq);
+            REQUIRE(h, c.mo_energy && c.B_ov && c.targets, QX_ERR_INVALID_INPUT,
+                    "instance %d: NULL pointer", q);
+            for (int t = 0; t < c.ntarget; ++t)
+                REQUIRE(h, c.targets[t] >= 0 && c.targets[t] < c.nocc + c.nvir,
+                        QX_ERR_INVALID_INPUT, "instance %d: target %d out of range", q,
+                        c.targets[t]);
+            ntot += c.ntarget;
+        }
+        REQUIRE(h, ntot < (1ll << 31), QX_ERR_INVALID_INPUT, "too many targets");
+        const void* outs[5] = {qp_energy, z_factor, sigma_re, sigma_im, state};
+        std::vector<unsigned long long> key = batch_key(d, ninst, inst, outs, 5);
+        if (!h->batch || h->batch->key != key) {
+            HIPCK(h, hipStreamSynchronize(h->stream));
+            delete_batch_plan(h->batch);
+            h->batch = nullptr;
+            h->batch = build_batch_plan(h, d, ninst, inst);
+            h->batch->key = std::move(key);
+            h->batch->e_out = qp_energy;
+            h->batch->z_out = z_factor;
+            h->batch->sre_out = sigma_re;
+            h->batch->sim_out = sigma_im;
+        }
+        BatchPlan& P = *h->batch;
+        const char* gv = std::getenv("QXGW_BATCH_GRAPH");
+        const bool use_graph = !(gv && std::atoi(gv) == 0) && h->stream != nullptr;
+        int replayed = 0;
+        if (!P.ran || !use_graph) {
+            enqueue_batch(h, P, true);
+            P.ran = true;
+        } else {
+            if (!P.exec) {
+                hipGraph_t g = nullptr;
+                HIPCK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+                try {
+                    enqueue_batch(h, P, false);
+                } catch (...) {
+                    (void)hipStreamEndCapture(h->stream, &g);
+                    if (g) (void)hipGraphDestroy(g);
+                    throw;
+                }
+                HIPCK(h, hipStreamEndCapture(h->stream, &g));
+                const hipError_t e = hipGraphInstantiate(&P.exec, g, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(g);
+                HIPCK(h, e);
+            }
+            HIPCK(h, hipGraphLaunch(P.exec, h->stream));
+            replayed = 1;
+        }
+        // results that need the single-instance path: a failed Cholesky somewhere, or a
+        // target whose Newton iterates reached a residue pole or failed
+        std::vector<int> fail((size_t)ninst * P.nf), st(P.ntot);
+        d2h(h, fail.data(), P.inst[0].fail, fail.size());
+        d2h(h, st.data(), P.state, st.size());
+        std::vector<int> redo;
+        for (int q = 0; q < ninst; ++q) {
+            const BatchInst& in = P.inst[q];
+            bool bad = false;
+            for (int f = 0; f < P.nf; ++f) bad |= fail[(size_t)q * P.nf + f] != 0;
+            for (int t = 0; t < in.nt; ++t)
+                bad |= st[in.out_off + t] != kNewtonConverged && st[in.out_off + t] != kNewtonBisection;
+            if (bad) redo.push_back(q);
+        }
+        if (state) {
+            for (int& v : st) v = v == kNewtonBisection ? QX_QP_BISECTION : QX_QP_NEWTON;
+            h2d(h, state, st.data(), st.size());
+        }
+        qx_stats acc{};
+        for (int q : redo) {
+            const BatchInst& in = P.inst[q];
+            const qx_batch_instance& c = inst[in.idx];
+            qx_g0w0_desc dq = *d;
+            dq.nocc = in.nocc;
+            dq.nvir = in.nvir;
+            dq.naux = in.naux;
+            dq.ntarget = in.nt;
+            qx_stats sq{};
+            const qx_status r = run_g0w0_ov(
+                h, &dq, c.mo_energy, nullptr, c.B_ov, in.targets_dev, c.static_terms, nullptr,
+                qp_energy + in.out_off, z_factor + in.out_off, sigma_re + in.out_off,
+                sigma_im + in.out_off, state ? state + in.out_off : nullptr, &sq);
+            if (r != QX_OK) return r;
+            acc.chol_fallbacks += sq.chol_fallbacks;
+            acc.singular_fallbacks += sq.singular_fallbacks;
+            acc.residue_targets += sq.residue_targets;
+            acc.residue_jobs += sq.residue_jobs;
+            acc.newton_failures += sq.newton_failures;
+        }
+        HIPCK(h, hipStreamSynchronize(h->stream));
+        resolve_profile(h);
+        if (stats) {
+            *stats = acc;
+            stats->pi_variant = P.waves[0].pi_variant;
+            stats->waves = (int)P.waves.size();
+            stats->graph_replay = replayed;
+            stats->resolved_instances = (int)redo.size();
+            stats->seconds =
+                std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        }
+        return QX_OK;
+    });
+}

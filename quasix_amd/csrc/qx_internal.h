@@ -30,6 +30,19 @@ inline int ceil_div(int x, int m) { return (x + m - 1) / m; }
 __device__ __forceinline__ int mfma_row_f32(int lane, int v) { return 4 * (lane >> 4) + v; }
 __device__ __forceinline__ int mfma_row_f64(int lane, int v) { return 4 * v + (lane >> 4); }
 
+// Work tables of grouped (batched) launches: off[0..n] are the first work items of the n
+// instances; returns the instance that owns item w (off[0] <= w < off[n]).  The search
+// runs on wave-uniform values.
+__device__ __forceinline__ int find_inst(const int* __restrict__ off, int n, int w) {
+    int lo = 0, hi = n;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (off[mid] <= w) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
 // ---- Polarizability (pi_kernels.hip) ---------------------------------------------------
 // Pi is built from the frequency weights s_f(r) = 4 e_r / (w_f^2 + e_r^2),
 // e_r = eps_i - eps_a for row r = i*nvir + a.
@@ -59,10 +72,59 @@ hipError_t launch_pi_syrk(int precision, const double* B, const PiOperands& ops,
                           double* out, PiOut mode, hipStream_t st, int variant = 0);
 int pi_variant_count(int precision);
 
+// One instance of a grouped Pi launch.  Mixed (BF16x3) kernels read bt_hi/bt_lo/bt_f32/s32,
+// FP64 kernels read B/s; both write eps = I - Pi (lower tiles) into out [nf][npad][npad].
+struct PiInst {
+    const uint16_t* bt_hi;
+    const uint16_t* bt_lo;
+    const float* bt_f32;
+    const float* s32;
+    const double* B;
+    const double* s;
+    double* out;
+    int nov, kpad, naux, npad;
+};
+// Persistent grouped Pi over all instances; off_dev: [ninst + 1] prefix sums of
+// ntiles_q * nf with ntiles_q = T (T + 1) / 2, T = ceil(npad_q / 128).  grid is rounded
+// down to a multiple of 8.  precision 0 (FP64) or 2 (BF16x3, variants 0 = K32, 1 = K64).
+hipError_t launch_pi_group(int precision, const PiInst* inst_dev, const int* off_dev, int ninst,
+                           int nf, int total, int grid, hipStream_t st, int variant = 0);
+int pi_group_variant_count(int precision);
+int pi_group_occupancy(int precision, int variant);   // resident workgroups per CU
+
+// Side streams for running independent groups of a batch concurrently: work forks from
+// the caller's stream through `fork` and joins back through `join[g]`.  All streams have
+// the default priority.
+struct StreamFan {
+    static constexpr int kMax = 3;
+    int n = 0;                              // side streams available
+    hipStream_t side[kMax] = {};
+    hipEvent_t fork = nullptr, join[kMax] = {};
+};
+
 // ---- Factorisation (chol_kernels.hip) --------------------------------------------------
 // Cholesky eps = L L^T (L into a, lower blocks) and X = L^-1 (lower blocks of x) together.
+// With a fan, the frequencies are split into `groups` contiguous groups whose step loops run
+// on separate streams, so one group's latency-bound diagonal and panel kernels overlap the
+// other groups' column updates.
 hipError_t launch_potrf_batched(double* a, double* x, double* linv_diag, int npad, int nf,
-                                int* fail, hipStream_t st);
+                                int* fail, hipStream_t st, const StreamFan* fan = nullptr,
+                                int groups = 1);
+// Grouped factorisation of a batch of instances with different sizes and a common nf.
+// inst_dev: [ninst] sorted by npad descending (nblk_sorted[q] = npad_q / NB on the host);
+// cu_off / rp_off: [ninst + 1] prefix sums of nblk_q * nf and (nblk_q - 1) * nf, on the
+// device and on the host.  Each step is one launch per kernel kind over all instances
+// that still have that block column.
+struct PotrfInst {
+    double* a;          // [nf][npad][npad]  eps -> L
+    double* x;          // [nf][npad][npad]  X = L^-1
+    double* linv;       // [nf][nblk][NB][NB]
+    int* fail;          // [nf]
+    int npad;
+};
+hipError_t launch_potrf_group(const PotrfInst* inst_dev, const int* nblk_sorted, int ninst,
+                              int nf, const int* cu_off_dev, const int* rp_off_dev,
+                              const int* cu_off_host, const int* rp_off_host, hipStream_t st);
 // eps^-1 - I from X = L^-1 :  W = X^T X - I  (full symmetric)
 hipError_t launch_wc_from_linv(const double* x, double* wc, int npad, int naux, int nf,
                                hipStream_t st);
@@ -87,6 +149,18 @@ hipError_t launch_wnm(int precision, const double* bpairs, const double* x, int 
 // wnm from an explicit Wc: wnm[f][j] = b_j^T Wc_f b_j  (FP64)
 hipError_t launch_wnm_from_wc(const double* bpairs, const double* wc, int npair, int npad,
                               int nf, double* wpart, double* wnm, hipStream_t st);
+
+// Grouped Wc_nm (FP64, batched instances).  off: prefix of ceil(npair_q/64) * nblk_q * nf
+// (tiles), red_off: prefix of nf * npair_q (reduction items).
+struct WnmInst {
+    const double* bp;   // [npair][npad] pair rows, zero padded
+    const double* x;    // [nf][npad][npad] X = L^-1
+    double* wpart;      // [nf][nblk][npair]
+    double* wnm;        // [nf][npair]
+    int npair, npad;
+};
+hipError_t launch_wnm_group(const WnmInst* inst_dev, const int* off_dev, const int* red_off_dev,
+                            int ninst, int nf, int total_tiles, int total_red, hipStream_t st);
 
 struct NewtonParams {
     int max_iterations;
@@ -122,6 +196,16 @@ hipError_t launch_newton_cd(const SigmaProblem& p, const NewtonParams& np,
                             const double* guess,        // nullable: linearised guess
                             double* e_out, double* z_out, double* sig_re, double* sig_im,
                             int* state, int* iters, hipStream_t st);
+// Grouped Newton over the targets of a batch (no guesses); target t of instance q writes
+// output index out_off_q + t.
+struct NewtonInst {
+    SigmaProblem p;
+    int out_off;
+};
+hipError_t launch_newton_group(const NewtonInst* inst_dev, const int* t_off_dev, int ninst,
+                               int total_targets, const NewtonParams& np, double* e_out,
+                               double* z_out, double* sig_re, double* sig_im, int* state,
+                               int* iters, hipStream_t st);
 // I_n(E) at nE energies per target: E[t][q] -> out[t][q] (complex, interleaved re/im)
 hipError_t launch_sigma_imag_eval(const SigmaProblem& p, const double* energies, int nq,
                                   double* out, hipStream_t st);

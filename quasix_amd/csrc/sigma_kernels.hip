@@ -19,6 +19,8 @@
 #include "qx_internal.h"
 #include "qx_newton.h"
 
+#include <algorithm>
+
 namespace qx {
 namespace {
 
@@ -31,14 +33,12 @@ __device__ __forceinline__ d4 mfma64(double a, double b, d4 c) {
 }
 
 // ---- Wc_nm contraction -----------------------------------------------------------------
-// grid (ceil(npair/64), nblk, nf): tile of 64 pairs x 64 auxiliary columns (block pb).
-__global__ __launch_bounds__(256) void wnm_linv_kernel(const double* __restrict__ bp,
-                                                       const double* __restrict__ x, int npair,
-                                                       int npad, double* __restrict__ wpart) {
-    __shared__ double Bs[NB][P];
-    __shared__ double Ms[NB][P];
-    const int j0 = blockIdx.x * NB, pb = blockIdx.y, f = blockIdx.z;
-    const int nblk = gridDim.y;
+// Tile of 64 pairs (from j0) x 64 auxiliary columns (block pb) of frequency f; Bs, Ms:
+// NB x P doubles of LDS.
+__device__ __forceinline__ void wnm_linv_body(const double* __restrict__ bp,
+                                              const double* __restrict__ x, int npair, int npad,
+                                              int nblk, double* __restrict__ wpart, int j0,
+                                              int pb, int f, double (*Bs)[P], double (*Ms)[P]) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const double* X = x + (size_t)f * npad * npad;
     d4 acc[4] = {d4{0, 0, 0, 0}, d4{0, 0, 0, 0}, d4{0, 0, 0, 0}, d4{0, 0, 0, 0}};
@@ -87,6 +87,45 @@ __global__ __launch_bounds__(256) void wnm_linv_kernel(const double* __restrict_
             const int j = j0 + wave * 16 + mfma_row_f64(lane, v);
             if (j < npair) wpart[((size_t)f * nblk + pb) * npair + j] = rs[v];
         }
+}
+
+// grid (ceil(npair/64), nblk, nf)
+__global__ __launch_bounds__(256) void wnm_linv_kernel(const double* __restrict__ bp,
+                                                       const double* __restrict__ x, int npair,
+                                                       int npad, double* __restrict__ wpart) {
+    __shared__ double Bs[NB][P];
+    __shared__ double Ms[NB][P];
+    wnm_linv_body(bp, x, npair, npad, gridDim.y, wpart, blockIdx.x * NB, blockIdx.y, blockIdx.z,
+                  Bs, Ms);
+}
+
+// Grouped form: instance q owns ceil(npair_q/64) * nblk_q * nf consecutive workgroups.
+__global__ __launch_bounds__(256) void wnm_linv_group_kernel(const WnmInst* __restrict__ inst,
+                                                             const int* __restrict__ off,
+                                                             int ninst, int nf) {
+    __shared__ double Bs[NB][P];
+    __shared__ double Ms[NB][P];
+    const int w = blockIdx.x;
+    const int q = find_inst(off, ninst, w);
+    const WnmInst in = inst[q];
+    const int nblk = in.npad / NB, njb = (in.npair + NB - 1) / NB;
+    const int local = w - off[q];
+    const int jb = local % njb, pb = (local / njb) % nblk, f = local / (njb * nblk);
+    wnm_linv_body(in.bp, in.x, in.npair, in.npad, nblk, in.wpart, jb * NB, pb, f, Bs, Ms);
+}
+
+// Grouped reduction: one thread per (instance, frequency, pair); off: prefix of nf * npair.
+__global__ void wnm_reduce_group_kernel(const WnmInst* __restrict__ inst,
+                                        const int* __restrict__ off, int ninst, int total) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x) {
+        const int q = find_inst(off, ninst, i);
+        const WnmInst in = inst[q];
+        const int local = i - off[q];
+        const int f = local / in.npair, j = local % in.npair, nblk = in.npad / NB;
+        double s = 0.0;
+        for (int b = 0; b < nblk; ++b) s += in.wpart[((size_t)f * nblk + b) * in.npair + j];
+        in.wnm[(size_t)f * in.npair + j] = s;
+    }
 }
 
 __global__ void wnm_reduce_kernel(const double* __restrict__ wpart, int npair, int nparts,
@@ -415,6 +454,37 @@ __global__ __launch_bounds__(64) void newton_cd_kernel(SigmaProblem p, NewtonPar
     }
 }
 
+// Grouped Newton: one wavefront per target of every instance; t_off: prefix of ntarget.
+// The Fermi level is recomputed from the instance's energies on the device, so a replayed
+// launch sequence follows the current contents of the energy arrays.
+__global__ __launch_bounds__(64) void newton_group_kernel(const NewtonInst* __restrict__ inst,
+                                                          const int* __restrict__ t_off,
+                                                          int ninst, NewtonParams np,
+                                                          double* e_out, double* z_out,
+                                                          double* sig_re, double* sig_im,
+                                                          int* state, int* iters) {
+    const int g = blockIdx.x;
+    const int q = find_inst(t_off, ninst, g);
+    SigmaProblem p = inst[q].p;
+    p.ef = 0.5 * (p.eps_g[p.nocc - 1] + p.eps_g[p.nocc]);
+    const int t = g - t_off[q], o = inst[q].out_off + t;
+    const int n = p.targets[t];
+    DeviceEval ev{&p, t};
+    double E = 0.0, Z = 0.0;
+    C2 s = {0.0, 0.0};
+    int it = 0;
+    const int stt = newton_core(ev, np, p.eps_ref[n], p.static_terms[t], p.ef, false, 0.0, E, Z,
+                                s, it);
+    if ((threadIdx.x & 63) == 0) {
+        e_out[o] = E;
+        z_out[o] = Z;
+        sig_re[o] = s.re;
+        sig_im[o] = s.im;
+        state[o] = stt;
+        iters[o] = it;
+    }
+}
+
 // ---- Residue jobs ---------------------------------------------------------------------------
 // eps(z) = I - Pi(z),  Pi(z) = 2 sum_r B_r B_r^T [1/(z + e_r) - 1/(z - e_r)],  z = w + i eta.
 // work per job: [naux][naux+1] complex (augmented with b).
@@ -593,6 +663,28 @@ hipError_t launch_wnm(int precision, const double* bpairs, const double* x, int 
     }
     dim3 g2(ceil_div(npair, 256), nf);
     hipLaunchKernelGGL(wnm_reduce_kernel, g2, dim3(256), 0, st, wpart, npair, nblk, wnm);
+    return hipGetLastError();
+}
+
+hipError_t launch_wnm_group(const WnmInst* inst_dev, const int* off_dev, const int* red_off_dev,
+                            int ninst, int nf, int total_tiles, int total_red, hipStream_t st) {
+    if (ninst <= 0) return hipSuccess;
+    if (total_tiles > 0)
+        hipLaunchKernelGGL(wnm_linv_group_kernel, dim3(total_tiles), dim3(256), 0, st, inst_dev,
+                           off_dev, ninst, nf);
+    if (total_red > 0)
+        hipLaunchKernelGGL(wnm_reduce_group_kernel, dim3(std::min(ceil_div(total_red, 256), 4096)),
+                           dim3(256), 0, st, inst_dev, red_off_dev, ninst, total_red);
+    return hipGetLastError();
+}
+
+hipError_t launch_newton_group(const NewtonInst* inst_dev, const int* t_off_dev, int ninst,
+                               int total_targets, const NewtonParams& np, double* e_out,
+                               double* z_out, double* sig_re, double* sig_im, int* state,
+                               int* iters, hipStream_t st) {
+    if (ninst <= 0 || total_targets <= 0) return hipSuccess;
+    hipLaunchKernelGGL(newton_group_kernel, dim3(total_targets), dim3(64), 0, st, inst_dev,
+                       t_off_dev, ninst, np, e_out, z_out, sig_re, sig_im, state, iters);
     return hipGetLastError();
 }
 

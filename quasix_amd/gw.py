@@ -334,3 +334,78 @@ class G0W0Solver:
         o = self.out.cpu().numpy()
         return dict(qp_energies=o[0], z_factors=o[1], sigma_c=o[2] + 1j * o[3],
                     state=self.state.cpu().numpy(), targets=self.targets)
+
+
+class G0W0BatchSolver:
+    """Repeated batched G0W0 solves (``qx_g0w0_batch_ov_dev``) over device-resident
+    instances of different sizes.  ``instances``: sequence of (mo_energy, B_ov, nocc,
+    targets) with ov-only B_ov [nocc*nvir, naux]; targets None = HOMO-5..LUMO+5.  The first
+    ``run()`` executes and tunes the grouped launches; later runs replay them as a HIP graph.
+    Results: ``results()`` gives one dict per instance."""
+
+    def __init__(self, instances, *, nfreq=32, x0=0.5, eta=0.01, precision="bf16x3", device=0,
+                 newton=None):
+        self.dev = _dev(device)
+        self.h = _lib.handle(device)
+        n = len(instances)
+        if n == 0:
+            raise ValueError("empty batch")
+        self.arr = (_lib.BatchInstance * n)()
+        self._keep = []
+        self.targets = []
+        off = 0
+        self.offsets = []
+        for q, (e, B, nocc, tg) in enumerate(instances):
+            e = _f64(e, self.dev)
+            B = _f64(B, self.dev)
+            nmo = e.numel()
+            tg = frontier_targets(nocc, nmo) if tg is None else np.ascontiguousarray(tg, np.int32)
+            if B.dim() != 2 or B.shape[0] != nocc * (nmo - nocc):
+                raise ValueError(f"instance {q}: B_ov must be [nocc*nvir, naux]")
+            a = self.arr[q]
+            a.nocc, a.nvir, a.naux, a.ntarget = int(nocc), int(nmo - nocc), int(B.shape[1]), len(tg)
+            a.mo_energy, a.B_ov = e.data_ptr(), B.data_ptr()
+            a.targets = tg.ctypes.data
+            a.static_terms = None
+            self._keep += [e, B, tg]
+            self.targets.append(tg)
+            self.offsets.append(off)
+            off += len(tg)
+        self.ntot = off
+        self.desc = make_desc(1, 1, 1, nfreq, x0, eta, precision, 1, 0, **(newton or {}))
+        self.out = torch.empty((4, off), dtype=torch.float64, device=self.dev)
+        self.state = torch.empty(off, dtype=torch.int32, device=self.dev)
+        self.stats = _lib.Stats()
+        torch.cuda.synchronize(self.dev)
+
+    def run(self):
+        o = self.out
+        self.h.check(self.h.lib.qx_g0w0_batch_ov_dev(
+            self.h.h, C.byref(self.desc), len(self.arr), self.arr, _ptr(o[0]), _ptr(o[1]),
+            _ptr(o[2]), _ptr(o[3]), _ptr(self.state), C.byref(self.stats)))
+
+    def set_profiling(self, on: bool):
+        self.h.check(self.h.lib.qx_set_profiling(self.h.h, 1 if on else 0))
+
+    def profile(self):
+        ms = (C.c_double * 8)()
+        n = (C.c_longlong * 8)()
+        self.h.check(self.h.lib.qx_get_profile(self.h.h, ms, n))
+        return {name: (ms[i], n[i]) for i, name in enumerate(_lib.PHASES)}
+
+    def results(self):
+        o = self.out.cpu().numpy()
+        st = self.state.cpu().numpy()
+        res = []
+        for off, tg in zip(self.offsets, self.targets):
+            sl = slice(off, off + len(tg))
+            res.append(dict(qp_energies=o[0, sl], z_factors=o[1, sl],
+                            sigma_c=o[2, sl] + 1j * o[3, sl], state=st[sl], targets=tg))
+        return res
+
+
+def g0w0_batch(instances, **kw):
+    """One batched solve; returns a list of per-instance result dicts."""
+    s = G0W0BatchSolver(instances, **kw)
+    s.run()
+    return s.results()

@@ -52,9 +52,12 @@ def test_struct_layouts_match_header(lib):
 #include <stddef.h>
 #include "qxgw.h"
 int main(void) {
-  printf("%zu %zu %zu %zu %zu %zu %zu\n", sizeof(qx_newton_config), sizeof(qx_g0w0_desc),
-         sizeof(qx_stats), offsetof(qx_g0w0_desc, newton), offsetof(qx_g0w0_desc, npair),
-         offsetof(qx_stats, seconds), offsetof(qx_stats, pi_variant));
+  printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(qx_newton_config),
+         sizeof(qx_g0w0_desc), sizeof(qx_stats), offsetof(qx_g0w0_desc, newton),
+         offsetof(qx_g0w0_desc, npair), offsetof(qx_stats, seconds),
+         offsetof(qx_stats, pi_variant), offsetof(qx_stats, resolved_instances),
+         sizeof(qx_batch_instance), offsetof(qx_batch_instance, targets),
+         offsetof(qx_batch_instance, static_terms));
   return 0;
 }
 """
@@ -68,7 +71,9 @@ int main(void) {
                                         check=True).stdout.split()))
     assert vals == [C.sizeof(L.NewtonConfig), C.sizeof(L.G0W0Desc), C.sizeof(L.Stats),
                     L.G0W0Desc.newton.offset, L.G0W0Desc.npair.offset, L.Stats.seconds.offset,
-                    L.Stats.pi_variant.offset]
+                    L.Stats.pi_variant.offset, L.Stats.resolved_instances.offset,
+                    C.sizeof(L.BatchInstance), L.BatchInstance.targets.offset,
+                    L.BatchInstance.static_terms.offset]
 
 
 @pytest.mark.parametrize("n", [1, 2, 7, 32, 48, 64, 100])
