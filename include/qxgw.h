@@ -192,6 +192,44 @@ QXGW_API qx_status qx_g0w0_cd_ov_dev(qx_handle h, const qx_g0w0_desc* d,
                                      const double* static_terms, const double* guess,
                                      double* qp_energy, double* z_factor, double* sigma_re,
                                      double* sigma_im, int* state, qx_stats* stats);
+/* ---- Analytic continuation (the reference's live self-energy method) ------------------- */
+/* Sigma_c(i nu) of the targets on the AC grid: Wc_nm at desc->nfreq scaled Gauss-Legendre
+ * frequencies (desc->x0; the reference default is 100), grid nu = {0} + {w_k < iw_cutoff},
+ * omega_q = -i nu_q for occupied and +i nu_q for virtual targets.  Pairs as in
+ * qx_g0w0_cd_dev (desc->npair rows).  sigma, omega: device [ntarget][nws] complex (re, im
+ * interleaved) with nws <= nfreq + 1 returned in *nws.  Energies in mo_energy serve the
+ * polarizability, the Green's function and the Fermi level, as in the reference.
+ * Replaces correlation_ac::get_sigma_diag (quasix_core/src/selfenergy/correlation_ac.rs:
+ * 241-449) and the quasix_core.compute_sigma_c_ac stage function (python/g0w0.rs:721-751). */
+QXGW_API qx_status qx_sigma_ac_dev(qx_handle h, const qx_g0w0_desc* d, const double* mo_energy,
+                                   const double* B_ov, const double* B_pair,
+                                   const int* pair_target, const int* pair_m,
+                                   const int* targets, double iw_cutoff, double* sigma,
+                                   double* omega, int* nws);
+/* Host-only Pade continuation (complex values as (re, im) pairs).
+ * qx_thiele: reciprocal-difference coefficients of n points (correlation_ac.rs:470-509).
+ * qx_pade_thiele: the continued fraction at a real omega (correlation_ac.rs:530-566).
+ * qx_ac_pade_thiele_diag: per-orbital fit on points 1, 7, ..., 37 then every 4th (even
+ * count) of sigma/omega [norbs][nw]; coeff [nfit][norbs], ofit [norbs][nfit] (allocate nw
+ * entries per orbital); *nfit returned (correlation_ac.rs:587-654). */
+QXGW_API qx_status qx_thiele(int n, const double* fn, const double* zn, double* coeff);
+QXGW_API qx_status qx_pade_thiele(int n, double omega, const double* zn, const double* coeff,
+                                  double* out);
+QXGW_API qx_status qx_ac_pade_thiele_diag(int norbs, int nw, const double* sigma,
+                                          const double* omega, double* coeff, double* ofit,
+                                          int* nfit);
+/* Host-only QP step of the evGW driver on the Pade model (quasix_core/src/qp/evgw.rs:
+ * 705-820): solver 0 = linearised at prev_qp[p] (or mo_energy[p] when prev_qp is NULL),
+ * solver 1 = Newton (newton_solver.rs) from prev_qp[p] or, when NULL, from the linearised
+ * value at mo_energy[p], with the linearised fallback on failure.  Z clamped to
+ * [cfg->z_min, cfg->z_max]; ef from mo_energy.  Outputs per listed orbital (host). */
+QXGW_API qx_status qx_qp_solve_ac(int norbs, int nfit, const double* coeff, const double* ofit,
+                                  const int* orbs, int nmo, int nocc, const double* mo_energy,
+                                  const double* sigma_x, const double* vxc, int solver,
+                                  const double* prev_qp, const qx_newton_config* cfg,
+                                  double* qp_energy, double* z_factor, double* sigma_re,
+                                  double* sigma_im, int* state);
+
 /* ---- Batched solves (no reference counterpart: the reference solves one system per call;
  * this is for callers with many small systems, e.g. a molecule set or a screening loop) -- */
 /* One instance of a batch.  Device pointers except `targets`. */

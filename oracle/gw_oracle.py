@@ -35,8 +35,17 @@ Parity status per function (checks live in tests/test_oracle_*.py):
   wmn_imag, sigma_cd, residues                              pinned by Casida brute force
   get_sigma_diag, thiele, pade_thiele, ac_pade_thiele_diag  pinned (rational-function
       exactness, reference tests correlation_ac.rs:755-810)
+  qp_step_ac                                                pinned through its parts
+      (pade_thiele, NewtonQPSolver) and the closed-form linearised step
+  evgw_ac                                                   parity unpinned beyond its
+      parts (as evgw_cd)
   NewtonQPSolver                                            pinned (reference tests
       newton_solver.rs:609-732 restated as known answers)
+  diis_solve, DIISMonitor, evgw_step_mix                    pinned (numpy.linalg.solve,
+      constrained least-squares closed form of the DIIS combination, the singular
+      zero-error case that the driver's call order produces, tests/test_convergence.py)
+  g0w0_adjoint                                              pinned (central finite
+      differences of g0w0_cd)
   evgw_cd                                                   parity unpinned beyond its
       parts (no reference evGW numbers exist without PySCF inputs)
 """
@@ -524,36 +533,122 @@ def g0w0_cd(mo_energy, B_ov, nocc, targets, freqs, wts, *, pairs=None, static=No
     return dict(E=E, Z=Z, sigma=S, converged=conv, wnm=wnm, pairs=pairs, ef=ef)
 
 
+def diis_solve(a, b):
+    """Gaussian elimination with partial pivoting and the reference's checks.
+
+    Restates ConvergenceMonitor::solve_linear_system (quasix_core/src/qp/convergence.rs:
+    440-540): pivot tolerance 1e-12 * max(max|a|, 1), rows eliminated only where
+    |a_ki| > 1e-10 |pivot|, coefficient sum (all but the multiplier) within 0.5 of 1.
+    Raises ArithmeticError where the reference returns NumericalError.
+    """
+    a = np.array(a, dtype=np.float64)
+    n = len(a)
+    m = np.hstack([a, np.asarray(b, dtype=np.float64).reshape(n, 1)])
+    tol = 1e-12 * max(np.abs(a).max(), 1.0)
+    for i in range(n - 1):
+        p = i + int(np.argmax(np.abs(m[i:, i])))
+        if abs(m[p, i]) < tol:
+            raise ArithmeticError("singular DIIS matrix")
+        m[[i, p]] = m[[p, i]]
+        for k in range(i + 1, n):
+            if abs(m[k, i]) > 1e-10 * abs(m[i, i]):
+                m[k] = m[k] - m[k, i] / m[i, i] * m[i]
+    if abs(m[n - 1, n - 1]) < tol:
+        raise ArithmeticError("singular DIIS matrix")
+    x = np.zeros(n)
+    for i in reversed(range(n)):
+        x[i] = (m[i, n] - np.dot(m[i, i + 1:n], x[i + 1:])) / m[i, i]
+    if abs(np.sum(x[:-1]) - 1.0) > 0.5 or not np.all(np.isfinite(x)):
+        raise ArithmeticError("DIIS coefficients rejected")
+    return x
+
+
+class DIISMonitor:
+    """The (E, Z) history and DIIS extrapolation of quasix_core/src/qp/convergence.rs:
+    146-189 (check_convergence: append, keep the last ``space`` entries) and 264-328
+    (apply_diis: error = E - E_last, Pulay matrix with a -1 border, keep ``space``
+    vectors)."""
+
+    def __init__(self, space=6):
+        self.space = space
+        self.hist_e, self.hist_z, self.vecs, self.errs = [], [], [], []
+
+    def record(self, e, z):
+        self.hist_e.append(np.array(e, dtype=float))
+        self.hist_z.append(np.array(z, dtype=float))
+        self.hist_e, self.hist_z = self.hist_e[-self.space:], self.hist_z[-self.space:]
+
+    def extrapolate(self, e, z):
+        e = np.asarray(e, dtype=float)
+        z = np.asarray(z, dtype=float)
+        if not self.hist_e:
+            return e, z
+        self.vecs = (self.vecs + [np.concatenate([e, z])])[-self.space:]
+        self.errs = (self.errs + [e - self.hist_e[-1]])[-self.space:]
+        n = len(self.vecs)
+        if n < 2:
+            return e, z
+        bm = -np.ones((n + 1, n + 1))
+        bm[n, n] = 0.0
+        bm[:n, :n] = np.array([[ei @ ej for ej in self.errs] for ei in self.errs])
+        c = diis_solve(bm, np.r_[np.zeros(n), -1.0])
+        x = sum(ci * v for ci, v in zip(c[:n], self.vecs))
+        return x[:e.size], x[e.size:]
+
+
+def evgw_step_mix(it, mon, e_old, z_old, e_new, z_new, damping, diis, z_min=0.1, z_max=0.999):
+    """Mixing step of quasix_core/src/qp/evgw.rs:832-927: new values at iteration 0, damping
+    at iteration 1 and whenever DIIS is off, fails or moves E by >= 0.5 Ha or Z by >= 0.2;
+    the new values are recorded in the history before the DIIS call (:848)."""
+    damped = ((1 - damping) * e_old + damping * e_new, (1 - damping) * z_old + damping * z_new)
+    if it == 0:
+        mon.record(e_new, z_new)
+        return np.array(e_new, dtype=float), np.array(z_new, dtype=float)
+    if it == 1:
+        mon.record(*damped)
+        return damped
+    if not diis:
+        return damped
+    mon.record(e_new, z_new)
+    try:
+        de, dz = mon.extrapolate(e_new, z_new)
+        ok = np.max(np.abs(de - e_new)) < 0.5 and np.max(np.abs(dz - z_new)) < 0.2
+        e, z = (de, dz) if ok else damped
+    except ArithmeticError:
+        e, z = damped
+    return e, np.clip(z, z_min, z_max)
+
+
 def evgw_cd(mo_energy, B_ov, nocc, targets, freqs, wts, *, n_iter=8, damping=0.5,
-            eta=0.01, tol=1e-4, static=None):
+            eta=0.01, tol=1e-4, static=None, diis=False, diis_space=6):
     """Eigenvalue-self-consistent GW on the CD self-energy.
 
     Follows quasix_core/src/qp/evgw.rs:603-1027: iteration 0 is G0W0 on the mean-field
     energies; iteration k > 0 rebuilds Pi, the Green's function poles and the residues
-    from the previous QP energies (:626-633); iteration 1+ mixes with ``damping``
-    (:842-850); Newton restarts from the previous QP energy (:742).  Orbitals outside
-    ``targets`` keep their mean-field energies.  Convergence: max |dE| over the targets
-    below ``tol`` (:931-1027 with the frontier window equal to the target set).
+    from the previous QP energies (:626-633); the (E, Z) update is ``evgw_step_mix``
+    (:832-927, damping and DIIS); Newton restarts from the previous QP energy (:742).
+    Orbitals outside ``targets`` keep their mean-field energies.  Convergence: max |dE| over
+    the targets below ``tol`` (:931-1027 with the frontier window equal to the target set).
     """
     e0 = np.asarray(mo_energy, dtype=np.float64)
     targets = np.asarray(targets)
     cur = e0.copy()
+    z = np.ones(len(targets))
+    mon = DIISMonitor(diis_space)
     hist = []
     res = None
     for it in range(n_iter):
         guesses = None if it == 0 else cur[targets]
         res = _g0w0_iter(cur, e0, B_ov, nocc, targets, freqs, wts, static, eta, guesses)
+        et, z = evgw_step_mix(it, mon, cur[targets], z, res["E"], res["Z"], damping, diis)
         new = cur.copy()
-        if it == 0:
-            new[targets] = res["E"]
-        else:
-            new[targets] = (1.0 - damping) * cur[targets] + damping * res["E"]
+        new[targets] = et
         dmax = float(np.max(np.abs(new[targets] - cur[targets])))
         hist.append(dict(cycle=it, qp_energies=new.copy(), energy_change=dmax))
         cur = new
         if it > 0 and dmax < tol:
             break
-    return dict(E=cur, history=hist, last=res)
+    return dict(E=cur, Z=z, history=hist, last=res)
 
 
 def _g0w0_iter(cur, e0, B_ov, nocc, targets, freqs, wts, static, eta, guesses):
@@ -690,6 +785,69 @@ def linearized_qp_ac(mo_energy, sigma_x, vxc, coeff, ofit, orbs, nocc, de=1e-6):
         z = 1.0 / (1.0 - (sp - s) / de)
         out[i] = e[p] + z * (sigma_x[p] + s - vxc[p])
     return out
+
+
+def qp_step_ac(coeff, ofit, orbs, e_hf, nocc, sigma_x, vxc, solver, prev_qp, z_min=0.1,
+               z_max=0.999):
+    """QP step of the evGW driver on the Pade model, quasix_core/src/qp/evgw.rs:705-820:
+    linearised at prev_qp (or the mean-field energy at iteration 0) with a clamped forward-
+    difference Z, or Newton from prev_qp / the linearised guess with the linearised value at
+    the starting point as the fallback.  ef from the mean-field energies."""
+    e = np.asarray(e_hf, dtype=np.float64)
+    ef = 0.5 * (e[nocc - 1] + e[nocc])
+    E, Z = np.empty(len(orbs)), np.empty(len(orbs))
+    for i, p in enumerate(orbs):
+        f = lambda om, i=i: pade_thiele(om, ofit[i], coeff[:, i])
+
+        def lin(ev):
+            s0, s1 = f(ev - ef).real, f(ev - ef + 1e-6).real
+            z = min(max(1.0 / (1.0 - (s1 - s0) / 1e-6), z_min), z_max)
+            return e[p] + z * (sigma_x[p] + s0 - vxc[p]), z
+
+        start = e[p] if prev_qp is None else prev_qp[p]
+        if solver == "linearized":
+            E[i], Z[i] = lin(start)
+            continue
+        guess = lin(e[p])[0] if prev_qp is None else start
+        try:
+            r = NewtonQPSolver(e[p], sigma_x[p], vxc[p], ef).solve(f, guess)
+            E[i], Z[i] = r.qp_energy, r.z_factor
+        except RuntimeError:
+            E[i], Z[i] = lin(guess)
+    return E, Z
+
+
+def evgw_ac(mo_energy, lpq, nocc, orbs=None, *, nw=100, x0=0.5, iw_cutoff=5.0, n_iter=8,
+            damping=0.5, tol=1e-4, diis=False, diis_space=6, solver="linearized",
+            sigma_x=None, vxc=None):
+    """evGW on the AC self-energy (the reference's live driver, quasix_core/src/qp/evgw.rs:
+    420-1049): get_sigma_diag on the previous QP energies (:603-690), ac_pade_thiele_diag,
+    the QP step ``qp_step_ac`` and the mixing step ``evgw_step_mix``; ``orbs`` (default all)
+    are updated, the others keep their mean-field energies; stop when max |dE| over ``orbs``
+    falls below ``tol`` after iteration 0."""
+    e0 = np.asarray(mo_energy, dtype=np.float64)
+    nmo = e0.size
+    orbs = np.arange(nmo) if orbs is None else np.asarray(orbs)
+    sx = np.zeros(nmo) if sigma_x is None else np.asarray(sigma_x, dtype=np.float64)
+    vx = np.zeros(nmo) if vxc is None else np.asarray(vxc, dtype=np.float64)
+    cur = e0.copy()
+    z = np.ones(len(orbs))
+    mon = DIISMonitor(diis_space)
+    hist = []
+    for it in range(n_iter):
+        sig, om = get_sigma_diag(cur, lpq, orbs, nocc, nw, x0, iw_cutoff)
+        coeff, ofit = ac_pade_thiele_diag(sig, om)
+        prev = None if it == 0 else cur
+        En, Zn = qp_step_ac(coeff, ofit, orbs, e0, nocc, sx, vx, solver, prev)
+        et, z = evgw_step_mix(it, mon, cur[orbs], z, En, Zn, damping, diis)
+        new = cur.copy()
+        new[orbs] = et
+        dmax = float(np.max(np.abs(new[orbs] - cur[orbs])))
+        hist.append(dict(cycle=it, qp_energies=new.copy(), energy_change=dmax))
+        cur = new
+        if it > 0 and dmax < tol:
+            break
+    return dict(E=cur, Z=z, history=hist)
 
 
 # --------------------------------------------------------------------------------------

@@ -27,6 +27,7 @@ EXPORTED = [
     "qx_gauss_legendre_grid", "qx_scaled_legendre_roots", "qx_rho_response_dev",
     "qx_screened_wc_dev", "qx_wmn_imag_dev", "qx_sigma_cd_dev", "qx_g0w0_cd_dev", "qx_g0w0_cd_ov_dev", "qx_g0w0_cd",
     "qx_set_profiling", "qx_get_profile", "qx_g0w0_adjoint_ov_dev", "qx_g0w0_batch_ov_dev",
+    "qx_sigma_ac_dev", "qx_thiele", "qx_pade_thiele", "qx_ac_pade_thiele_diag", "qx_qp_solve_ac",
 ]
 
 
@@ -118,6 +119,14 @@ def load():
         lib.qx_g0w0_batch_ov_dev.argtypes = [vp, C.POINTER(G0W0Desc), C.c_int,
                                              C.POINTER(BatchInstance), dp, dp, dp, dp, ip,
                                              C.POINTER(Stats)]
+        lib.qx_sigma_ac_dev.argtypes = [vp, C.POINTER(G0W0Desc), dp, dp, dp, ip, ip, ip,
+                                        C.c_double, dp, dp, C.POINTER(C.c_int)]
+        lib.qx_thiele.argtypes = [C.c_int, dp, dp, dp]
+        lib.qx_pade_thiele.argtypes = [C.c_int, C.c_double, dp, dp, dp]
+        lib.qx_ac_pade_thiele_diag.argtypes = [C.c_int, C.c_int, dp, dp, dp, dp,
+                                               C.POINTER(C.c_int)]
+        lib.qx_qp_solve_ac.argtypes = [C.c_int, C.c_int, dp, dp, ip, C.c_int, C.c_int, dp, dp, dp,
+                                       C.c_int, dp, C.POINTER(NewtonConfig), dp, dp, dp, dp, ip]
         lib.qx_set_profiling.argtypes = [vp, C.c_int]
         lib.qx_get_profile.argtypes = [vp, dp, dp]
         non_status = ("qx_destroy", "qx_last_error", "qx_newton_config_default",

@@ -166,3 +166,119 @@ def compute_sigma_c_cd(mo_energy, B_ov, nocc, targets, energies, *, pairs=None, 
                                   _ptr(wt_d), _ptr(wnm), nq, _ptr(E), _ptr(out)))
     o = out.cpu().numpy()
     return o[..., 0] + 1j * o[..., 1]
+
+
+# ---- Analytic continuation (the reference's live self-energy path) --------------------------
+
+def _cplx(a):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.complex128))
+
+
+def _vp(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def pairs_from_lpq(lpq, orbs):
+    """(pair target, partner m, rows [npair, naux]) from a symmetric [naux, nmo, nmo] tensor:
+    the rows lpq[:, n, m] of every listed orbital n that are not identically zero."""
+    lpq = np.asarray(lpq, dtype=np.float64)
+    pt, pm, rows = [], [], []
+    for t, n in enumerate(orbs):
+        blk = lpq[:, n, :].T                          # [nmo, naux]
+        nz = np.flatnonzero(np.any(blk != 0.0, axis=1))
+        pt += [t] * len(nz)
+        pm += list(nz)
+        rows.append(blk[nz])
+    rows = np.concatenate(rows) if rows else np.zeros((0, lpq.shape[0]))
+    return np.asarray(pt, np.int32), np.asarray(pm, np.int32), np.ascontiguousarray(rows)
+
+
+def compute_sigma_c_ac(mo_energy, lpq, nocc, orbs=None, nw=100, x0=0.5, iw_cutoff=5.0, *,
+                       precision="fp64", device=0):
+    """quasix_core.compute_sigma_c_ac (python/g0w0.rs:721-751, correlation_ac.rs:241-449):
+    Sigma_c(i nu) on the AC grid for ``orbs`` (default all) from the full symmetric DF tensor
+    lpq [naux, nmo, nmo].  Returns {sigma_c [norbs, nw_sigma], omega [norbs, nw_sigma],
+    freqs, wts, orbs, nocc}; Wc, the quadrature and the contraction run on the GPU."""
+    dev = _dev(device)
+    h = _lib.handle(device)
+    lpq = np.asarray(lpq, dtype=np.float64)
+    naux, nmo, _ = lpq.shape
+    if nocc <= 0 or nocc >= nmo:
+        raise RuntimeError(f"Invalid nocc={nocc}")
+    orbs = np.arange(nmo, dtype=np.int32) if orbs is None else np.asarray(orbs, np.int32)
+    nvir = nmo - nocc
+    B = _f64(np.ascontiguousarray(lpq[:, :nocc, nocc:].transpose(1, 2, 0)).reshape(-1, naux), dev)
+    e = _f64(mo_energy, dev)
+    pt, pm, rows = pairs_from_lpq(lpq, orbs)
+    d = make_desc(nocc, nvir, naux, nw, x0, 0.01, precision, len(orbs), len(pt))
+    sig = torch.empty((len(orbs), nw + 1, 2), dtype=torch.float64, device=dev)
+    om = torch.empty_like(sig)
+    nws = C.c_int()
+    args = [_f64(rows, dev), _i32(pt, dev), _i32(pm, dev), _i32(orbs, dev)]
+    torch.cuda.synchronize(dev)
+    _run(h, h.lib.qx_sigma_ac_dev(h.h, C.byref(d), _ptr(e), _ptr(B), *map(_ptr, args),
+                                  float(iw_cutoff), _ptr(sig), _ptr(om), C.byref(nws)))
+    n = nws.value
+    s = sig.reshape(-1)[: len(orbs) * n * 2].cpu().numpy().reshape(len(orbs), n, 2)
+    o = om.reshape(-1)[: len(orbs) * n * 2].cpu().numpy().reshape(len(orbs), n, 2)
+    fr, wt = get_scaled_legendre_roots(nw, x0)
+    return dict(sigma_c=s[..., 0] + 1j * s[..., 1], omega=o[..., 0] + 1j * o[..., 1],
+                freqs=fr, wts=wt, orbs=orbs, nocc=nocc)
+
+
+def thiele(fn, zn):
+    """correlation_ac::thiele (correlation_ac.rs:470-509), host code in libqxgw."""
+    fn, zn = _cplx(fn), _cplx(zn)
+    out = np.empty(len(fn), np.complex128)
+    if _lib.load().qx_thiele(len(fn), _vp(fn), _vp(zn), _vp(out)) != _lib.QX_OK:
+        raise RuntimeError("qx_thiele failed")
+    return out
+
+
+def pade_thiele(omega, zn, coeff):
+    """correlation_ac::pade_thiele (correlation_ac.rs:530-566) at a real omega."""
+    zn, coeff = _cplx(zn), _cplx(coeff)
+    out = np.empty(1, np.complex128)
+    if _lib.load().qx_pade_thiele(len(coeff), float(omega), _vp(zn), _vp(coeff),
+                                  _vp(out)) != _lib.QX_OK:
+        raise RuntimeError("qx_pade_thiele failed")
+    return complex(out[0])
+
+
+def ac_pade_thiele_diag(sigma, omega):
+    """correlation_ac::ac_pade_thiele_diag (correlation_ac.rs:587-654):
+    (coeff [nfit, norbs], omega_fit [norbs, nfit])."""
+    sigma, omega = _cplx(sigma), _cplx(omega)
+    norbs, nw = sigma.shape
+    coeff = np.empty((nw, norbs), np.complex128)
+    ofit = np.empty((norbs, nw), np.complex128)
+    nfit = C.c_int()
+    if _lib.load().qx_ac_pade_thiele_diag(norbs, nw, _vp(sigma), _vp(omega), _vp(coeff),
+                                          _vp(ofit), C.byref(nfit)) != _lib.QX_OK:
+        raise RuntimeError("qx_ac_pade_thiele_diag failed")
+    n = nfit.value
+    return (np.ascontiguousarray(coeff.reshape(-1)[: n * norbs].reshape(n, norbs)),
+            np.ascontiguousarray(ofit.reshape(-1)[: norbs * n].reshape(norbs, n)))
+
+
+def qp_solve_ac(coeff, omega_fit, orbs, mo_energy, nocc, sigma_x, vxc, *, solver="linearized",
+                prev_qp=None, newton=None):
+    """QP step of the evGW driver on the Pade model (quasix_core/src/qp/evgw.rs:705-820)."""
+    coeff, ofit = _cplx(coeff), _cplx(omega_fit)
+    orbs = np.ascontiguousarray(orbs, np.int32)
+    e = np.ascontiguousarray(mo_energy, np.float64)
+    sx = np.ascontiguousarray(sigma_x, np.float64)
+    vx = np.ascontiguousarray(vxc, np.float64)
+    prev = None if prev_qp is None else np.ascontiguousarray(prev_qp, np.float64)
+    cfg = _lib.newton_config(**(newton or {}))
+    n = len(orbs)
+    E, Z, sr, si = (np.empty(n) for _ in range(4))
+    st = np.empty(n, np.int32)
+    rc = _lib.load().qx_qp_solve_ac(
+        n, coeff.shape[0], _vp(coeff), _vp(ofit), _vp(orbs), e.size, int(nocc), _vp(e), _vp(sx),
+        _vp(vx), {"linearized": 0, "newton": 1}[solver],
+        None if prev is None else _vp(prev), C.byref(cfg), _vp(E), _vp(Z), _vp(sr), _vp(si),
+        _vp(st))
+    if rc != _lib.QX_OK:
+        raise RuntimeError(f"qx_qp_solve_ac failed ({_lib.STATUS_NAMES.get(rc, rc)})")
+    return dict(qp_energies=E, z_factors=Z, sigma_c=sr + 1j * si, state=st)

@@ -1043,6 +1043,52 @@ qx_status qx_sigma_cd_dev(qx_handle h, const qx_g0w0_desc* d, const double* mo_e
     });
 }
 
+qx_status qx_sigma_ac_dev(qx_handle h, const qx_g0w0_desc* d, const double* mo_energy,
+                          const double* B_ov, const double* B_pair, const int* pair_target,
+                          const int* pair_m, const int* targets, double iw_cutoff,
+                          double* sigma, double* omega, int* nws_out) {
+    return guarded(h, [&] {
+        validate_desc(h, d);
+        REQUIRE(h, mo_energy && B_ov && targets && sigma && omega && nws_out,
+                QX_ERR_INVALID_INPUT, "required pointer is NULL");
+        REQUIRE(h, d->npair == 0 || (B_pair && pair_target && pair_m), QX_ERR_INVALID_INPUT,
+                "pair arrays are NULL");
+        const int nf = d->nfreq, nt = d->ntarget;
+        std::vector<double> fr, wt;
+        scaled_roots(nf, d->x0, fr, wt);
+        double* fr_dev = ws<double>(h, "freqs", nf);
+        double* wt_dev = ws<double>(h, "wts", nf);
+        h2d(h, fr_dev, fr.data(), nf);
+        h2d(h, wt_dev, wt.data(), nf);
+        double* wnm = ws<double>(h, "wnm", (size_t)nf * std::max(d->npair, 1));
+        HostCD cd;
+        setup_cd(h, d, mo_energy, nullptr, B_ov, B_pair, pair_target, pair_m, targets, nullptr,
+                 fr_dev, wt_dev, wnm, cd);
+        build_wnm(h, d->precision, d->nocc, d->nvir, d->naux, mo_energy, B_ov, nf, fr_dev,
+                  d->npair, B_pair, wnm);
+        // grid nu = {0} + {w_k < iw_cutoff} (correlation_ac.rs:270-281)
+        std::vector<double> nu{0.0};
+        for (int k = 0; k < nf; ++k)
+            if (fr[k] < iw_cutoff) nu.push_back(fr[k]);
+        const int nws = std::min((int)nu.size(), nf + 1);
+        double* nu_dev = ws<double>(h, "ac_nu", nws);
+        h2d(h, nu_dev, nu.data(), nws);
+        {
+            Phase ph(h, QX_PHASE_NEWTON);
+            HIPCK(h, launch_sigma_ac(cd.prob, nu_dev, nws, sigma, h->stream));
+        }
+        std::vector<double> om((size_t)nt * nws * 2, 0.0);
+        for (int t = 0; t < nt; ++t)
+            for (int q = 0; q < nws; ++q)
+                om[((size_t)t * nws + q) * 2 + 1] = (cd.targets[t] < d->nocc ? -1.0 : 1.0) * nu[q];
+        h2d(h, omega, om.data(), om.size());
+        HIPCK(h, hipStreamSynchronize(h->stream));
+        resolve_profile(h);
+        *nws_out = nws;
+        return QX_OK;
+    });
+}
+
 qx_status qx_g0w0_cd_dev(qx_handle h, const qx_g0w0_desc* d, const double* mo_energy,
                          const double* mo_energy_ref, const double* B_ov, const double* B_pair,
                          const int* pair_target, const int* pair_m, const int* targets,

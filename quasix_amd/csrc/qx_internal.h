@@ -206,6 +206,10 @@ hipError_t launch_newton_group(const NewtonInst* inst_dev, const int* t_off_dev,
                                int total_targets, const NewtonParams& np, double* e_out,
                                double* z_out, double* sig_re, double* sig_im, int* state,
                                int* iters, hipStream_t st);
+// AC self-energy Sigma_t(omega_q) on the imaginary grid nu[0..nws) (omega_q = -i nu_q for
+// occupied targets, +i nu_q for virtual ones); out [nt][nws] complex (re, im).
+hipError_t launch_sigma_ac(const SigmaProblem& p, const double* nu, int nws, double* out,
+                           hipStream_t st);
 // I_n(E) at nE energies per target: E[t][q] -> out[t][q] (complex, interleaved re/im)
 hipError_t launch_sigma_imag_eval(const SigmaProblem& p, const double* energies, int nq,
                                   double* out, hipStream_t st);

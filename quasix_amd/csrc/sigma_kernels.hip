@@ -416,6 +416,53 @@ __global__ __launch_bounds__(64) void sigma_imag_eval_kernel(SigmaProblem p,
     }
 }
 
+// AC self-energy on the imaginary grid (quasix_core/src/selfenergy/correlation_ac.rs:
+// 283-311, 404-438): for target t (orbital n) and grid point q, omega_q = -i nu_q for an
+// occupied n and +i nu_q for a virtual one (nu_0 = 0),
+//   Sigma_t(omega_q) = -(1/pi) sum_k sum_j W_k wnm[k][j] z / (z^2 + w_k^2),
+//   z = omega_q + ef - eps_m(j).
+// One wavefront per target, lanes over its pairs, QB grid points per pass.
+__global__ __launch_bounds__(64) void sigma_ac_kernel(SigmaProblem p,
+                                                      const double* __restrict__ nu, int nws,
+                                                      double* __restrict__ out) {
+    constexpr int QB = 8;
+    const int t = blockIdx.x, lane = threadIdx.x & 63;
+    const double sgn = p.targets[t] < p.nocc ? -1.0 : 1.0;
+    for (int q0 = 0; q0 < nws; q0 += QB) {
+        double nv[QB], sre[QB], sim[QB];
+        #pragma unroll
+        for (int q = 0; q < QB; ++q) {
+            nv[q] = q0 + q < nws ? sgn * nu[q0 + q] : 0.0;
+            sre[q] = 0.0;
+            sim[q] = 0.0;
+        }
+        for (int j = p.pair_off[t] + lane; j < p.pair_off[t + 1]; j += 64) {
+            const double a = p.ef - p.eps_g[p.pair_m[j]];
+            for (int k = 0; k < p.nf; ++k) {
+                const double w = p.wnm[(size_t)k * p.npair + j] * p.wts[k];
+                const double w2 = p.freqs[k] * p.freqs[k];
+                #pragma unroll
+                for (int q = 0; q < QB; ++q) {
+                    // z = a + i v;  z^2 + w2 = (a^2 - v^2 + w2) + i 2 a v
+                    const double v = nv[q];
+                    const double dre = a * a - v * v + w2, dim = 2.0 * a * v;
+                    const double inv = w / (dre * dre + dim * dim);
+                    sre[q] += (a * dre + v * dim) * inv;
+                    sim[q] += (v * dre - a * dim) * inv;
+                }
+            }
+        }
+        #pragma unroll
+        for (int q = 0; q < QB; ++q) {
+            const double r = -wave_sum(sre[q]) / kPi, i = -wave_sum(sim[q]) / kPi;
+            if (lane == 0 && q0 + q < nws) {
+                out[((size_t)t * nws + q0 + q) * 2] = r;
+                out[((size_t)t * nws + q0 + q) * 2 + 1] = i;
+            }
+        }
+    }
+}
+
 struct DeviceEval {
     const SigmaProblem* p;
     int t;
@@ -704,6 +751,13 @@ hipError_t launch_newton_cd(const SigmaProblem& p, const NewtonParams& np, const
     if (p.nt <= 0) return hipSuccess;
     hipLaunchKernelGGL(newton_cd_kernel, dim3(p.nt), dim3(64), 0, st, p, np, guess, e_out, z_out,
                        sig_re, sig_im, state, iters);
+    return hipGetLastError();
+}
+
+hipError_t launch_sigma_ac(const SigmaProblem& p, const double* nu, int nws, double* out,
+                           hipStream_t st) {
+    if (p.nt <= 0 || nws <= 0) return hipSuccess;
+    hipLaunchKernelGGL(sigma_ac_kernel, dim3(p.nt), dim3(64), 0, st, p, nu, nws, out);
     return hipGetLastError();
 }
 

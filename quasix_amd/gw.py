@@ -167,13 +167,15 @@ def g0w0(mo_energy, B_ov, nocc, targets=None, *, nfreq=48, x0=0.5, eta=0.01, pre
 
 
 def evgw_core(mo_energy, B_ov, nocc, targets=None, *, n_iter=8, damping=0.5, conv_tol=1e-4,
-              nfreq=64, x0=0.5, eta=0.01, precision="fp64", static=None, pairs=None,
-              device=0, newton=None):
+              conv_tol_z=1e-3, diis=False, diis_space=6, nfreq=64, x0=0.5, eta=0.01,
+              precision="fp64", static=None, pairs=None, device=0, newton=None):
     """evGW on the CD self-energy: Pi, poles and Green's function from the previous QP
     energies, QP equation anchored at the mean-field energies (quasix_core/src/qp/evgw.rs:
-    603-1027: iteration 0 direct, later iterations damped, Newton restarted from the previous
-    QP energy).  Orbitals outside ``targets`` keep their mean-field energies.  B stays resident
-    on the device across iterations."""
+    603-1027: iteration 0 direct, iteration 1 damped, later iterations DIIS-or-damped through
+    ``convergence.evgw_mix``; Newton restarted from the previous QP energy).  Orbitals outside
+    ``targets`` keep their mean-field energies.  B stays resident on the device across
+    iterations."""
+    from .convergence import ConvergenceMonitor, evgw_mix
     dev = _dev(device)
     e0 = _f64(mo_energy, dev)
     B = _f64(B_ov, dev)
@@ -181,6 +183,8 @@ def evgw_core(mo_energy, B_ov, nocc, targets=None, *, n_iter=8, damping=0.5, con
     tg = frontier_targets(nocc, nmo) if targets is None else np.asarray(targets, dtype=np.int32)
     tgt = torch.as_tensor(tg, dtype=torch.long, device=dev)
     cur = e0.clone()
+    zc = np.ones(len(tg))
+    mon = ConvergenceMonitor(conv_tol, conv_tol_z, diis_space, use_diis=diis)
     history = []
     last = None
     converged = False
@@ -189,17 +193,111 @@ def evgw_core(mo_energy, B_ov, nocc, targets=None, *, n_iter=8, damping=0.5, con
         last = g0w0(cur, B, nocc, tg, nfreq=nfreq, x0=x0, eta=eta, precision=precision,
                     static=static, pairs=pairs, mo_energy_ref=e0, guess=guess, device=device,
                     newton=newton)
+        e_old = cur[tgt].cpu().numpy()
+        e_new, z_new = np.asarray(last["qp_energies"]), np.asarray(last["z_factors"])
+        et, zt, active = evgw_mix(it, mon, e_old, zc, e_new, z_new, damping, use_diis=diis)
         new = cur.clone()
-        Et = torch.as_tensor(last["qp_energies"], device=dev)
-        new[tgt] = Et if it == 0 else (1.0 - damping) * cur[tgt] + damping * Et
-        dmax = float(torch.max(torch.abs(new[tgt] - cur[tgt])))
+        new[tgt] = torch.as_tensor(et, device=dev)
+        dmax = float(np.max(np.abs(et - e_old)))
         history.append(dict(cycle=it, qp_energies=new.cpu().numpy(), energy_change=dmax,
-                            z_factors=last["z_factors"]))
-        cur = new
+                            z_factors=zt, diis_active=active))
+        cur, zc = new, zt
         if it > 0 and dmax < conv_tol:
             converged = True
             break
-    return dict(qp_energies=cur.cpu().numpy(), converged=converged, n_cycles=len(history),
+    return dict(qp_energies=cur.cpu().numpy(), z_factors=zc, converged=converged,
+                n_cycles=len(history), history=history, last=last, targets=tg)
+
+
+def ov_pairs(B_ov, nocc, nvir, targets, device=0) -> Pairs:
+    """Pair rows of an ov-only tensor: every virtual partner of an occupied target (rows
+    n*nvir + a) and every occupied partner of a virtual target (rows i*nvir + n - nocc)."""
+    dev = _dev(device)
+    B = _f64(B_ov, dev)
+    ts, ms, idx = [], [], []
+    for t, n in enumerate(np.asarray(targets).tolist()):
+        if n < nocc:
+            r = n * nvir + np.arange(nvir)
+            m = nocc + np.arange(nvir)
+        else:
+            r = np.arange(nocc) * nvir + (n - nocc)
+            m = np.arange(nocc)
+        ts.append(np.full(len(r), t))
+        ms.append(m)
+        idx.append(r)
+    rows = B[torch.as_tensor(np.concatenate(idx), device=dev)].contiguous()
+    return Pairs(_i32(np.concatenate(ts), dev), _i32(np.concatenate(ms), dev), rows)
+
+
+def sigma_ac(mo_energy, B_ov, nocc, pairs: Pairs, targets, *, nw=100, x0=0.5, iw_cutoff=5.0,
+             precision="fp64", device=0):
+    """Sigma_c(i nu) of the targets on the AC grid (qx_sigma_ac_dev; correlation_ac.rs:
+    241-449).  Returns (sigma [nt, nws], omega [nt, nws]) complex numpy arrays."""
+    dev = _dev(device)
+    h = _lib.handle(device)
+    e = _f64(mo_energy, dev)
+    B = _f64(B_ov, dev)
+    nmo, naux = e.numel(), B.shape[1]
+    tg = np.asarray(targets, dtype=np.int32)
+    d = make_desc(nocc, nmo - nocc, naux, nw, x0, 0.01, precision, len(tg), pairs.rows.shape[0])
+    sig = torch.empty((len(tg), nw + 1, 2), dtype=torch.float64, device=dev)
+    om = torch.empty_like(sig)
+    nws = C.c_int()
+    t_dev = _i32(tg, dev)
+    torch.cuda.synchronize(dev)
+    h.check(h.lib.qx_sigma_ac_dev(h.h, C.byref(d), _ptr(e), _ptr(B), _ptr(pairs.rows),
+                                  _ptr(pairs.target), _ptr(pairs.m), _ptr(t_dev),
+                                  float(iw_cutoff), _ptr(sig), _ptr(om), C.byref(nws)))
+    n = nws.value
+    sv = sig.reshape(-1)[: len(tg) * n * 2].cpu().numpy().reshape(len(tg), n, 2)
+    ov = om.reshape(-1)[: len(tg) * n * 2].cpu().numpy().reshape(len(tg), n, 2)
+    return sv[..., 0] + 1j * sv[..., 1], ov[..., 0] + 1j * ov[..., 1]
+
+
+def evgw_ac_core(mo_energy, B_ov, nocc, targets=None, *, pairs: Pairs | None = None, n_iter=8,
+                 damping=0.5, conv_tol=1e-4, conv_tol_z=1e-3, diis=False, diis_space=6, nw=100,
+                 x0=0.5, iw_cutoff=5.0, solver="linearized", sigma_x=None, vxc=None,
+                 precision="fp64", device=0, newton=None):
+    """evGW on the AC self-energy, the reference's live driver (quasix_core/src/qp/evgw.rs:
+    420-1049): Sigma_c(i nu) from the previous QP energies on the GPU, Thiele-Pade fit and
+    the QP step on the host (libqxgw), the damping/DIIS step of ``convergence.evgw_mix``.
+    ``targets`` (default all orbitals, as the reference) are updated; ef of the QP step is the
+    mean-field one, as there."""
+    from . import core
+    from .convergence import ConvergenceMonitor, evgw_mix
+    dev = _dev(device)
+    e0 = np.ascontiguousarray(np.asarray(mo_energy.cpu() if torch.is_tensor(mo_energy)
+                                         else mo_energy, dtype=np.float64))
+    B = _f64(B_ov, dev)
+    nmo = e0.size
+    tg = np.arange(nmo, dtype=np.int32) if targets is None else np.asarray(targets, np.int32)
+    if pairs is None:
+        pairs = ov_pairs(B, nocc, nmo - nocc, tg, device)
+    sx = np.zeros(nmo) if sigma_x is None else np.asarray(sigma_x, dtype=np.float64)
+    vx = np.zeros(nmo) if vxc is None else np.asarray(vxc, dtype=np.float64)
+    cur = e0.copy()
+    zc = np.ones(len(tg))
+    mon = ConvergenceMonitor(conv_tol, conv_tol_z, diis_space, use_diis=diis)
+    history, last, converged = [], None, False
+    for it in range(n_iter):
+        sig, om = sigma_ac(cur, B, nocc, pairs, tg, nw=nw, x0=x0, iw_cutoff=iw_cutoff,
+                           precision=precision, device=device)
+        coeff, ofit = core.ac_pade_thiele_diag(sig, om)
+        last = core.qp_solve_ac(coeff, ofit, tg, e0, nocc, sx, vx, solver=solver,
+                                prev_qp=None if it == 0 else cur, newton=newton)
+        e_old = cur[tg]
+        et, zt, active = evgw_mix(it, mon, e_old, zc, last["qp_energies"], last["z_factors"],
+                                  damping, use_diis=diis)
+        cur = cur.copy()
+        cur[tg] = et
+        dmax = float(np.max(np.abs(et - e_old)))
+        history.append(dict(cycle=it, qp_energies=cur.copy(), energy_change=dmax, z_factors=zt,
+                            diis_active=active))
+        zc = zt
+        if it > 0 and dmax < conv_tol:
+            converged = True
+            break
+    return dict(qp_energies=cur, z_factors=zc, converged=converged, n_cycles=len(history),
                 history=history, last=last, targets=tg)
 
 
@@ -208,11 +306,13 @@ def evgw(mo_energy, mo_occ, iaP, ijP, abP, chol_v, vxc_dft, max_cycle=12, conv_t
          nfreq=48, eta=0.01, verbose=1, energy_window=2.0, frontier_convergence=True,
          frontier_window=1.0, n_occ_active=None, n_vir_active=None, qp_solver="linearized",
          newton_max_iterations=50, newton_energy_tolerance=1e-8, *, precision="fp64",
-         targets=None, sigma_x=None, device=0):
+         targets=None, sigma_x=None, sigma_method="cd", device=0):
     """quasix.quasix.evgw on the MI355X path (quasix/src/evgw.rs:22-453).
 
-    Same positional arguments and keyword defaults; the self-energy is the contour-deformation
-    one of this library.  ``sigma_x`` (per orbital) defaults to vxc_dft, i.e. Sigma_x - V_xc = 0
+    Same positional arguments and keyword defaults.  ``sigma_method="cd"`` (default) uses the
+    contour-deformation self-energy of this library; ``"ac"`` uses the reference's own method
+    (Sigma(i nu) + Thiele-Pade, ``qp_solver`` linearized or newton as there, nfreq imaginary
+    points, iw_cutoff 5) and reproduces its iterates.  ``sigma_x`` (per orbital) defaults to vxc_dft, i.e. Sigma_x - V_xc = 0
     (the convention of quasix/quasix/g0w0.py:284-292).  Only the target orbitals (default: the
     n_occ_active/n_vir_active window, else HOMO-5..LUMO+5) are updated.
     """
@@ -230,7 +330,9 @@ def evgw(mo_energy, mo_occ, iaP, ijP, abP, chol_v, vxc_dft, max_cycle=12, conv_t
         ab = np.ascontiguousarray(abP, dtype=np.float64).reshape(nvir * nvir, naux)
     except ValueError as exc:
         raise ValueError(f"Failed to reshape DF tensors: {exc}") from exc
-    if targets is None:
+    if targets is None and sigma_method == "ac":
+        targets = np.arange(nmo)                 # the reference's AC driver updates all orbitals
+    elif targets is None:
         no = 5 if n_occ_active is None else int(n_occ_active) - 1
         nv = 5 if n_vir_active is None else int(n_vir_active) - 1
         targets = frontier_targets(nocc, nmo, no, nv)
@@ -242,12 +344,22 @@ def evgw(mo_energy, mo_occ, iaP, ijP, abP, chol_v, vxc_dft, max_cycle=12, conv_t
     pairs = pairs_from_blocks(ia, mi, ab, nocc, nvir, naux, tg, device)
     newton = dict(max_iterations=int(newton_max_iterations),
                   energy_tolerance=float(newton_energy_tolerance))
-    r = evgw_core(e, B, nocc, tg, n_iter=int(max_cycle), damping=damping, conv_tol=conv_tol,
-                  nfreq=nfreq, eta=eta, precision=precision, static=static, pairs=pairs,
-                  device=device, newton=newton)
+    if sigma_method == "ac":
+        r = evgw_ac_core(e, B, nocc, tg, pairs=pairs, n_iter=int(max_cycle), damping=damping,
+                         conv_tol=conv_tol, conv_tol_z=conv_tol_z, diis=bool(diis),
+                         diis_space=int(diis_space), nw=int(nfreq), solver=qp_solver,
+                         sigma_x=sx, vxc=vxc, precision=precision, device=device,
+                         newton=newton)
+    elif sigma_method == "cd":
+        r = evgw_core(e, B, nocc, tg, n_iter=int(max_cycle), damping=damping,
+                      conv_tol=conv_tol, conv_tol_z=conv_tol_z, diis=bool(diis),
+                      diis_space=int(diis_space), nfreq=nfreq, eta=eta, precision=precision,
+                      static=static, pairs=pairs, device=device, newton=newton)
+    else:
+        raise ValueError(f"sigma_method must be 'cd' or 'ac', got {sigma_method!r}")
     qp = r["qp_energies"]
     z = np.ones(nmo)
-    z[tg] = r["last"]["z_factors"]
+    z[tg] = r["z_factors"]
     sig = np.zeros(nmo, dtype=np.complex128)
     sig[tg] = r["last"]["sigma_c"]
     hist = [dict(cycle=h["cycle"], qp_energies=h["qp_energies"], z_factors=z.copy(),

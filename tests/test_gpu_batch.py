@@ -104,8 +104,9 @@ def test_batch_waves_and_fallback(gpu, monkeypatch):
     from quasix_amd import gw
     inst = _instances(torch, SHAPES[:4], seed0=80)
     # instance 2 gets an occupied level above the virtuals and a large B: eps = I - Pi is
-    # indefinite at the low frequencies, so its Cholesky fails and it is re-solved alone
-    e2, B2 = inst[2][0].clone(), inst[2][1] * 6.0
+    # indefinite at the 9 lowest frequencies (checked with the oracle: min eigenvalue -0.65),
+    # so its Cholesky fails and it is re-solved alone
+    e2, B2 = inst[2][0].clone(), inst[2][1] * 20.0
     e2[inst[2][2] - 1] = 3.4
     inst[2] = (e2, B2, inst[2][2], None)
     monkeypatch.setenv("QXGW_BATCH_GB", "0.02")   # forces several waves
