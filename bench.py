@@ -26,6 +26,22 @@ PEAK = {"bf16": 2.5e15, "fp32": 157.3e12, "fp64": 78.6e12}
 MEV = 1.0 / 27211.386245988
 
 
+PI_KERNELS = {0: "pi_syrk_bf16x3_kernel<32>", 1: "pi_syrk_bf16x3_kernel<64>",
+              2: "pi_syrk_bf16x3_wide_kernel"}
+TRAFFIC = os.path.join(ROOT, "profiles", "pi_hbm_traffic.json")
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of ``kernel`` at this workload, from the rocprofv3 FETCH_SIZE /
+    WRITE_SIZE passes recorded in profiles/ (scaled by the copy calibration there); None when
+    no record matches the kernel the autotuner picked."""
+    try:
+        rec = json.load(open(TRAFFIC))
+    except (OSError, ValueError):
+        return None
+    return rec["bytes_per_launch"] if rec.get("kernel") == kernel else None
+
+
 def pi_flops(nov, naux, nf):
     """Algorithmic flops of the Pi launch: the symmetric rank-N_ov update of every
     frequency, N_ov * naux * (naux + 1) per frequency (2 flops per multiply-add over the
@@ -131,6 +147,47 @@ def run_batch(args):
     print(json.dumps(line), flush=True)
 
 
+def run_evgw(args):
+    """evGW workload: nocc 100 / nvir 1200 / naux 5000 (B 4.8 GB FP64, resident across the
+    iterations), 64 frequencies, 8 eigenvalue iterations updating HOMO-5..LUMO+5; the mixed
+    path (BF16x3 Pi) against the FP64 GPU path; reports the converged HOMO-LUMO gap error."""
+    import torch
+    from quasix_amd import gw, synthetic
+    nocc, nvir, naux, nf, nit = 100, 1200, 5000, 64, 8
+    dev = torch.device("cuda", 0)
+    e, B = synthetic.make_inputs_device(nocc, nvir, naux, seed=4, device=dev)
+    torch.cuda.synchronize()
+    print("inputs ready", file=sys.stderr, flush=True)
+    tg = gw.frontier_targets(nocc, nocc + nvir)
+    out = {}
+    for prec in ("bf16x3", "fp64"):
+        t0 = time.perf_counter()
+        r = gw.evgw_core(e, B, nocc, tg, n_iter=nit, damping=0.5, conv_tol=0.0, nfreq=nf,
+                         precision=prec, progress=lambda it, dm: print(
+                             f"{prec} iteration {it}: max |dE| {dm:.3e} Ha", file=sys.stderr,
+                             flush=True))
+        dt = time.perf_counter() - t0
+        qp = r["qp_energies"]
+        out[prec] = dict(seconds=dt, gap=float(qp[nocc] - qp[nocc - 1]), qp=qp[tg])
+    dgap = abs(out["bf16x3"]["gap"] - out["fp64"]["gap"])
+    line = {
+        "metric": "evGW iterations/s (nocc100/nvir1200/naux5000, 64 frequencies, 8 iterations)",
+        "value": nit / out["bf16x3"]["seconds"], "unit": "iterations/s", "n_gpus": 1,
+        "steps": nit, "warmup": 0, "ms_per_step": out["bf16x3"]["seconds"] / nit * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "bf16x3+f64", "data": "synthetic (device-generated, same distributions)",
+        "config": {"workload": "evgw_cd nocc100/nvir1200/naux5000, 64 frequencies, 8 iterations, "
+                               "HOMO-5..LUMO+5", "nocc": nocc, "nvir": nvir, "naux": naux,
+                   "nfreq": nf, "iterations": nit},
+        "fp64_iterations_per_s": nit / out["fp64"]["seconds"],
+        "speedup_vs_fp64": out["fp64"]["seconds"] / out["bf16x3"]["seconds"],
+        "gap_eV_fp64": out["fp64"]["gap"] * 27.211386245988,
+        "gap_error_meV": dgap / MEV,
+        "max_abs_dE_qp_meV": float(np.max(np.abs(out["bf16x3"]["qp"] - out["fp64"]["qp"]))) / MEV,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -140,8 +197,9 @@ def main():
     ap.add_argument("--precision", default="bf16x3")
     ap.add_argument("--mixed-only", action="store_true",
                     help="skip the FP64 comparison (short profiled runs)")
-    ap.add_argument("--workload", choices=["single", "batch"], default="single",
-                    help="batch: the 256-instance grouped workload (its own JSON line)")
+    ap.add_argument("--workload", choices=["single", "batch", "evgw"], default="single",
+                    help="batch: the 256-instance grouped workload; evgw: 8 evGW iterations at "
+                         "nocc 100 / nvir 1200 / naux 5000 (each its own JSON line)")
     ap.add_argument("--batch-size", type=int, default=256)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB device-to-device 3 times (scales HBM counters)")
@@ -164,6 +222,9 @@ def main():
         return
     if args.workload == "batch":
         run_batch(args)
+        return
+    if args.workload == "evgw":
+        run_evgw(args)
         return
     from quasix_amd import gw, synthetic
 
@@ -188,7 +249,9 @@ def main():
         dt = (time.perf_counter() - t0) / steps
         prof = s.profile() if profile else None
         s.set_profiling(False)
-        return s.result(), dt, prof
+        res = s.result()
+        res["pi_variant"] = s.stats.pi_variant
+        return res, dt, prof
 
     r_mix, dt_mix, prof = timed(args.precision, args.steps, args.warmup, profile=True)
     assert np.all(r_mix["state"] == 0)
@@ -207,6 +270,9 @@ def main():
     phases = {k2: round(v[0] / max(v[1], 1), 4) for k2, v in prof.items() if v[1]}
     phases64 = {k2: round(v[0] / max(v[1], 1), 4) for k2, v in prof64.items() if v[1]}
 
+    kernel = (PI_KERNELS[r_mix["pi_variant"]] if args.precision == "bf16x3"
+              else "pi_syrk_kernel<float, 32>" if args.precision == "fp32"
+              else "pi_syrk_kernel<double, 16>")
     line = {
         "metric": METRIC,
         "value": 1.0 / dt_mix,
@@ -232,10 +298,10 @@ def main():
         "phase_ms_mixed": phases,
         "phase_ms_fp64": phases64,
         "solve_tflops_mixed": solve_flops(nov, NAUX, NF, k) / dt_mix / 1e12,
-        "roofline": {"kernel": "pi_syrk_bf16x3_kernel" if args.precision == "bf16x3" else "pi_syrk_kernel",
-                     "bound": "mfma", "achieved": achieved / 1e12, "peak": peak / 1e12,
-                     "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None,
-                     "flops_per_launch": f_pi, "launch_ms": t_pi * 1e3},
+        "roofline": {"kernel": kernel, "bound": "mfma", "achieved": achieved / 1e12,
+                     "peak": peak / 1e12, "unit": "TFLOP/s", "frac": achieved / peak,
+                     "traffic": measured_traffic(kernel), "flops_per_launch": f_pi,
+                     "launch_ms": t_pi * 1e3},
     }
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(e, B, NOCC, targets)

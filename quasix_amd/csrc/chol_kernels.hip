@@ -28,11 +28,21 @@ __device__ __forceinline__ d4 mfma64(double a, double b, d4 c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
 
+// 64x64 block into LDS by 256 threads: all 16 loads of a thread are issued before the LDS
+// writes, so the block costs one memory latency, not sixteen.
 __device__ __forceinline__ void load_block(double (*dst)[P], const double* __restrict__ src,
                                            int ld, int tid) {
-    for (int idx = tid; idx < NB * NB; idx += 256) {
-        const int r = idx / NB, c = idx % NB;
-        dst[r][c] = src[(size_t)r * ld + c];
+    constexpr int PER = NB * NB / 256;
+    double v[PER];
+    #pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const int idx = tid + 256 * q;
+        v[q] = src[(size_t)(idx / NB) * ld + idx % NB];
+    }
+    #pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const int idx = tid + 256 * q;
+        dst[idx / NB][idx % NB] = v[q];
     }
 }
 
@@ -74,17 +84,32 @@ __device__ __forceinline__ void for_acc(const d4 (&acc)[4], int wave, int lane, 
 // broadcast through `col` (all lanes read the same addresses).  Inverse: lane c solves
 // L x = e_c by column-oriented forward substitution with the same rotation, reading
 // column k of L from Ls[k][.].
-// col: NB doubles of LDS, Ls: NB * NB doubles of LDS (both 16-byte aligned).
+// 1/sqrt(d) to full FP64 precision: the hardware estimate (v_rsq_f64) refined by two
+// Newton steps y <- y + y (1/2 - d y^2 / 2), which keeps the pivot's dependency chain to a
+// few FMAs instead of an IEEE sqrt followed by a division.
+__device__ __forceinline__ double rsqrt_nr(double d) {
+    double y = __builtin_amdgcn_rsq(d);
+    const double h = 0.5 * d;
+    #pragma unroll
+    for (int i = 0; i < 2; ++i) y = fma(y, fma(-h * y, y, 0.5), y);
+    return y;
+}
+
+// col: NB doubles of LDS, Ls: NB * NB doubles of LDS (both 16-byte aligned), idg: NB doubles
+// of LDS (1 / L_kk).
 __device__ __forceinline__ void diag_body(double* __restrict__ a, double* __restrict__ x,
                                           double* __restrict__ linv_diag, int npad, int j,
                                           int* __restrict__ fail, int f, double* col,
-                                          double* Ls) {
+                                          double* Ls, double* idg) {
     const int lane = threadIdx.x;
     const int nblk = npad / NB;
     const size_t off = (size_t)f * npad * npad + (size_t)j * NB * npad + (size_t)j * NB;
     double* A = a + off;
     double* X = x + off;
     double* D = linv_diag + ((size_t)f * nblk + j) * NB * NB;
+    // This wave is on the critical path of the step loop and shares its CU with the column
+    // updates of other frequency groups: give it instruction-issue priority.
+    __builtin_amdgcn_s_setprio(3);
     bool bad = false;
     {
         double r[NB];
@@ -96,7 +121,8 @@ __device__ __forceinline__ void diag_body(double* __restrict__ a, double* __rest
             __syncthreads();
             double d = col[0];
             if (!(d > 0.0)) { bad = true; d = 1.0; }
-            const double sd = sqrt(d), isd = 1.0 / sd;
+            const double isd = rsqrt_nr(d), sd = d * isd;
+            if (lane == k) idg[k] = isd;
             const double l = lane > k ? r[0] * isd : sd;             // L[lane][k] for lane >= k
             Ls[k * NB + slot] = lane >= k ? l : 0.0;
             const double t = lane > k ? l * isd : 0.0;
@@ -112,7 +138,7 @@ __device__ __forceinline__ void diag_body(double* __restrict__ a, double* __rest
         for (int c = 0; c < NB; ++c) y[c] = c == lane ? 1.0 : 0.0;
         for (int k = 0; k < NB; ++k) {
             const double* Lk = Ls + k * NB;      // Lk[c] = L[k + c][k]
-            const double yk = y[0] / Lk[0];      // Linv[k][lane]
+            const double yk = y[0] * idg[k];     // Linv[k][lane]
             X[(size_t)k * npad + lane] = yk;
             D[k * NB + lane] = yk;
             #pragma unroll
@@ -128,7 +154,8 @@ __global__ __launch_bounds__(64) void potrf_diag_kernel(double* __restrict__ a,
                                                         int npad, int j, int* __restrict__ fail) {
     __shared__ __attribute__((aligned(16))) double col[NB];
     __shared__ __attribute__((aligned(16))) double Ls[NB * NB];
-    diag_body(a, x, linv_diag, npad, j, fail, blockIdx.x, col, Ls);
+    __shared__ double idg[NB];
+    diag_body(a, x, linv_diag, npad, j, fail, blockIdx.x, col, Ls, idg);
 }
 
 // After the diagonal factor of step j:
@@ -141,6 +168,7 @@ __device__ __forceinline__ void rowpanel_body(double* __restrict__ a, double* __
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int nblk = npad / NB;
     const double* Djj = linv_diag + ((size_t)f * nblk + j) * NB * NB;
+    __builtin_amdgcn_s_setprio(2);           // critical path of the step loop, as above
     load_block(Ls, Djj, NB, tid);
     d4 acc[4] = {d4{0, 0, 0, 0}, d4{0, 0, 0, 0}, d4{0, 0, 0, 0}, d4{0, 0, 0, 0}};
     if (bx < j) {
@@ -173,11 +201,18 @@ __global__ __launch_bounds__(256) void potrf_rowpanel_kernel(double* __restrict_
 //   bx <  nblk - j : Cholesky tile i = j + bx:   A_ij -= sum_{m<j} L_im L_jm^T   (K = 64 j)
 //   bx >= nblk - j : inverse sums k = bx - (nblk - j) < j:
 //                    X_jk := S_jk = sum_{k<=m<j} L_jm X_mk                    (K = 64 (j - k))
+// LDS tiles are unpadded with XOR-swizzled columns: [row][k] tiles store k at
+// k ^ 4 (row % 8), the [k][col] tile of the NN case stores col at col ^ 16 (k % 2).  The
+// MFMA operand reads (16 rows x 4 k per wave) then touch every double-bank exactly twice,
+// the minimum for 64 lanes, and the staging writes stay conflict-free per row.
 constexpr int GK = 32;
-// As: [row][k]; Bs: NT [col][k], NN [k][col] (first GK rows), both NB x (GK + 1) in LDS.
+__device__ __forceinline__ int swz_rk(int row, int k) { return row * GK + (k ^ (4 * (row & 7))); }
+__device__ __forceinline__ int swz_kc(int k, int c) { return k * NB + (c ^ (16 * (k & 1))); }
+
+// As, Bs: NB * GK doubles of LDS each.
 __device__ __forceinline__ void colupdate_body(double* __restrict__ a, double* __restrict__ x,
-                                               int npad, int j, int bx, int f,
-                                               double (*As)[GK + 1], double (*Bs)[GK + 1]) {
+                                               int npad, int j, int bx, int f, double* As,
+                                               double* Bs) {
     const int nblk = npad / NB;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     double* Af = a + (size_t)f * npad * npad;
@@ -211,22 +246,20 @@ __device__ __forceinline__ void colupdate_body(double* __restrict__ a, double* _
         #pragma unroll
         for (int q = 0; q < 8; ++q) {
             const int idx = tid + 256 * q;
-            As[idx / GK][idx % GK] = ra[q];
-            if (chol) Bs[idx / GK][idx % GK] = rb[q];
-            else reinterpret_cast<double*>(Bs)[(idx / NB) * (NB + 1) + idx % NB] = rb[q];
+            As[swz_rk(idx / GK, idx % GK)] = ra[q];
+            Bs[chol ? swz_rk(idx / GK, idx % GK) : swz_kc(idx / NB, idx % NB)] = rb[q];
         }
         __syncthreads();
         if (k0 + GK < kend) fetch(k0 + GK);
         const int rr = wave * 16 + (lane & 15);
-        const double* BsNN = reinterpret_cast<const double*>(Bs);
         #pragma unroll
         for (int kk = 0; kk < GK; kk += 4) {
             const int q = kk + (lane >> 4);
-            const double av = As[rr][q];
+            const double av = As[swz_rk(rr, q)];
             #pragma unroll
             for (int bj = 0; bj < 4; ++bj) {
                 const int c = bj * 16 + (lane & 15);
-                const double bv = chol ? Bs[c][q] : BsNN[q * (NB + 1) + c];
+                const double bv = Bs[chol ? swz_rk(c, q) : swz_kc(q, c)];
                 acc[bj] = mfma64(av, bv, acc[bj]);
             }
         }
@@ -245,8 +278,8 @@ __device__ __forceinline__ void colupdate_body(double* __restrict__ a, double* _
 __global__ __launch_bounds__(256) void potrf_colupdate_kernel(double* __restrict__ a,
                                                               double* __restrict__ x, int npad,
                                                               int j) {
-    __shared__ double As[NB][GK + 1];
-    __shared__ double Bs[NB][GK + 1];
+    __shared__ double As[NB * GK];
+    __shared__ double Bs[NB * GK];
     colupdate_body(a, x, npad, j, blockIdx.x, blockIdx.y, As, Bs);
 }
 
@@ -256,8 +289,8 @@ __global__ __launch_bounds__(256) void potrf_colupdate_kernel(double* __restrict
 // whose per-instance grid is (items per frequency) x nf.
 __global__ __launch_bounds__(256) void potrf_colupdate_group_kernel(
     const PotrfInst* __restrict__ inst, const int* __restrict__ off, int nact, int j) {
-    __shared__ double As[NB][GK + 1];
-    __shared__ double Bs[NB][GK + 1];
+    __shared__ double As[NB * GK];
+    __shared__ double Bs[NB * GK];
     const int w = blockIdx.x;
     const int q = find_inst(off, nact, w);
     const PotrfInst in = inst[q];
@@ -269,8 +302,9 @@ __global__ __launch_bounds__(64) void potrf_diag_group_kernel(const PotrfInst* _
                                                               int nf, int j) {
     __shared__ __attribute__((aligned(16))) double col[NB];
     __shared__ __attribute__((aligned(16))) double Ls[NB * NB];
+    __shared__ double idg[NB];
     const PotrfInst in = inst[blockIdx.x / nf];
-    diag_body(in.a, in.x, in.linv, in.npad, j, in.fail, blockIdx.x % nf, col, Ls);
+    diag_body(in.a, in.x, in.linv, in.npad, j, in.fail, blockIdx.x % nf, col, Ls, idg);
 }
 
 __global__ __launch_bounds__(256) void potrf_rowpanel_group_kernel(

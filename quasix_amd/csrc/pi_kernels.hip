@@ -353,6 +353,111 @@ __device__ __forceinline__ void pi_tile_bf16x3_body(
     store_tile<float, false>(acc, I0, J0, wr, wc, lane, npad, naux, f, mode, out, 1.0);
 }
 
+// Double-buffered variant: K chunks of 32, two LDS buffers (64 KB, two workgroups per CU),
+// one barrier per chunk.  While the MFMAs of chunk i read buffer i % 2, the same waves split
+// chunk i + 1 (already in registers) into the other buffer and issue the global loads of
+// chunk i + 2, so the VALU split work and the load latency overlap the matrix work.  Rows of
+// 32 bf16 (4 chunks of 16 B) are unpadded with the chunk index XOR-swizzled by (row / 4) % 4:
+// the 16 rows x 4 chunks that a wave's fragment read or a 64-thread staging write touches
+// cover all 16 bank groups once.
+constexpr int DKB = 32;
+__device__ __forceinline__ int db_off(int buf, int arr, int row, int chunk) {
+    return ((buf * 4 + arr) * TILE + row) * DKB + ((chunk ^ ((row >> 2) & 3)) << 3);
+}
+
+// L: 2 * 4 * TILE * DKB bf16 of LDS (16-byte aligned).
+__device__ __forceinline__ void pi_tile_bf16x3_db_body(
+    const uint16_t* __restrict__ bt_hi, const uint16_t* __restrict__ bt_lo,
+    const float* __restrict__ bt_f32, const float* __restrict__ s32, int kpad, int naux,
+    int npad, double* __restrict__ out, PiOut mode, int I0, int J0, int f, __bf16* L) {
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);   // wave-uniform (scalar)
+    const int wr = wave >> 1, wc = wave & 1;
+    const float* sf = s32 + (size_t)f * kpad;
+    // staging: thread owns chunk tid % 4 of rows tid / 4 and tid / 4 + 64
+    const int row0 = tid >> 2, ch0 = tid & 3, kq0 = ch0 * 8;
+    u16x8 rh[2], rl[2];
+    f4 rc[4], rs[2];
+    auto fetch = [&](int k0) {
+        #pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const int row = row0 + q * 64;
+            const bool va = I0 + row < npad, vc = J0 + row < npad;
+            const size_t oa = (size_t)(I0 + row) * kpad + k0 + kq0;
+            const size_t oc = (size_t)(J0 + row) * kpad + k0 + kq0;
+            rh[q] = va ? *reinterpret_cast<const u16x8*>(bt_hi + oa) : u16x8{0, 0, 0, 0, 0, 0, 0, 0};
+            rl[q] = va ? *reinterpret_cast<const u16x8*>(bt_lo + oa) : u16x8{0, 0, 0, 0, 0, 0, 0, 0};
+            rc[2 * q] = vc ? *reinterpret_cast<const f4*>(bt_f32 + oc) : f4{0.f, 0.f, 0.f, 0.f};
+            rc[2 * q + 1] = vc ? *reinterpret_cast<const f4*>(bt_f32 + oc + 4) : f4{0.f, 0.f, 0.f, 0.f};
+        }
+        rs[0] = *reinterpret_cast<const f4*>(sf + k0 + kq0);
+        rs[1] = *reinterpret_cast<const f4*>(sf + k0 + kq0 + 4);
+    };
+    auto stage = [&](int buf) {
+        #pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const int row = row0 + q * 64;
+            *reinterpret_cast<u16x8*>(L + db_off(buf, 0, row, ch0)) = rh[q];
+            *reinterpret_cast<u16x8*>(L + db_off(buf, 1, row, ch0)) = rl[q];
+            #pragma unroll
+            for (int hq = 0; hq < 2; ++hq) {
+                const f4 v = rc[2 * q + hq] * rs[hq];
+                const bf16x4 h = __builtin_convertvector(v, bf16x4);
+                const bf16x4 l = __builtin_convertvector(v - __builtin_convertvector(h, f4), bf16x4);
+                *reinterpret_cast<bf16x4*>(L + db_off(buf, 2, row, ch0) + 4 * hq) = h;
+                *reinterpret_cast<bf16x4*>(L + db_off(buf, 3, row, ch0) + 4 * hq) = l;
+            }
+        }
+    };
+
+    f4 acc[4][4];
+    #pragma unroll
+    for (int bi = 0; bi < 4; ++bi)
+        #pragma unroll
+        for (int bj = 0; bj < 4; ++bj) acc[bi][bj] = f4{0.f, 0.f, 0.f, 0.f};
+    bool live[4][4];
+    #pragma unroll
+    for (int bi = 0; bi < 4; ++bi)
+        #pragma unroll
+        for (int bj = 0; bj < 4; ++bj) live[bi][bj] = block_live(I0, J0, wr, wc, bi, bj, naux);
+
+    const int nch = kpad / DKB;
+    fetch(0);
+    stage(0);
+    if (nch > 1) fetch(DKB);
+    __syncthreads();
+    const int r16 = lane & 15, kc = lane >> 4;
+    for (int i = 0; i < nch; ++i) {
+        const int b = i & 1;
+        bf16x8 ah[4], al[4], ch[4], cl[4];
+        #pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            ah[q] = *reinterpret_cast<const bf16x8*>(L + db_off(b, 0, wr * 64 + q * 16 + r16, kc));
+            al[q] = *reinterpret_cast<const bf16x8*>(L + db_off(b, 1, wr * 64 + q * 16 + r16, kc));
+            ch[q] = *reinterpret_cast<const bf16x8*>(L + db_off(b, 2, wc * 64 + q * 16 + r16, kc));
+            cl[q] = *reinterpret_cast<const bf16x8*>(L + db_off(b, 3, wc * 64 + q * 16 + r16, kc));
+        }
+        mfma_bf16x3(acc, ah, al, ch, cl, live);
+        if (i + 1 < nch) stage(b ^ 1);
+        if (i + 2 < nch) fetch((i + 2) * DKB);
+        __syncthreads();
+    }
+    store_tile<float, false>(acc, I0, J0, wr, wc, lane, npad, naux, f, mode, out, 1.0);
+}
+
+__global__ __launch_bounds__(256, 2) void pi_syrk_bf16x3_db_kernel(
+    const uint16_t* __restrict__ bt_hi, const uint16_t* __restrict__ bt_lo,
+    const float* __restrict__ bt_f32, const float* __restrict__ s32, int kpad, int naux,
+    int npad, int nf, int ntiles, double* __restrict__ out, PiOut mode) {
+    __shared__ __attribute__((aligned(16))) __bf16 L[2 * 4 * TILE * DKB];
+    const int work = xcd_remap(blockIdx.x, ntiles * nf);
+    const int tile = work / nf, f = work % nf;
+    int I, J;
+    tri_index(tile, I, J);
+    pi_tile_bf16x3_db_body(bt_hi, bt_lo, bt_f32, s32, kpad, naux, npad, out, mode, I * TILE,
+                           J * TILE, f, L);
+}
+
 template <int KB>
 __global__ __launch_bounds__(256, 2) void pi_syrk_bf16x3_kernel(
     const uint16_t* __restrict__ bt_hi, const uint16_t* __restrict__ bt_lo,
@@ -408,6 +513,16 @@ __global__ __launch_bounds__(256, 2) void pi_group_bf16x3_kernel(
     for (int it = 0; group_item(inst, off, ninst, nf, total, it, in, I0, J0, f); ++it)
         pi_tile_bf16x3_body<KB>(in.bt_hi, in.bt_lo, in.bt_f32, in.s32, in.kpad, in.naux, in.npad,
                                 in.out, PiOut::EpsLower, I0, J0, f, Ah, Al, Ch, Cl);
+}
+
+__global__ __launch_bounds__(256, 2) void pi_group_bf16x3_db_kernel(
+    const PiInst* __restrict__ inst, const int* __restrict__ off, int ninst, int nf, int total) {
+    __shared__ __attribute__((aligned(16))) __bf16 L[2 * 4 * TILE * DKB];
+    PiInst in;
+    int I0, J0, f;
+    for (int it = 0; group_item(inst, off, ninst, nf, total, it, in, I0, J0, f); ++it)
+        pi_tile_bf16x3_db_body(in.bt_hi, in.bt_lo, in.bt_f32, in.s32, in.kpad, in.naux, in.npad,
+                               in.out, PiOut::EpsLower, I0, J0, f, L);
 }
 
 __global__ __launch_bounds__(256, 2) void pi_group_f64_kernel(
@@ -599,7 +714,10 @@ hipError_t launch_pi_syrk(int precision, const double* B, const PiOperands& ops,
             hipLaunchKernelGGL(pi_syrk_bf16x3_wide_kernel, dim3(nt * nf), dim3(512), 0, st,
                                ops.bt_hi, ops.bt_lo, ops.bt_f32, ops.s32, ops.kpad, naux, npad, nf,
                                nt, TJ, out, mode);
-        } else if (variant == 1)
+        } else if (variant == 3)
+            hipLaunchKernelGGL(pi_syrk_bf16x3_db_kernel, grid, block, 0, st, ops.bt_hi, ops.bt_lo,
+                               ops.bt_f32, ops.s32, ops.kpad, naux, npad, nf, ntiles, out, mode);
+        else if (variant == 1)
             hipLaunchKernelGGL(pi_syrk_bf16x3_kernel<64>, grid, block, 0, st, ops.bt_hi, ops.bt_lo,
                                ops.bt_f32, ops.s32, ops.kpad, naux, npad, nf, ntiles, out, mode);
         else
@@ -609,7 +727,7 @@ hipError_t launch_pi_syrk(int precision, const double* B, const PiOperands& ops,
     return hipGetLastError();
 }
 
-int pi_variant_count(int precision) { return precision == 2 ? 3 : 1; }
+int pi_variant_count(int precision) { return precision == 2 ? 4 : 1; }
 
 hipError_t launch_pi_group(int precision, const PiInst* inst_dev, const int* off_dev, int ninst,
                            int nf, int total, int grid, hipStream_t st, int variant) {
@@ -618,6 +736,9 @@ hipError_t launch_pi_group(int precision, const PiInst* inst_dev, const int* off
     if (precision == 0)
         hipLaunchKernelGGL(pi_group_f64_kernel, dim3(grid), dim3(256), 0, st, inst_dev, off_dev,
                            ninst, nf, total);
+    else if (precision == 2 && variant == 2)
+        hipLaunchKernelGGL(pi_group_bf16x3_db_kernel, dim3(grid), dim3(256), 0, st, inst_dev,
+                           off_dev, ninst, nf, total);
     else if (precision == 2 && variant == 1)
         hipLaunchKernelGGL(pi_group_bf16x3_kernel<64>, dim3(grid), dim3(256), 0, st, inst_dev,
                            off_dev, ninst, nf, total);
@@ -629,13 +750,15 @@ hipError_t launch_pi_group(int precision, const PiInst* inst_dev, const int* off
     return hipGetLastError();
 }
 
-int pi_group_variant_count(int precision) { return precision == 2 ? 2 : 1; }
+int pi_group_variant_count(int precision) { return precision == 2 ? 3 : 1; }
 
 int pi_group_occupancy(int precision, int variant) {
     int n = 0;
     hipError_t e;
     if (precision == 0)
         e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, pi_group_f64_kernel, 256, 0);
+    else if (variant == 2)
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, pi_group_bf16x3_db_kernel, 256, 0);
     else if (variant == 1)
         e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, pi_group_bf16x3_kernel<64>, 256, 0);
     else

@@ -34,24 +34,45 @@ __device__ __forceinline__ d4 mfma64(double a, double b, d4 c) {
 
 // ---- Wc_nm contraction -----------------------------------------------------------------
 // Tile of 64 pairs (from j0) x 64 auxiliary columns (block pb) of frequency f; Bs, Ms:
-// NB x P doubles of LDS.
+// NB x P doubles of LDS.  The K loop runs over the blocks qb <= pb of the triangular M; the
+// next block's 2 x 16 loads per thread are in flight during the current block's MFMAs, and
+// the last staged pair block (qb = pb) is the one the row-dot epilogue needs.
 __device__ __forceinline__ void wnm_linv_body(const double* __restrict__ bp,
                                               const double* __restrict__ x, int npair, int npad,
                                               int nblk, double* __restrict__ wpart, int j0,
                                               int pb, int f, double (*Bs)[P], double (*Ms)[P]) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const double* X = x + (size_t)f * npad * npad;
+    // staging: thread owns columns 2 c2, 2 c2 + 1 of rows r0 + 8 q (16-byte loads)
+    constexpr int PER = NB * NB / 512;
+    const int c2 = (tid & 31) * 2, r0 = tid >> 5;
+    double2 rb[PER], rm[PER];
+    auto fetch = [&](int qb) {
+        #pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            const int r = r0 + 8 * q, j = j0 + r;
+            rb[q] = j < npair ? *reinterpret_cast<const double2*>(bp + (size_t)j * npad + qb * NB + c2)
+                              : double2{0.0, 0.0};
+            double2 m = *reinterpret_cast<const double2*>(X + (size_t)(pb * NB + r) * npad +
+                                                          qb * NB + c2);
+            if (qb == pb && r == c2) m.x -= 1.0;
+            if (qb == pb && r == c2 + 1) m.y -= 1.0;
+            rm[q] = m;
+        }
+    };
     d4 acc[4] = {d4{0, 0, 0, 0}, d4{0, 0, 0, 0}, d4{0, 0, 0, 0}, d4{0, 0, 0, 0}};
+    fetch(0);
     for (int qb = 0; qb <= pb; ++qb) {
-        for (int idx = tid; idx < NB * NB; idx += 256) {
-            const int r = idx / NB, c = idx % NB;
-            const int j = j0 + r;
-            Bs[r][c] = j < npair ? bp[(size_t)j * npad + qb * NB + c] : 0.0;
-            double m = X[(size_t)(pb * NB + r) * npad + qb * NB + c];
-            if (qb == pb && r == c) m -= 1.0;
-            Ms[r][c] = m;
+        #pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            const int r = r0 + 8 * q;
+            Bs[r][c2] = rb[q].x;
+            Bs[r][c2 + 1] = rb[q].y;
+            Ms[r][c2] = rm[q].x;
+            Ms[r][c2 + 1] = rm[q].y;
         }
         __syncthreads();
+        if (qb < pb) fetch(qb + 1);
         const int r = wave * 16 + (lane & 15);
         #pragma unroll
         for (int kk = 0; kk < NB; kk += 4) {
@@ -60,15 +81,9 @@ __device__ __forceinline__ void wnm_linv_body(const double* __restrict__ bp,
             #pragma unroll
             for (int bj = 0; bj < 4; ++bj) acc[bj] = mfma64(a, Ms[bj * 16 + (lane & 15)][q], acc[bj]);
         }
-        __syncthreads();
+        if (qb < pb) __syncthreads();
     }
-    // epilogue: b[j][P] for P in block pb
-    for (int idx = tid; idx < NB * NB; idx += 256) {
-        const int r = idx / NB, c = idx % NB;
-        const int j = j0 + r;
-        Bs[r][c] = j < npair ? bp[(size_t)j * npad + pb * NB + c] : 0.0;
-    }
-    __syncthreads();
+    // epilogue: Bs holds b[j][P] for P in block pb (the last staged block)
     double rs[4] = {0, 0, 0, 0};
     #pragma unroll
     for (int bj = 0; bj < 4; ++bj)

@@ -168,13 +168,13 @@ def g0w0(mo_energy, B_ov, nocc, targets=None, *, nfreq=48, x0=0.5, eta=0.01, pre
 
 def evgw_core(mo_energy, B_ov, nocc, targets=None, *, n_iter=8, damping=0.5, conv_tol=1e-4,
               conv_tol_z=1e-3, diis=False, diis_space=6, nfreq=64, x0=0.5, eta=0.01,
-              precision="fp64", static=None, pairs=None, device=0, newton=None):
+              precision="fp64", static=None, pairs=None, device=0, newton=None, progress=None):
     """evGW on the CD self-energy: Pi, poles and Green's function from the previous QP
     energies, QP equation anchored at the mean-field energies (quasix_core/src/qp/evgw.rs:
     603-1027: iteration 0 direct, iteration 1 damped, later iterations DIIS-or-damped through
     ``convergence.evgw_mix``; Newton restarted from the previous QP energy).  Orbitals outside
     ``targets`` keep their mean-field energies.  B stays resident on the device across
-    iterations."""
+    iterations.  ``progress(iteration, max_dE)`` is called after each iteration if given."""
     from .convergence import ConvergenceMonitor, evgw_mix
     dev = _dev(device)
     e0 = _f64(mo_energy, dev)
@@ -202,6 +202,8 @@ def evgw_core(mo_energy, B_ov, nocc, targets=None, *, n_iter=8, damping=0.5, con
         history.append(dict(cycle=it, qp_energies=new.cpu().numpy(), energy_change=dmax,
                             z_factors=zt, diis_active=active))
         cur, zc = new, zt
+        if progress is not None:
+            progress(it, dmax)
         if it > 0 and dmax < conv_tol:
             converged = True
             break

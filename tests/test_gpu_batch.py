@@ -120,3 +120,18 @@ def test_batch_waves_and_fallback(gpu, monkeypatch):
     ref = _single(gw, inst, "fp64")
     for q in range(len(inst)):
         _close(f"waves inst {q}", got[q]["qp_energies"], ref[q]["qp_energies"], 1e-10)
+
+
+@pytest.mark.parametrize("variant", [0, 1, 2])
+def test_batch_every_grouped_pi_variant(gpu, variant, monkeypatch):
+    """Each grouped BF16x3 Pi kernel instance (K32, K64, double-buffered K32), forced."""
+    import torch
+    from quasix_amd import gw
+    inst = _instances(torch, SHAPES[:4], seed0=100)
+    r64 = gw.g0w0_batch(inst, nfreq=NF, precision="fp64")
+    monkeypatch.setenv("QXGW_PI_VARIANT", str(variant))
+    bs = gw.G0W0BatchSolver(inst, nfreq=NF, precision="bf16x3")
+    bs.run()
+    assert bs.stats.pi_variant == variant
+    for q, (a, b) in enumerate(zip(bs.results(), r64)):
+        _close(f"grouped v{variant} inst {q}", a["qp_energies"], b["qp_energies"], MEV / 100)
