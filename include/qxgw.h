@@ -192,6 +192,24 @@ QXGW_API qx_status qx_g0w0_cd_ov_dev(qx_handle h, const qx_g0w0_desc* d,
                                      const double* static_terms, const double* guess,
                                      double* qp_energy, double* z_factor, double* sigma_re,
                                      double* sigma_im, int* state, qx_stats* stats);
+/* ---- DF tensor construction ----------------------------------------------------------- */
+/* AO -> MO transform of Cholesky-normalised 3-centre vectors (device pointers, row-major):
+ * b_ov[i * nvir + a][P] = sum_{mu,nu} c_occ[mu][i] j3c_ao[mu][nu][P] c_vir[nu][a], with
+ * j3c_ao [nao][nao][naux], c_occ [nao][nocc], c_vir [nao][nvir]; FP64 MFMA GEMMs through a
+ * [nocc][nao][naux] workspace.  Replaces df::mo_transform::transform_mo_3center
+ * (quasix_core/src/df/mo_transform.rs:136-260). */
+QXGW_API qx_status qx_df_mo_transform_dev(qx_handle h, int nao, int nocc, int nvir, int naux,
+                                          const double* j3c_ao, const double* c_occ,
+                                          const double* c_vir, double* b_ov);
+
+/* ---- Exchange ------------------------------------------------------------------------- */
+/* Diagonal of the exchange self-energy from the [nmo][nocc][naux] block mi (the reference's
+ * ijP / miP tensor): sigma_x[p] = -sum_i sum_R mi[p][i][R]^2 (device pointers).  Replaces
+ * EvGWDriver::compute_exchange (quasix_core/src/qp/evgw.rs:1167-1193), whose diagonal the
+ * driver uses (:501-502) and quasix/src/evgw.rs:330 returns as "sigma_x". */
+QXGW_API qx_status qx_exchange_diag_dev(qx_handle h, int nmo, int nocc, int naux,
+                                        const double* mi, double* sigma_x);
+
 /* ---- Analytic continuation (the reference's live self-energy method) ------------------- */
 /* Sigma_c(i nu) of the targets on the AC grid: Wc_nm at desc->nfreq scaled Gauss-Legendre
  * frequencies (desc->x0; the reference default is 100), grid nu = {0} + {w_k < iw_cutoff},

@@ -41,6 +41,10 @@ Parity status per function (checks live in tests/test_oracle_*.py):
       parts (as evgw_cd)
   NewtonQPSolver                                            pinned (reference tests
       newton_solver.rs:609-732 restated as known answers)
+  transform_mo_3center                                      pinned (identity coefficients
+      select (i, nocc + a) AO pairs; orthogonal rotation preserves the Frobenius norm)
+  exchange_diag                                             pinned (rank-one closed form
+      -a_p^2 |b|^2 |c|^2, tests/test_oracle_core.py)
   diis_solve, DIISMonitor, evgw_step_mix                    pinned (numpy.linalg.solve,
       constrained least-squares closed form of the DIIS combination, the singular
       zero-error case that the driver's call order produces, tests/test_convergence.py)
@@ -531,6 +535,20 @@ def g0w0_cd(mo_energy, B_ov, nocc, targets, freqs, wts, *, pairs=None, static=No
             z = min(max(1.0 / (1.0 - (s1.real - s0.real) / 1e-6), 0.1), 0.999)
             E[t], Z[t], S[t], conv[t] = e[n] + z * (static[t] + s0.real), z, s0, False
     return dict(E=E, Z=Z, sigma=S, converged=conv, wnm=wnm, pairs=pairs, ef=ef)
+
+
+def transform_mo_3center(j3c_ao, c_occ, c_vir):
+    """(ia|P) = sum_{mu nu} C_occ[mu, i] (mu nu|P) C_vir[nu, a] as [nocc * nvir, naux].
+    Restates quasix_core/src/df/mo_transform.rs:136-260 (the AO -> MO transform)."""
+    t = np.einsum("mi,mnP->inP", c_occ, j3c_ao)
+    return np.einsum("inP,na->iaP", t, c_vir).reshape(c_occ.shape[1] * c_vir.shape[1], -1)
+
+
+def exchange_diag(mi, nmo, nocc):
+    """Diagonal of EvGWDriver::compute_exchange (quasix_core/src/qp/evgw.rs:1167-1193):
+    Sigma_x[p, p] = -sum_{i < nocc} sum_R (p i | R)^2 over the [nmo, nocc, naux] block."""
+    m = np.asarray(mi, dtype=np.float64).reshape(nmo, nocc, -1)
+    return -np.einsum("piR,piR->p", m, m)
 
 
 def diis_solve(a, b):

@@ -28,6 +28,7 @@ EXPORTED = [
     "qx_screened_wc_dev", "qx_wmn_imag_dev", "qx_sigma_cd_dev", "qx_g0w0_cd_dev", "qx_g0w0_cd_ov_dev", "qx_g0w0_cd",
     "qx_set_profiling", "qx_get_profile", "qx_g0w0_adjoint_ov_dev", "qx_g0w0_batch_ov_dev",
     "qx_sigma_ac_dev", "qx_thiele", "qx_pade_thiele", "qx_ac_pade_thiele_diag", "qx_qp_solve_ac",
+    "qx_exchange_diag_dev", "qx_df_mo_transform_dev",
 ]
 
 
@@ -127,6 +128,9 @@ def load():
                                                C.POINTER(C.c_int)]
         lib.qx_qp_solve_ac.argtypes = [C.c_int, C.c_int, dp, dp, ip, C.c_int, C.c_int, dp, dp, dp,
                                        C.c_int, dp, C.POINTER(NewtonConfig), dp, dp, dp, dp, ip]
+        lib.qx_exchange_diag_dev.argtypes = [vp, C.c_int, C.c_int, C.c_int, dp, dp]
+        lib.qx_df_mo_transform_dev.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, dp, dp,
+                                               dp, dp]
         lib.qx_set_profiling.argtypes = [vp, C.c_int]
         lib.qx_get_profile.argtypes = [vp, dp, dp]
         non_status = ("qx_destroy", "qx_last_error", "qx_newton_config_default",

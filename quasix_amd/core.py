@@ -282,3 +282,38 @@ def qp_solve_ac(coeff, omega_fit, orbs, mo_energy, nocc, sigma_x, vxc, *, solver
     if rc != _lib.QX_OK:
         raise RuntimeError(f"qx_qp_solve_ac failed ({_lib.STATUS_NAMES.get(rc, rc)})")
     return dict(qp_energies=E, z_factors=Z, sigma_c=sr + 1j * si, state=st)
+
+
+def compute_exchange_diag(mi, nmo, nocc, *, device=0):
+    """Diagonal exchange self-energy from the [nmo, nocc, naux] block (the reference's ijP /
+    miP): sigma_x[p] = -sum_i sum_R mi[p, i, R]^2 (EvGWDriver::compute_exchange,
+    quasix_core/src/qp/evgw.rs:1167-1193, whose diagonal the driver uses)."""
+    dev = _dev(device)
+    h = _lib.handle(device)
+    m = _f64(mi, dev).reshape(nmo, nocc, -1)
+    out = torch.empty(nmo, dtype=torch.float64, device=dev)
+    torch.cuda.synchronize(dev)
+    _run(h, h.lib.qx_exchange_diag_dev(h.h, int(nmo), int(nocc), int(m.shape[2]), _ptr(m),
+                                       _ptr(out)))
+    return out.cpu().numpy()
+
+
+def transform_mo_3center(j3c_ao, c_occ, c_vir, *, device=0):
+    """df::mo_transform::transform_mo_3center (quasix_core/src/df/mo_transform.rs:136-260):
+    (ia|P) = sum_{mu nu} C_occ[mu, i] (mu nu|P) C_vir[nu, a] for Cholesky-normalised
+    3-centre vectors j3c_ao [nao, nao, naux]; returns [nocc * nvir, naux] (the iaP layout),
+    computed with FP64 MFMA GEMMs on the GPU."""
+    dev = _dev(device)
+    h = _lib.handle(device)
+    J = _f64(j3c_ao, dev)
+    co = _f64(c_occ, dev)
+    cv = _f64(c_vir, dev)
+    nao, _, naux = J.shape
+    if co.shape[0] != nao or cv.shape[0] != nao or J.shape[1] != nao:
+        raise RuntimeError("MO coefficient dimensions don't match the AO basis")
+    nocc, nvir = co.shape[1], cv.shape[1]
+    out = torch.empty((nocc * nvir, naux), dtype=torch.float64, device=dev)
+    torch.cuda.synchronize(dev)
+    _run(h, h.lib.qx_df_mo_transform_dev(h.h, nao, nocc, nvir, naux, _ptr(J), _ptr(co),
+                                         _ptr(cv), _ptr(out)))
+    return out.cpu().numpy()

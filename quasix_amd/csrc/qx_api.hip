@@ -1043,6 +1043,35 @@ qx_status qx_sigma_cd_dev(qx_handle h, const qx_g0w0_desc* d, const double* mo_e
     });
 }
 
+qx_status qx_df_mo_transform_dev(qx_handle h, int nao, int nocc, int nvir, int naux,
+                                 const double* j3c_ao, const double* c_occ, const double* c_vir,
+                                 double* b_ov) {
+    return guarded(h, [&] {
+        REQUIRE(h, nao > 0 && nocc > 0 && nvir > 0 && naux > 0, QX_ERR_INVALID_INPUT,
+                "Invalid dimensions nao=%d nocc=%d nvir=%d naux=%d", nao, nocc, nvir, naux);
+        REQUIRE(h, j3c_ao && c_occ && c_vir && b_ov, QX_ERR_INVALID_INPUT, "NULL pointer");
+        REQUIRE(h, nocc <= 65535 && (long long)nao * naux < (1ll << 31), QX_ERR_INVALID_INPUT,
+                "dimensions exceed the transform's grid limits");
+        double* tmp = ws<double>(h, "mo_tmp", (size_t)nocc * nao * naux);
+        HIPCK(h, launch_mo_transform(j3c_ao, c_occ, c_vir, nao, nocc, nvir, naux, tmp, b_ov,
+                                     h->stream));
+        HIPCK(h, hipStreamSynchronize(h->stream));
+        return QX_OK;
+    });
+}
+
+qx_status qx_exchange_diag_dev(qx_handle h, int nmo, int nocc, int naux, const double* mi,
+                               double* sigma_x) {
+    return guarded(h, [&] {
+        REQUIRE(h, nmo > 0 && nocc > 0 && nocc <= nmo && naux > 0, QX_ERR_INVALID_INPUT,
+                "Invalid dimensions nmo=%d nocc=%d naux=%d", nmo, nocc, naux);
+        REQUIRE(h, mi && sigma_x, QX_ERR_INVALID_INPUT, "NULL pointer");
+        HIPCK(h, launch_exchange_diag(mi, nmo, nocc, naux, sigma_x, h->stream));
+        HIPCK(h, hipStreamSynchronize(h->stream));
+        return QX_OK;
+    });
+}
+
 qx_status qx_sigma_ac_dev(qx_handle h, const qx_g0w0_desc* d, const double* mo_energy,
                           const double* B_ov, const double* B_pair, const int* pair_target,
                           const int* pair_m, const int* targets, double iw_cutoff,

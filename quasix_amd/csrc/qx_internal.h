@@ -209,6 +209,9 @@ hipError_t launch_newton_group(const NewtonInst* inst_dev, const int* t_off_dev,
                                int total_targets, const NewtonParams& np, double* e_out,
                                double* z_out, double* sig_re, double* sig_im, int* state,
                                int* iters, hipStream_t st);
+// sx[p] = -sum_{i,R} mi[p][i][R]^2 for the [nmo][nocc][naux] block (exchange diagonal).
+hipError_t launch_exchange_diag(const double* mi, int nmo, int nocc, int naux, double* sx,
+                                hipStream_t st);
 // AC self-energy Sigma_t(omega_q) on the imaginary grid nu[0..nws) (omega_q = -i nu_q for
 // occupied targets, +i nu_q for virtual ones); out [nt][nws] complex (re, im).
 hipError_t launch_sigma_ac(const SigmaProblem& p, const double* nu, int nws, double* out,
@@ -225,6 +228,13 @@ hipError_t launch_residue_jobs(const double* B, const double* eps, int nocc, int
                                const double* brows,      // [njobs][naux]
                                double* work,             // [njobs][naux][2*naux+2] complex
                                double* w_out,            // [njobs][2] re, im
+                               hipStream_t st);
+
+// ---- DF construction (df_kernels.hip) -----------------------------------------------------
+// out[i * nvir + a][P] = sum_{mu,nu} c_occ[mu][i] j3c[mu][nu][P] c_vir[nu][a];
+// tmp: nocc * nao * naux doubles.
+hipError_t launch_mo_transform(const double* j3c, const double* c_occ, const double* c_vir,
+                               int nao, int nocc, int nvir, int naux, double* tmp, double* out,
                                hipStream_t st);
 
 // ---- Adjoint (adjoint_kernels.hip) -----------------------------------------------------

@@ -357,12 +357,30 @@ __global__ __launch_bounds__(256, 2) void wnm_bf16x3_kernel(
     }
 }
 
-// ---- CD quadrature ------------------------------------------------------------------------
-
 __device__ __forceinline__ double wave_sum(double v) {
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
     return v;
 }
+
+// ---- Exchange self-energy (diagonal) -----------------------------------------------------
+// sigma_x[p] = -sum_i sum_R mi[p][i][R]^2 over the [nmo][nocc][naux] block (the driver's
+// compute_exchange, quasix_core/src/qp/evgw.rs:1167-1193, diagonal).  One workgroup per p
+// streams its contiguous nocc * naux row once (memory-bound).
+__global__ __launch_bounds__(256) void exchange_diag_kernel(const double* __restrict__ mi,
+                                                            long long row, double* __restrict__ sx) {
+    __shared__ double red[4];
+    const int p = blockIdx.x, tid = threadIdx.x;
+    const double* r = mi + (size_t)p * row;
+    double acc = 0.0;
+    for (long long k = tid; k < row; k += 256) acc += r[k] * r[k];
+    acc = wave_sum(acc);
+    if ((tid & 63) == 0) red[tid >> 6] = acc;
+    __syncthreads();
+    if (tid == 0) sx[p] = -((red[0] + red[1]) + (red[2] + red[3]));
+}
+
+// ---- CD quadrature ------------------------------------------------------------------------
+
 
 // I_n(E_q) for nq energies of target t (whole wavefront); returns 1 if any E_q has a
 // residue pole among the target's partners.
@@ -766,6 +784,14 @@ hipError_t launch_newton_cd(const SigmaProblem& p, const NewtonParams& np, const
     if (p.nt <= 0) return hipSuccess;
     hipLaunchKernelGGL(newton_cd_kernel, dim3(p.nt), dim3(64), 0, st, p, np, guess, e_out, z_out,
                        sig_re, sig_im, state, iters);
+    return hipGetLastError();
+}
+
+hipError_t launch_exchange_diag(const double* mi, int nmo, int nocc, int naux, double* sx,
+                                hipStream_t st) {
+    if (nmo <= 0) return hipSuccess;
+    hipLaunchKernelGGL(exchange_diag_kernel, dim3(nmo), dim3(256), 0, st, mi,
+                       (long long)nocc * naux, sx);
     return hipGetLastError();
 }
 

@@ -311,12 +311,14 @@ def evgw(mo_energy, mo_occ, iaP, ijP, abP, chol_v, vxc_dft, max_cycle=12, conv_t
          targets=None, sigma_x=None, sigma_method="cd", device=0):
     """quasix.quasix.evgw on the MI355X path (quasix/src/evgw.rs:22-453).
 
-    Same positional arguments and keyword defaults.  ``sigma_method="cd"`` (default) uses the
-    contour-deformation self-energy of this library; ``"ac"`` uses the reference's own method
-    (Sigma(i nu) + Thiele-Pade, ``qp_solver`` linearized or newton as there, nfreq imaginary
-    points, iw_cutoff 5) and reproduces its iterates.  ``sigma_x`` (per orbital) defaults to vxc_dft, i.e. Sigma_x - V_xc = 0
-    (the convention of quasix/quasix/g0w0.py:284-292).  Only the target orbitals (default: the
-    n_occ_active/n_vir_active window, else HOMO-5..LUMO+5) are updated.
+    Same positional arguments and keyword defaults.  Sigma_x is computed from ijP (the
+    [nmo, nocc, naux] block) on the GPU as the reference does (quasix_core/src/qp/evgw.rs:
+    501-502); ``sigma_x`` (per orbital) overrides it.  ``sigma_method="cd"`` (default) uses
+    the contour-deformation self-energy of this library; ``"ac"`` uses the reference's own
+    method (Sigma(i nu) + Thiele-Pade, ``qp_solver`` linearized or newton as there, nfreq
+    imaginary points, iw_cutoff 5) and reproduces its iterates.  Only the target orbitals are
+    updated (default: all orbitals for "ac", as the reference; for "cd" the
+    n_occ_active/n_vir_active window, else HOMO-5..LUMO+5).
     """
     if qp_solver not in ("linearized", "newton"):
         raise ValueError(f"qp_solver must be 'linearized' or 'newton', got {qp_solver!r}")
@@ -340,7 +342,11 @@ def evgw(mo_energy, mo_occ, iaP, ijP, abP, chol_v, vxc_dft, max_cycle=12, conv_t
         targets = frontier_targets(nocc, nmo, no, nv)
     tg = np.asarray(targets, dtype=np.int32)
     vxc = np.asarray(vxc_dft, dtype=np.float64)
-    sx = vxc if sigma_x is None else np.asarray(sigma_x, dtype=np.float64)
+    if sigma_x is None:
+        from .core import compute_exchange_diag
+        sx = compute_exchange_diag(mi, nmo, nocc, device=device)
+    else:
+        sx = np.asarray(sigma_x, dtype=np.float64)
     static = (sx - vxc)[tg]
     B = symmetrized_ov(ia, mi, nocc, nvir, naux, device)
     pairs = pairs_from_blocks(ia, mi, ab, nocc, nvir, naux, tg, device)

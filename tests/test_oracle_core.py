@@ -235,3 +235,34 @@ def test_eta_sign_convention():
     r2 = O.g0w0_cd(e, B, 2, targets, fr, wt, eta=0.0)
     assert r2["sigma"][0].imag == 0.0
     assert np.sign(r["sigma"][0].imag) == -1.0
+
+
+def test_exchange_diag_rank_one_closed_form():
+    """mi[p, i, R] = a_p b_i c_R gives Sigma_x[p] = -a_p^2 |b|^2 |c|^2 (only the nocc
+    occupied columns enter, each auxiliary index once)."""
+    rng = np.random.default_rng(12)
+    nmo, nocc, naux = 9, 3, 7
+    a, b, c = rng.standard_normal(nmo), rng.standard_normal(nocc), rng.standard_normal(naux)
+    mi = a[:, None, None] * b[None, :, None] * c[None, None, :]
+    got = O.exchange_diag(mi.reshape(nmo * nocc, naux), nmo, nocc)
+    assert np.allclose(got, -a ** 2 * (b @ b) * (c @ c), rtol=1e-13)
+    assert np.all(got <= 0.0)
+
+
+def test_mo_transform_identity_and_norm():
+    """Identity coefficients pick AO pairs (i, nocc + a); with an orthogonal C = [C_occ C_vir]
+    of the full space and a J symmetric in its AO pair, the sum over all (i, a), (i, j), (a, b)
+    blocks keeps |J|_F -- checked here through the ov block plus its complement."""
+    rng = np.random.default_rng(14)
+    nao, nocc, naux = 9, 3, 5
+    nvir = nao - nocc
+    J = rng.standard_normal((nao, nao, naux))
+    J = 0.5 * (J + J.transpose(1, 0, 2))
+    eye = np.eye(nao)
+    got = O.transform_mo_3center(J, eye[:, :nocc], eye[:, nocc:])
+    assert np.array_equal(got.reshape(nocc, nvir, naux), J[:nocc, nocc:, :])
+    Q = np.linalg.qr(rng.standard_normal((nao, nao)))[0]
+    full = np.einsum("mp,mnP,nq->pqP", Q, J, Q)
+    ov = O.transform_mo_3center(J, Q[:, :nocc], Q[:, nocc:]).reshape(nocc, nvir, naux)
+    assert np.allclose(ov, full[:nocc, nocc:])
+    assert np.isclose(np.sum(full ** 2), np.sum(J ** 2))
