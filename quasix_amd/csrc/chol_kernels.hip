@@ -209,7 +209,11 @@ constexpr int GK = 32;
 __device__ __forceinline__ int swz_rk(int row, int k) { return row * GK + (k ^ (4 * (row & 7))); }
 __device__ __forceinline__ int swz_kc(int k, int c) { return k * NB + (c ^ (16 * (k & 1))); }
 
-// As, Bs: NB * GK doubles of LDS each.
+// As, Bs: NB * GK doubles of LDS each.  Chol: Cholesky tile (NT product) or inverse sum (NN);
+// the choice is uniform per workgroup, so each instance carries only its own addressing.
+// Staging uses 16-byte loads: a thread moves 4 pairs of consecutive doubles per operand,
+// and the swizzles keep each pair adjacent and 16-byte aligned in LDS.
+template <bool Chol>
 __device__ __forceinline__ void colupdate_body(double* __restrict__ a, double* __restrict__ x,
                                                int npad, int j, int bx, int f, double* As,
                                                double* Bs) {
@@ -217,37 +221,41 @@ __device__ __forceinline__ void colupdate_body(double* __restrict__ a, double* _
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     double* Af = a + (size_t)f * npad * npad;
     double* Xf = x + (size_t)f * npad * npad;
-    const bool chol = bx < nblk - j;
-    const int i = chol ? j + bx : j;                 // output block row
-    const int k = chol ? j : bx - (nblk - j);        // output block column
-    const int kbeg = chol ? 0 : k * NB, kend = j * NB;
+    const int i = Chol ? j + bx : j;                 // output block row
+    const int k = Chol ? j : bx - (nblk - j);        // output block column
+    const int kbeg = Chol ? 0 : k * NB, kend = j * NB;
     // A operand rows: L[i-block][kbeg:kend]; B operand: NT -> L[j-block][kbeg:kend] (rows),
     // NN -> X[kbeg:kend][k-block].
     const double* arow = Af + (size_t)i * NB * npad;
-    const double* brow = chol ? Af + (size_t)j * NB * npad : Xf + (size_t)k * NB;
-    // staging: 64 rows x 32 k per operand = 2048 doubles, 8 per thread
-    double ra[8], rb[8];
+    const double* brow = Chol ? Af + (size_t)j * NB * npad : Xf + (size_t)k * NB;
+    // [row][k] staging: pair p = tid + 256 q -> row p / 16, k 2 (p % 16)
+    // [k][col] staging (NN B): pair p -> k p / 32, col 2 (p % 32)
+    const int rr0 = tid >> 4, kk2 = (tid & 15) * 2;
+    const int kr0 = tid >> 5, cc2 = (tid & 31) * 2;
+    double ax[4], ay[4], bxv[4], byv[4];     // pairs held as scalars (kept in registers)
     auto fetch = [&](int k0) {
         #pragma unroll
-        for (int q = 0; q < 8; ++q) {
-            const int idx = tid + 256 * q;
-            const int rr = idx / GK, kk = idx % GK;      // rows of A / NT-B
-            ra[q] = arow[(size_t)rr * npad + k0 + kk];
-            if (chol) rb[q] = brow[(size_t)rr * npad + k0 + kk];
-            else {
-                const int kr = idx / NB, cc = idx % NB;  // NN-B: row k0+kr, column cc
-                rb[q] = brow[(size_t)(k0 + kr) * npad + cc];
-            }
+        for (int q = 0; q < 4; ++q) {
+            const double2 va =
+                *reinterpret_cast<const double2*>(arow + (size_t)(rr0 + 16 * q) * npad + k0 + kk2);
+            const double2 vb =
+                Chol ? *reinterpret_cast<const double2*>(brow + (size_t)(rr0 + 16 * q) * npad + k0 + kk2)
+                     : *reinterpret_cast<const double2*>(brow + (size_t)(k0 + kr0 + 8 * q) * npad + cc2);
+            ax[q] = va.x;
+            ay[q] = va.y;
+            bxv[q] = vb.x;
+            byv[q] = vb.y;
         }
     };
     d4 acc[4] = {d4{0, 0, 0, 0}, d4{0, 0, 0, 0}, d4{0, 0, 0, 0}, d4{0, 0, 0, 0}};
     if (kbeg < kend) fetch(kbeg);
     for (int k0 = kbeg; k0 < kend; k0 += GK) {
         #pragma unroll
-        for (int q = 0; q < 8; ++q) {
-            const int idx = tid + 256 * q;
-            As[swz_rk(idx / GK, idx % GK)] = ra[q];
-            Bs[chol ? swz_rk(idx / GK, idx % GK) : swz_kc(idx / NB, idx % NB)] = rb[q];
+        for (int q = 0; q < 4; ++q) {
+            *reinterpret_cast<double2*>(As + swz_rk(rr0 + 16 * q, kk2)) = double2{ax[q], ay[q]};
+            *reinterpret_cast<double2*>(Bs + (Chol ? swz_rk(rr0 + 16 * q, kk2)
+                                                   : swz_kc(kr0 + 8 * q, cc2))) =
+                double2{bxv[q], byv[q]};
         }
         __syncthreads();
         if (k0 + GK < kend) fetch(k0 + GK);
@@ -259,13 +267,13 @@ __device__ __forceinline__ void colupdate_body(double* __restrict__ a, double* _
             #pragma unroll
             for (int bj = 0; bj < 4; ++bj) {
                 const int c = bj * 16 + (lane & 15);
-                const double bv = Bs[chol ? swz_rk(c, q) : swz_kc(q, c)];
+                const double bv = Bs[Chol ? swz_rk(c, q) : swz_kc(q, c)];
                 acc[bj] = mfma64(av, bv, acc[bj]);
             }
         }
         __syncthreads();
     }
-    if (chol) {
+    if (Chol) {
         if (j == 0) return;
         double* C = Af + (size_t)i * NB * npad + (size_t)j * NB;
         for_acc(acc, wave, lane, [&](int r, int c, double v) { C[(size_t)r * npad + c] -= v; });
@@ -275,27 +283,33 @@ __device__ __forceinline__ void colupdate_body(double* __restrict__ a, double* _
     }
 }
 
-__global__ __launch_bounds__(256) void potrf_colupdate_kernel(double* __restrict__ a,
+__device__ __forceinline__ void colupdate_tile(double* a, double* x, int npad, int j, int bx,
+                                               int f, double* As, double* Bs) {
+    if (bx < npad / NB - j) colupdate_body<true>(a, x, npad, j, bx, f, As, Bs);
+    else colupdate_body<false>(a, x, npad, j, bx, f, As, Bs);
+}
+
+__global__ __launch_bounds__(256, 3) void potrf_colupdate_kernel(double* __restrict__ a,
                                                               double* __restrict__ x, int npad,
                                                               int j) {
-    __shared__ double As[NB * GK];
-    __shared__ double Bs[NB * GK];
-    colupdate_body(a, x, npad, j, blockIdx.x, blockIdx.y, As, Bs);
+    __shared__ __attribute__((aligned(16))) double As[NB * GK];
+    __shared__ __attribute__((aligned(16))) double Bs[NB * GK];
+    colupdate_tile(a, x, npad, j, blockIdx.x, blockIdx.y, As, Bs);
 }
 
 // ---- Grouped forms: one launch per step covers every active instance of a batch --------
 // Instances are sorted by block count, descending, so at step j the active ones (nblk > j)
 // are a prefix of the table.  off[q] is the first work item of instance q in a launch
 // whose per-instance grid is (items per frequency) x nf.
-__global__ __launch_bounds__(256) void potrf_colupdate_group_kernel(
+__global__ __launch_bounds__(256, 3) void potrf_colupdate_group_kernel(
     const PotrfInst* __restrict__ inst, const int* __restrict__ off, int nact, int j) {
-    __shared__ double As[NB * GK];
-    __shared__ double Bs[NB * GK];
+    __shared__ __attribute__((aligned(16))) double As[NB * GK];
+    __shared__ __attribute__((aligned(16))) double Bs[NB * GK];
     const int w = blockIdx.x;
     const int q = find_inst(off, nact, w);
     const PotrfInst in = inst[q];
     const int nblk = in.npad / NB, local = w - off[q];
-    colupdate_body(in.a, in.x, in.npad, j, local % nblk, local / nblk, As, Bs);
+    colupdate_tile(in.a, in.x, in.npad, j, local % nblk, local / nblk, As, Bs);
 }
 
 __global__ __launch_bounds__(64) void potrf_diag_group_kernel(const PotrfInst* __restrict__ inst,

@@ -130,7 +130,12 @@ __device__ __forceinline__ bool block_live(int I0, int J0, int wr, int wc, int b
 }
 
 // One 128x128 lower-triangle tile (I0, J0) of frequency f.  As/Bs: KC x (TILE + 2) in LDS.
-template <typename T, int KC>
+// Every 16x16 block of every wave of the tile lies in the lower triangle and inside naux.
+__device__ __forceinline__ bool tile_full(int I0, int J0, int naux) {
+    return I0 != J0 && I0 + TILE <= naux && J0 + TILE <= naux;
+}
+
+template <typename T, int KC, bool Guarded>
 __device__ __forceinline__ void pi_tile_body(const double* __restrict__ B,
                                              const double* __restrict__ s, int nov, int naux,
                                              int npad, int sld, double* __restrict__ out,
@@ -144,7 +149,7 @@ __device__ __forceinline__ void pi_tile_body(const double* __restrict__ B,
     // Staging: thread owns column c of the tile and rows kr0 + 2q of each chunk.
     constexpr int PER = KC * TILE / 256;
     const int c = tid % TILE, kr0 = tid / TILE;
-    const bool ca = I0 + c < naux, cb = J0 + c < naux;
+    const bool ca = !Guarded || I0 + c < naux, cb = !Guarded || J0 + c < naux;
     double ra[PER], rb[PER];
     auto fetch = [&](int r0) {
         #pragma unroll
@@ -189,7 +194,7 @@ __device__ __forceinline__ void pi_tile_body(const double* __restrict__ B,
             for (int bi = 0; bi < 4; ++bi)
                 #pragma unroll
                 for (int bj = 0; bj < 4; ++bj)
-                    if (block_live(I0, J0, wr, wc, bi, bj, naux))
+                    if (!Guarded || block_live(I0, J0, wr, wc, bi, bj, naux))
                         acc[bi][bj] = Mfma<T>::run(av[bi], bv[bj], acc[bi][bj]);
         }
         __syncthreads();
@@ -207,7 +212,12 @@ __global__ __launch_bounds__(256, 2) void pi_syrk_kernel(
     const int tile = work / nf, f = work % nf;
     int I, J;
     tri_index(tile, I, J);
-    pi_tile_body<T, KC>(B, s, nov, naux, npad, sld, out, mode, I * TILE, J * TILE, f, As, Bs);
+    if (tile_full(I * TILE, J * TILE, naux))
+        pi_tile_body<T, KC, false>(B, s, nov, naux, npad, sld, out, mode, I * TILE, J * TILE, f,
+                                   As, Bs);
+    else
+        pi_tile_body<T, KC, true>(B, s, nov, naux, npad, sld, out, mode, I * TILE, J * TILE, f,
+                                  As, Bs);
 }
 
 // ---- BF16x3 kernel ----------------------------------------------------------------------
@@ -216,8 +226,11 @@ __global__ __launch_bounds__(256, 2) void pi_syrk_kernel(
 typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 
 // The three split products over a wave's 4x4 blocks, product-outer so that 16 independent
-// MFMAs separate dependent accumulations.  Blocks outside the triangle/naux are skipped
-// (the wave index is scalar, so the skips are scalar branches, not exec masking).
+// MFMAs separate dependent accumulations.  Guarded: blocks outside the triangle/naux are
+// skipped (scalar branches; the wave index is scalar).  Tiles whose blocks are all live take
+// the unguarded instance, decided once per tile outside the K loop, so their MFMAs form one
+// straight-line block.
+template <bool Guarded>
 __device__ __forceinline__ void mfma_bf16x3(f4 (&acc)[4][4], const bf16x8 (&ah)[4],
                                             const bf16x8 (&al)[4], const bf16x8 (&ch)[4],
                                             const bf16x8 (&cl)[4], const bool (&live)[4][4]) {
@@ -225,19 +238,19 @@ __device__ __forceinline__ void mfma_bf16x3(f4 (&acc)[4][4], const bf16x8 (&ah)[
     for (int bi = 0; bi < 4; ++bi)
         #pragma unroll
         for (int bj = 0; bj < 4; ++bj)
-            if (live[bi][bj])
+            if (!Guarded || live[bi][bj])
                 acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[bi], ch[bj], acc[bi][bj], 0, 0, 0);
     #pragma unroll
     for (int bi = 0; bi < 4; ++bi)
         #pragma unroll
         for (int bj = 0; bj < 4; ++bj)
-            if (live[bi][bj])
+            if (!Guarded || live[bi][bj])
                 acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[bi], cl[bj], acc[bi][bj], 0, 0, 0);
     #pragma unroll
     for (int bi = 0; bi < 4; ++bi)
         #pragma unroll
         for (int bj = 0; bj < 4; ++bj)
-            if (live[bi][bj])
+            if (!Guarded || live[bi][bj])
                 acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[bi], ch[bj], acc[bi][bj], 0, 0, 0);
 }
 
@@ -271,7 +284,7 @@ __global__ void pi_prepare_mixed_kernel(const double* __restrict__ B, int nov, i
 // KB: N_ov rows per LDS chunk (32 or 64).  Staging map: each thread owns R = KB/16 tile
 // rows and 8 consecutive k of each; a row's KB values are read by KB/8 consecutive threads
 // (coalesced), and a thread needs only its 8 frequency weights per chunk.
-template <int KB>
+template <int KB, bool Guarded>
 __device__ __forceinline__ void pi_tile_bf16x3_body(
     const uint16_t* __restrict__ bt_hi, const uint16_t* __restrict__ bt_lo,
     const float* __restrict__ bt_f32, const float* __restrict__ s32, int kpad, int naux,
@@ -292,13 +305,14 @@ __device__ __forceinline__ void pi_tile_bf16x3_body(
         #pragma unroll
         for (int q = 0; q < R; ++q) {
             const int row = row0 + q * RSTEP;
-            const bool va = I0 + row < npad, vc = J0 + row < npad;
+            // operand rows are allocated and zero-filled up to a multiple of 256 >= the
+            // tile grid, so tile rows past npad load zeros without a bounds select
             const size_t oa = (size_t)(I0 + row) * kpad + k0 + kq0;
             const size_t oc = (size_t)(J0 + row) * kpad + k0 + kq0;
-            rh[q] = va ? *reinterpret_cast<const u16x8*>(bt_hi + oa) : u16x8{0, 0, 0, 0, 0, 0, 0, 0};
-            rl[q] = va ? *reinterpret_cast<const u16x8*>(bt_lo + oa) : u16x8{0, 0, 0, 0, 0, 0, 0, 0};
-            rc[2 * q] = vc ? *reinterpret_cast<const f4*>(bt_f32 + oc) : f4{0.f, 0.f, 0.f, 0.f};
-            rc[2 * q + 1] = vc ? *reinterpret_cast<const f4*>(bt_f32 + oc + 4) : f4{0.f, 0.f, 0.f, 0.f};
+            rh[q] = *reinterpret_cast<const u16x8*>(bt_hi + oa);
+            rl[q] = *reinterpret_cast<const u16x8*>(bt_lo + oa);
+            rc[2 * q] = *reinterpret_cast<const f4*>(bt_f32 + oc);
+            rc[2 * q + 1] = *reinterpret_cast<const f4*>(bt_f32 + oc + 4);
         }
         rs[0] = *reinterpret_cast<const f4*>(sf + k0 + kq0);
         rs[1] = *reinterpret_cast<const f4*>(sf + k0 + kq0 + 4);
@@ -346,7 +360,7 @@ __device__ __forceinline__ void pi_tile_bf16x3_body(
                 cl[q] = *reinterpret_cast<const bf16x8*>(&Cl[wc * 64 + q * 16 + r16][kq]);
             }
             // product-outer order: 16 independent MFMAs between dependent accumulations
-            mfma_bf16x3(acc, ah, al, ch, cl, live);
+            mfma_bf16x3<Guarded>(acc, ah, al, ch, cl, live);
         }
         __syncthreads();
     }
@@ -366,6 +380,7 @@ __device__ __forceinline__ int db_off(int buf, int arr, int row, int chunk) {
 }
 
 // L: 2 * 4 * TILE * DKB bf16 of LDS (16-byte aligned).
+template <bool Guarded>
 __device__ __forceinline__ void pi_tile_bf16x3_db_body(
     const uint16_t* __restrict__ bt_hi, const uint16_t* __restrict__ bt_lo,
     const float* __restrict__ bt_f32, const float* __restrict__ s32, int kpad, int naux,
@@ -382,13 +397,14 @@ __device__ __forceinline__ void pi_tile_bf16x3_db_body(
         #pragma unroll
         for (int q = 0; q < 2; ++q) {
             const int row = row0 + q * 64;
-            const bool va = I0 + row < npad, vc = J0 + row < npad;
+            // operand rows are allocated and zero-filled up to a multiple of 256 >= the
+            // tile grid, so tile rows past npad load zeros without a bounds select
             const size_t oa = (size_t)(I0 + row) * kpad + k0 + kq0;
             const size_t oc = (size_t)(J0 + row) * kpad + k0 + kq0;
-            rh[q] = va ? *reinterpret_cast<const u16x8*>(bt_hi + oa) : u16x8{0, 0, 0, 0, 0, 0, 0, 0};
-            rl[q] = va ? *reinterpret_cast<const u16x8*>(bt_lo + oa) : u16x8{0, 0, 0, 0, 0, 0, 0, 0};
-            rc[2 * q] = vc ? *reinterpret_cast<const f4*>(bt_f32 + oc) : f4{0.f, 0.f, 0.f, 0.f};
-            rc[2 * q + 1] = vc ? *reinterpret_cast<const f4*>(bt_f32 + oc + 4) : f4{0.f, 0.f, 0.f, 0.f};
+            rh[q] = *reinterpret_cast<const u16x8*>(bt_hi + oa);
+            rl[q] = *reinterpret_cast<const u16x8*>(bt_lo + oa);
+            rc[2 * q] = *reinterpret_cast<const f4*>(bt_f32 + oc);
+            rc[2 * q + 1] = *reinterpret_cast<const f4*>(bt_f32 + oc + 4);
         }
         rs[0] = *reinterpret_cast<const f4*>(sf + k0 + kq0);
         rs[1] = *reinterpret_cast<const f4*>(sf + k0 + kq0 + 4);
@@ -437,7 +453,7 @@ __device__ __forceinline__ void pi_tile_bf16x3_db_body(
             ch[q] = *reinterpret_cast<const bf16x8*>(L + db_off(b, 2, wc * 64 + q * 16 + r16, kc));
             cl[q] = *reinterpret_cast<const bf16x8*>(L + db_off(b, 3, wc * 64 + q * 16 + r16, kc));
         }
-        mfma_bf16x3(acc, ah, al, ch, cl, live);
+        mfma_bf16x3<Guarded>(acc, ah, al, ch, cl, live);
         if (i + 1 < nch) stage(b ^ 1);
         if (i + 2 < nch) fetch((i + 2) * DKB);
         __syncthreads();
@@ -454,8 +470,12 @@ __global__ __launch_bounds__(256, 2) void pi_syrk_bf16x3_db_kernel(
     const int tile = work / nf, f = work % nf;
     int I, J;
     tri_index(tile, I, J);
-    pi_tile_bf16x3_db_body(bt_hi, bt_lo, bt_f32, s32, kpad, naux, npad, out, mode, I * TILE,
-                           J * TILE, f, L);
+    if (tile_full(I * TILE, J * TILE, naux))
+        pi_tile_bf16x3_db_body<false>(bt_hi, bt_lo, bt_f32, s32, kpad, naux, npad, out, mode,
+                                      I * TILE, J * TILE, f, L);
+    else
+        pi_tile_bf16x3_db_body<true>(bt_hi, bt_lo, bt_f32, s32, kpad, naux, npad, out, mode,
+                                     I * TILE, J * TILE, f, L);
 }
 
 template <int KB>
@@ -472,8 +492,12 @@ __global__ __launch_bounds__(256, 2) void pi_syrk_bf16x3_kernel(
     const int tile = work / nf, f = work % nf;
     int I, J;
     tri_index(tile, I, J);
-    pi_tile_bf16x3_body<KB>(bt_hi, bt_lo, bt_f32, s32, kpad, naux, npad, out, mode, I * TILE,
-                            J * TILE, f, Ah, Al, Ch, Cl);
+    if (tile_full(I * TILE, J * TILE, naux))
+        pi_tile_bf16x3_body<KB, false>(bt_hi, bt_lo, bt_f32, s32, kpad, naux, npad, out, mode,
+                                       I * TILE, J * TILE, f, Ah, Al, Ch, Cl);
+    else
+        pi_tile_bf16x3_body<KB, true>(bt_hi, bt_lo, bt_f32, s32, kpad, naux, npad, out, mode,
+                                      I * TILE, J * TILE, f, Ah, Al, Ch, Cl);
 }
 
 // ---- Grouped (batched) kernels ------------------------------------------------------------
@@ -511,8 +535,14 @@ __global__ __launch_bounds__(256, 2) void pi_group_bf16x3_kernel(
     PiInst in;
     int I0, J0, f;
     for (int it = 0; group_item(inst, off, ninst, nf, total, it, in, I0, J0, f); ++it)
-        pi_tile_bf16x3_body<KB>(in.bt_hi, in.bt_lo, in.bt_f32, in.s32, in.kpad, in.naux, in.npad,
-                                in.out, PiOut::EpsLower, I0, J0, f, Ah, Al, Ch, Cl);
+        if (tile_full(I0, J0, in.naux))
+            pi_tile_bf16x3_body<KB, false>(in.bt_hi, in.bt_lo, in.bt_f32, in.s32, in.kpad,
+                                           in.naux, in.npad, in.out, PiOut::EpsLower, I0, J0, f,
+                                           Ah, Al, Ch, Cl);
+        else
+            pi_tile_bf16x3_body<KB, true>(in.bt_hi, in.bt_lo, in.bt_f32, in.s32, in.kpad,
+                                          in.naux, in.npad, in.out, PiOut::EpsLower, I0, J0, f,
+                                          Ah, Al, Ch, Cl);
 }
 
 __global__ __launch_bounds__(256, 2) void pi_group_bf16x3_db_kernel(
@@ -521,8 +551,14 @@ __global__ __launch_bounds__(256, 2) void pi_group_bf16x3_db_kernel(
     PiInst in;
     int I0, J0, f;
     for (int it = 0; group_item(inst, off, ninst, nf, total, it, in, I0, J0, f); ++it)
-        pi_tile_bf16x3_db_body(in.bt_hi, in.bt_lo, in.bt_f32, in.s32, in.kpad, in.naux, in.npad,
-                               in.out, PiOut::EpsLower, I0, J0, f, L);
+        if (tile_full(I0, J0, in.naux))
+            pi_tile_bf16x3_db_body<false>(in.bt_hi, in.bt_lo, in.bt_f32, in.s32, in.kpad,
+                                          in.naux, in.npad, in.out, PiOut::EpsLower, I0, J0, f,
+                                          L);
+        else
+            pi_tile_bf16x3_db_body<true>(in.bt_hi, in.bt_lo, in.bt_f32, in.s32, in.kpad,
+                                         in.naux, in.npad, in.out, PiOut::EpsLower, I0, J0, f,
+                                         L);
 }
 
 __global__ __launch_bounds__(256, 2) void pi_group_f64_kernel(
@@ -532,8 +568,12 @@ __global__ __launch_bounds__(256, 2) void pi_group_f64_kernel(
     PiInst in;
     int I0, J0, f;
     for (int it = 0; group_item(inst, off, ninst, nf, total, it, in, I0, J0, f); ++it)
-        pi_tile_body<double, 16>(in.B, in.s, in.nov, in.naux, in.npad, in.kpad, in.out,
-                                 PiOut::EpsLower, I0, J0, f, As, Bs);
+        if (tile_full(I0, J0, in.naux))
+            pi_tile_body<double, 16, false>(in.B, in.s, in.nov, in.naux, in.npad, in.kpad, in.out,
+                                            PiOut::EpsLower, I0, J0, f, As, Bs);
+        else
+            pi_tile_body<double, 16, true>(in.B, in.s, in.nov, in.naux, in.npad, in.kpad, in.out,
+                                           PiOut::EpsLower, I0, J0, f, As, Bs);
 }
 
 // Wide variant: 256 (A, unscaled) x 128 (C = s B, split on the fly) tile per 512-thread
@@ -646,7 +686,7 @@ __global__ __launch_bounds__(512, 1) void pi_syrk_bf16x3_wide_kernel(
                     chv[q] = *reinterpret_cast<const bf16x8*>(&Ch[oc]);
                     clv[q] = *reinterpret_cast<const bf16x8*>(&Cl[oc]);
                 }
-                mfma_bf16x3(acc, ah, al, chv, clv, live);
+                mfma_bf16x3<true>(acc, ah, al, chv, clv, live);
             }
         }
         __syncthreads();
@@ -704,7 +744,8 @@ hipError_t launch_pi_syrk(int precision, const double* B, const PiOperands& ops,
         hipLaunchKernelGGL((pi_syrk_kernel<float, 32>), grid, block, 0, st, B, s, nov, naux,
                            npad, nf, ntiles, sld, out, mode);
     } else {
-        if (!ops.bt_hi || !ops.s32 || ops.kpad != sld) return hipErrorInvalidValue;
+        if (!ops.bt_hi || !ops.s32 || ops.kpad != sld || ops.rows < T * TILE)
+            return hipErrorInvalidValue;
         if (variant == 2) {
             if (ops.rows < round_up(npad > naux ? npad : naux, WTI)) return hipErrorInvalidValue;
             const int TJ = ceil_div(mode == PiOut::EpsLower ? npad : naux, WTJ);

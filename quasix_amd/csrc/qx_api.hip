@@ -434,6 +434,23 @@ WnmResult build_wnm(qx_handle h, int precision, int nocc, int nvir, int naux,
         }
         fill_i32(h, fail, 0, nf);
     }
+    double* x = ws<double>(h, "xinv", (size_t)nf * npad * npad);
+    double* bp = ws<double>(h, "bpairs", (size_t)std::max(npair, 1) * npad);
+    if (npair > 0) {
+        hipLaunchKernelGGL(pad_rows_kernel, grid1d((size_t)npair * npad), dim3(256), 0, st,
+                           B_pair, npair, naux, bp, npad);
+        HIPCK(h, hipGetLastError());
+    }
+    double* wpart = ws<double>(h, "wpart", (size_t)nf * nblk * std::max(npair, 1));
+    WnmMixed mx{};
+    if (precision == QX_PREC_BF16X3 && npair > 0) {
+        const size_t npp = (size_t)round_up(npair, 128);
+        mx.bh = ws<__bf16>(h, "wnm_bh", npp * npad);
+        mx.bl = ws<__bf16>(h, "wnm_bl", npp * npad);
+        mx.mh = ws<__bf16>(h, "wnm_mh", (size_t)nf * npad * npad);
+        mx.ml = ws<__bf16>(h, "wnm_ml", (size_t)nf * npad * npad);
+        HIPCK(h, launch_wnm_split_rows(bp, npair, npad, &mx, st));
+    }
     const int pv = choose_variant(h, "pi", precision, pi_variant_count(precision),
                                   {naux, nov, nf}, [&](int v) {
         HIPCK(h, launch_pi_syrk(precision, B_ov, ops, s, nov, naux, npad, nf, eps_mat,
@@ -445,38 +462,21 @@ WnmResult build_wnm(qx_handle h, int precision, int nocc, int nvir, int naux,
         HIPCK(h, launch_pi_syrk(precision, B_ov, ops, s, nov, naux, npad, nf, eps_mat,
                                 PiOut::EpsLower, st, pv));
     }
-    double* x = ws<double>(h, "xinv", (size_t)nf * npad * npad);
     {
         Phase ph(h, QX_PHASE_POTRF);
         int groups = 1;
         const StreamFan* fan = potrf_fan(h, groups);
         HIPCK(h, launch_potrf_batched(eps_mat, x, linv, npad, nf, fail, st, fan, groups));
     }
+    if (npair > 0) {
+        // Wc_nm before the failure check: on the rare Cholesky failure it is recomputed below
+        Phase ph(h, QX_PHASE_WNM);
+        HIPCK(h, launch_wnm(precision, bp, x, npair, npad, nf, wpart, wnm_out, &mx, st, false));
+    }
     std::vector<int> fail_h(nf);
     d2h(h, fail_h.data(), fail, nf);
     for (int f = 0; f < nf; ++f) r.chol_fallbacks += fail_h[f] != 0;
-
-    double* bp = ws<double>(h, "bpairs", (size_t)std::max(npair, 1) * npad);
-    if (npair > 0) {
-        hipLaunchKernelGGL(pad_rows_kernel, grid1d((size_t)npair * npad), dim3(256), 0, st,
-                           B_pair, npair, naux, bp, npad);
-        HIPCK(h, hipGetLastError());
-    }
-    double* wpart = ws<double>(h, "wpart", (size_t)nf * nblk * std::max(npair, 1));
-    if (r.chol_fallbacks == 0) {
-        if (npair > 0) {
-            WnmMixed mx{};
-            if (precision == QX_PREC_BF16X3) {
-                const size_t npp = (size_t)round_up(npair, 128);
-                mx.bh = ws<__bf16>(h, "wnm_bh", npp * npad);
-                mx.bl = ws<__bf16>(h, "wnm_bl", npp * npad);
-                mx.mh = ws<__bf16>(h, "wnm_mh", (size_t)nf * npad * npad);
-                mx.ml = ws<__bf16>(h, "wnm_ml", (size_t)nf * npad * npad);
-            }
-            Phase ph(h, QX_PHASE_WNM);
-            HIPCK(h, launch_wnm(precision, bp, x, npair, npad, nf, wpart, wnm_out, &mx, st));
-        }
-    } else {
+    if (r.chol_fallbacks > 0) {
         // Reference chain: Cholesky failed somewhere -> LU inverse (here Gauss-Jordan with
         // partial pivoting on every frequency) -> identity if singular.
         double* pi = ws<double>(h, "xinv", (size_t)nf * npad * npad);

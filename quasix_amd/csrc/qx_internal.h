@@ -146,9 +146,12 @@ struct WnmMixed {
 };
 // wnm[f][j] = 2 b_j.(M b_j) + |M b_j|^2,  M = X - I.  precision 2 runs the product on BF16
 // MFMA (hi/lo split, FP32 accumulation), otherwise FP64 MFMA.
+// split_rows = false reuses bh/bl from an earlier launch_wnm_split_rows.
 hipError_t launch_wnm(int precision, const double* bpairs, const double* x, int npair,
                       int npad, int nf, double* wpart, double* wnm, const WnmMixed* mixed,
-                      hipStream_t st);
+                      hipStream_t st, bool split_rows = true);
+hipError_t launch_wnm_split_rows(const double* bpairs, int npair, int npad,
+                                 const WnmMixed* mixed, hipStream_t st);
 // wnm from an explicit Wc: wnm[f][j] = b_j^T Wc_f b_j  (FP64)
 hipError_t launch_wnm_from_wc(const double* bpairs, const double* wc, int npair, int npad,
                               int nf, double* wpart, double* wnm, hipStream_t st);

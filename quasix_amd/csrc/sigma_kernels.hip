@@ -724,14 +724,24 @@ __global__ __launch_bounds__(1024) void residue_solve_kernel(int naux,
 
 }  // namespace
 
+hipError_t launch_wnm_split_rows(const double* bpairs, int npair, int npad,
+                                 const WnmMixed* mixed, hipStream_t st) {
+    if (!mixed) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(split_rows_kernel, dim3(2048), dim3(256), 0, st, bpairs, npair,
+                       round_up(npair, WT), npad, mixed->bh, mixed->bl);
+    return hipGetLastError();
+}
+
 hipError_t launch_wnm(int precision, const double* bpairs, const double* x, int npair, int npad,
-                      int nf, double* wpart, double* wnm, const WnmMixed* mixed, hipStream_t st) {
+                      int nf, double* wpart, double* wnm, const WnmMixed* mixed, hipStream_t st,
+                      bool split_rows) {
     const int nblk = npad / NB;
     if (precision == 2) {
         if (!mixed) return hipErrorInvalidValue;
         const int npp = round_up(npair, WT);
-        hipLaunchKernelGGL(split_rows_kernel, dim3(2048), dim3(256), 0, st, bpairs, npair, npp,
-                           npad, mixed->bh, mixed->bl);
+        if (split_rows)
+            hipLaunchKernelGGL(split_rows_kernel, dim3(2048), dim3(256), 0, st, bpairs, npair, npp,
+                               npad, mixed->bh, mixed->bl);
         hipLaunchKernelGGL(split_m_kernel, dim3(8192), dim3(256), 0, st, x, npad, nf, mixed->mh,
                            mixed->ml);
         dim3 grid(npp / WT, ceil_div(npad, WT), nf);
