@@ -367,36 +367,41 @@ __device__ __forceinline__ void pi_tile_bf16x3_body(
     store_tile<float, false>(acc, I0, J0, wr, wc, lane, npad, naux, f, mode, out, 1.0);
 }
 
-// Double-buffered variant: K chunks of 32, two LDS buffers (64 KB, two workgroups per CU),
-// one barrier per chunk.  While the MFMAs of chunk i read buffer i % 2, the same waves split
-// chunk i + 1 (already in registers) into the other buffer and issue the global loads of
-// chunk i + 2, so the VALU split work and the load latency overlap the matrix work.  Rows of
-// 32 bf16 (4 chunks of 16 B) are unpadded with the chunk index XOR-swizzled by (row / 4) % 4:
-// the 16 rows x 4 chunks that a wave's fragment read or a 64-thread staging write touches
-// cover all 16 bank groups once.
-constexpr int DKB = 32;
+// Double-buffered variants: two LDS buffers, one barrier per chunk.  While the MFMAs of
+// chunk i read buffer i % 2, the same waves split chunk i + 1 (already in registers) into the
+// other buffer and issue the global loads of chunk i + 2, so the split work and the load
+// latency overlap the matrix work.  KB = 32: 64 KB of LDS (two workgroups per CU); KB = 64:
+// 128 KB (one workgroup per CU, two MFMA k-steps per chunk).  Rows of KB bf16 are unpadded
+// with the 16-byte chunk index XOR-swizzled ((row / 4) % 4 for 4 chunks per row, (row / 2) %
+// 8 for 8): the 16 rows x 4 chunks of a fragment read or a 64-thread staging write cover all
+// 16 bank groups once.
+template <int KB>
 __device__ __forceinline__ int db_off(int buf, int arr, int row, int chunk) {
-    return ((buf * 4 + arr) * TILE + row) * DKB + ((chunk ^ ((row >> 2) & 3)) << 3);
+    constexpr int CPR = KB / 8;
+    const int sw = CPR == 4 ? (row >> 2) & 3 : (row >> 1) & 7;
+    return ((buf * 4 + arr) * TILE + row) * KB + ((chunk ^ sw) << 3);
 }
 
-// L: 2 * 4 * TILE * DKB bf16 of LDS (16-byte aligned).
-template <bool Guarded>
+// L: 2 * 4 * TILE * KB bf16 of LDS (16-byte aligned).
+template <int KB, bool Guarded>
 __device__ __forceinline__ void pi_tile_bf16x3_db_body(
     const uint16_t* __restrict__ bt_hi, const uint16_t* __restrict__ bt_lo,
     const float* __restrict__ bt_f32, const float* __restrict__ s32, int kpad, int naux,
     int npad, double* __restrict__ out, PiOut mode, int I0, int J0, int f, __bf16* L) {
+    constexpr int TPR = KB / 8;                  // threads per row (one 16-byte chunk each)
+    constexpr int RSTEP = 256 / TPR;             // row stride between a thread's rows
+    constexpr int R = TILE / RSTEP;              // rows per thread
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);   // wave-uniform (scalar)
     const int wr = wave >> 1, wc = wave & 1;
     const float* sf = s32 + (size_t)f * kpad;
-    // staging: thread owns chunk tid % 4 of rows tid / 4 and tid / 4 + 64
-    const int row0 = tid >> 2, ch0 = tid & 3, kq0 = ch0 * 8;
-    u16x8 rh[2], rl[2];
-    f4 rc[4], rs[2];
+    const int row0 = tid / TPR, ch0 = tid % TPR, kq0 = ch0 * 8;
+    u16x8 rh[R], rl[R];
+    f4 rc[2 * R], rs[2];
     auto fetch = [&](int k0) {
         #pragma unroll
-        for (int q = 0; q < 2; ++q) {
-            const int row = row0 + q * 64;
+        for (int q = 0; q < R; ++q) {
+            const int row = row0 + q * RSTEP;
             // operand rows are allocated and zero-filled up to a multiple of 256 >= the
             // tile grid, so tile rows past npad load zeros without a bounds select
             const size_t oa = (size_t)(I0 + row) * kpad + k0 + kq0;
@@ -411,17 +416,17 @@ __device__ __forceinline__ void pi_tile_bf16x3_db_body(
     };
     auto stage = [&](int buf) {
         #pragma unroll
-        for (int q = 0; q < 2; ++q) {
-            const int row = row0 + q * 64;
-            *reinterpret_cast<u16x8*>(L + db_off(buf, 0, row, ch0)) = rh[q];
-            *reinterpret_cast<u16x8*>(L + db_off(buf, 1, row, ch0)) = rl[q];
+        for (int q = 0; q < R; ++q) {
+            const int row = row0 + q * RSTEP;
+            *reinterpret_cast<u16x8*>(L + db_off<KB>(buf, 0, row, ch0)) = rh[q];
+            *reinterpret_cast<u16x8*>(L + db_off<KB>(buf, 1, row, ch0)) = rl[q];
             #pragma unroll
             for (int hq = 0; hq < 2; ++hq) {
                 const f4 v = rc[2 * q + hq] * rs[hq];
                 const bf16x4 h = __builtin_convertvector(v, bf16x4);
                 const bf16x4 l = __builtin_convertvector(v - __builtin_convertvector(h, f4), bf16x4);
-                *reinterpret_cast<bf16x4*>(L + db_off(buf, 2, row, ch0) + 4 * hq) = h;
-                *reinterpret_cast<bf16x4*>(L + db_off(buf, 3, row, ch0) + 4 * hq) = l;
+                *reinterpret_cast<bf16x4*>(L + db_off<KB>(buf, 2, row, ch0) + 4 * hq) = h;
+                *reinterpret_cast<bf16x4*>(L + db_off<KB>(buf, 3, row, ch0) + 4 * hq) = l;
             }
         }
     };
@@ -437,45 +442,50 @@ __device__ __forceinline__ void pi_tile_bf16x3_db_body(
         #pragma unroll
         for (int bj = 0; bj < 4; ++bj) live[bi][bj] = block_live(I0, J0, wr, wc, bi, bj, naux);
 
-    const int nch = kpad / DKB;
+    const int nch = kpad / KB;
     fetch(0);
     stage(0);
-    if (nch > 1) fetch(DKB);
+    if (nch > 1) fetch(KB);
     __syncthreads();
     const int r16 = lane & 15, kc = lane >> 4;
     for (int i = 0; i < nch; ++i) {
         const int b = i & 1;
-        bf16x8 ah[4], al[4], ch[4], cl[4];
         #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            ah[q] = *reinterpret_cast<const bf16x8*>(L + db_off(b, 0, wr * 64 + q * 16 + r16, kc));
-            al[q] = *reinterpret_cast<const bf16x8*>(L + db_off(b, 1, wr * 64 + q * 16 + r16, kc));
-            ch[q] = *reinterpret_cast<const bf16x8*>(L + db_off(b, 2, wc * 64 + q * 16 + r16, kc));
-            cl[q] = *reinterpret_cast<const bf16x8*>(L + db_off(b, 3, wc * 64 + q * 16 + r16, kc));
+        for (int ks = 0; ks < KB / 32; ++ks) {
+            bf16x8 ah[4], al[4], ch[4], cl[4];
+            #pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int ra = wr * 64 + q * 16 + r16, rcn = wc * 64 + q * 16 + r16;
+                ah[q] = *reinterpret_cast<const bf16x8*>(L + db_off<KB>(b, 0, ra, 4 * ks + kc));
+                al[q] = *reinterpret_cast<const bf16x8*>(L + db_off<KB>(b, 1, ra, 4 * ks + kc));
+                ch[q] = *reinterpret_cast<const bf16x8*>(L + db_off<KB>(b, 2, rcn, 4 * ks + kc));
+                cl[q] = *reinterpret_cast<const bf16x8*>(L + db_off<KB>(b, 3, rcn, 4 * ks + kc));
+            }
+            mfma_bf16x3<Guarded>(acc, ah, al, ch, cl, live);
         }
-        mfma_bf16x3<Guarded>(acc, ah, al, ch, cl, live);
         if (i + 1 < nch) stage(b ^ 1);
-        if (i + 2 < nch) fetch((i + 2) * DKB);
+        if (i + 2 < nch) fetch((i + 2) * KB);
         __syncthreads();
     }
     store_tile<float, false>(acc, I0, J0, wr, wc, lane, npad, naux, f, mode, out, 1.0);
 }
 
-__global__ __launch_bounds__(256, 2) void pi_syrk_bf16x3_db_kernel(
+template <int KB>
+__global__ __launch_bounds__(256, KB == 32 ? 2 : 1) void pi_syrk_bf16x3_db_kernel(
     const uint16_t* __restrict__ bt_hi, const uint16_t* __restrict__ bt_lo,
     const float* __restrict__ bt_f32, const float* __restrict__ s32, int kpad, int naux,
     int npad, int nf, int ntiles, double* __restrict__ out, PiOut mode) {
-    __shared__ __attribute__((aligned(16))) __bf16 L[2 * 4 * TILE * DKB];
+    __shared__ __attribute__((aligned(16))) __bf16 L[2 * 4 * TILE * KB];
     const int work = xcd_remap(blockIdx.x, ntiles * nf);
     const int tile = work / nf, f = work % nf;
     int I, J;
     tri_index(tile, I, J);
     if (tile_full(I * TILE, J * TILE, naux))
-        pi_tile_bf16x3_db_body<false>(bt_hi, bt_lo, bt_f32, s32, kpad, naux, npad, out, mode,
-                                      I * TILE, J * TILE, f, L);
+        pi_tile_bf16x3_db_body<KB, false>(bt_hi, bt_lo, bt_f32, s32, kpad, naux, npad, out, mode,
+                                          I * TILE, J * TILE, f, L);
     else
-        pi_tile_bf16x3_db_body<true>(bt_hi, bt_lo, bt_f32, s32, kpad, naux, npad, out, mode,
-                                     I * TILE, J * TILE, f, L);
+        pi_tile_bf16x3_db_body<KB, true>(bt_hi, bt_lo, bt_f32, s32, kpad, naux, npad, out, mode,
+                                         I * TILE, J * TILE, f, L);
 }
 
 template <int KB>
@@ -547,16 +557,16 @@ __global__ __launch_bounds__(256, 2) void pi_group_bf16x3_kernel(
 
 __global__ __launch_bounds__(256, 2) void pi_group_bf16x3_db_kernel(
     const PiInst* __restrict__ inst, const int* __restrict__ off, int ninst, int nf, int total) {
-    __shared__ __attribute__((aligned(16))) __bf16 L[2 * 4 * TILE * DKB];
+    __shared__ __attribute__((aligned(16))) __bf16 L[2 * 4 * TILE * 32];
     PiInst in;
     int I0, J0, f;
     for (int it = 0; group_item(inst, off, ninst, nf, total, it, in, I0, J0, f); ++it)
         if (tile_full(I0, J0, in.naux))
-            pi_tile_bf16x3_db_body<false>(in.bt_hi, in.bt_lo, in.bt_f32, in.s32, in.kpad,
+            pi_tile_bf16x3_db_body<32, false>(in.bt_hi, in.bt_lo, in.bt_f32, in.s32, in.kpad,
                                           in.naux, in.npad, in.out, PiOut::EpsLower, I0, J0, f,
                                           L);
         else
-            pi_tile_bf16x3_db_body<true>(in.bt_hi, in.bt_lo, in.bt_f32, in.s32, in.kpad,
+            pi_tile_bf16x3_db_body<32, true>(in.bt_hi, in.bt_lo, in.bt_f32, in.s32, in.kpad,
                                          in.naux, in.npad, in.out, PiOut::EpsLower, I0, J0, f,
                                          L);
 }
@@ -756,8 +766,13 @@ hipError_t launch_pi_syrk(int precision, const double* B, const PiOperands& ops,
                                ops.bt_hi, ops.bt_lo, ops.bt_f32, ops.s32, ops.kpad, naux, npad, nf,
                                nt, TJ, out, mode);
         } else if (variant == 3)
-            hipLaunchKernelGGL(pi_syrk_bf16x3_db_kernel, grid, block, 0, st, ops.bt_hi, ops.bt_lo,
-                               ops.bt_f32, ops.s32, ops.kpad, naux, npad, nf, ntiles, out, mode);
+            hipLaunchKernelGGL(pi_syrk_bf16x3_db_kernel<32>, grid, block, 0, st, ops.bt_hi,
+                               ops.bt_lo, ops.bt_f32, ops.s32, ops.kpad, naux, npad, nf, ntiles,
+                               out, mode);
+        else if (variant == 4)
+            hipLaunchKernelGGL(pi_syrk_bf16x3_db_kernel<64>, grid, block, 0, st, ops.bt_hi,
+                               ops.bt_lo, ops.bt_f32, ops.s32, ops.kpad, naux, npad, nf, ntiles,
+                               out, mode);
         else if (variant == 1)
             hipLaunchKernelGGL(pi_syrk_bf16x3_kernel<64>, grid, block, 0, st, ops.bt_hi, ops.bt_lo,
                                ops.bt_f32, ops.s32, ops.kpad, naux, npad, nf, ntiles, out, mode);
@@ -768,7 +783,7 @@ hipError_t launch_pi_syrk(int precision, const double* B, const PiOperands& ops,
     return hipGetLastError();
 }
 
-int pi_variant_count(int precision) { return precision == 2 ? 4 : 1; }
+int pi_variant_count(int precision) { return precision == 2 ? 5 : 1; }
 
 hipError_t launch_pi_group(int precision, const PiInst* inst_dev, const int* off_dev, int ninst,
                            int nf, int total, int grid, hipStream_t st, int variant) {
