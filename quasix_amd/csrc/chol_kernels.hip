@@ -6,7 +6,7 @@
 //   * left-looking blocked Cholesky, block NB = 64, with X = L^-1 formed in the same step
 //     loop.  Step j: one MFMA launch updates block column j of A from all finished columns
 //     (A_ij -= L_i,<j L_j,<j^T) and forms the block-row sums S_jk = L_j,k:j X_k:j,k of the
-//     inverse (long K, pipelined); a one-wavefront kernel factors and inverts A_jj; one
+//     inverse (long K, pipelined); a one-workgroup kernel factors and inverts A_jj; one
 //     launch forms the panel L_ij = A_ij Linv_jj^T and X_jk = -Linv_jj S_jk.  Only block
 //     column j of A and block row j of X are written per step, so the HBM traffic is the
 //     left-looking reads, not a rewrite of the trailing matrix every step;
@@ -74,16 +74,26 @@ __device__ __forceinline__ void for_acc(const d4 (&acc)[4], int wave, int lane, 
             fn(wave * 16 + mfma_row_f64(lane, v), bj * 16 + (lane & 15), acc[bj][v]);
 }
 
-// Diagonal block j: L_jj = chol(A_jj) and Linv_jj, one wavefront per frequency.
-// Both loops over the 64 pivots stay rolled (fully unrolled, the body is ~100 KB of code and
-// runs instruction-fetch bound).  Registers keep static indices by rotation: at pivot k,
-// r[c] holds row `lane`, column k + c of the partially eliminated block; each step applies
-// the rank-1 update and shifts the row down by one slot.  The LDS vectors are stored
-// pre-shifted the same way (entry c = row k + c), so every read has a static, 16-byte
-// aligned offset; slots past row 63 are written as zero.  Factorisation: column k is
-// broadcast through `col` (all lanes read the same addresses).  Inverse: lane c solves
-// L x = e_c by column-oriented forward substitution with the same rotation, reading
-// column k of L from Ls[k][.].
+// Diagonal block j: L_jj = chol(A_jj) and Linv_jj, one 256-thread workgroup per frequency.
+// Thread t owns row i = t / 4 of the block in the factorisation and column t / 4 of the
+// inverse, and within it the 16 entries g + 4 m (g = t % 4, m = 0..15).  Both loops run over
+// 16 groups of four pivots.  Registers keep static indices by rotation: in group kk, slot m
+// holds entry g + 4 (m + kk), so slot 0 of the four threads of a row (a quad) is that row's
+// entries in the group's four columns; it is finished after the group and shifted out.
+// Only the first 16 - kk slots are live, which the four unrolled phases (16, 12, 8, 4
+// slots updated) exploit.
+// Factorisation, blocked right-looking on the lower triangle only, two barriers per group:
+// every thread publishes its slot 0 (the panel of the four columns); every thread factors
+// the 4x4 diagonal block redundantly, solves its own row of the panel against it and
+// records the row in Lp; after the second barrier each thread applies the rank-4 update
+// A_ic -= sum_s L_i,s L_c,s to its live slots with the rows L_c read from Lp.
+// Inverse: the quad of column jc solves L x = e_jc by forward substitution, four pivots per
+// group: the holder of row 4 kk + s scales it by 1 / L_kk and a DPP broadcast hands x to
+// the quad, then all apply the rank-4 update with rows of L from Lp.  No barrier is needed.
+// Lp holds L grouped by four columns, [group][row][4], so a row's four entries are one
+// 32-byte run; the rows past 63 are padding that only dead slots read.
+constexpr int LPR = 80;         // rows per Lp group (64 + padding for dead slots)
+
 // 1/sqrt(d) to full FP64 precision: the hardware estimate (v_rsq_f64) refined by two
 // Newton steps y <- y + y (1/2 - d y^2 / 2), which keeps the pivot's dependency chain to a
 // few FMAs instead of an IEEE sqrt followed by a division.
@@ -95,67 +105,172 @@ __device__ __forceinline__ double rsqrt_nr(double d) {
     return y;
 }
 
-// col: NB doubles of LDS, Ls: NB * NB doubles of LDS (both 16-byte aligned), idg: NB doubles
-// of LDS (1 / L_kk).
+// Broadcast lane S of every quad (DPP quad_perm [S, S, S, S]).
+template <int S>
+__device__ __forceinline__ double quad_bcast(double v) {
+    constexpr int ctrl = S | (S << 2) | (S << 4) | (S << 6);
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const int lo = __builtin_amdgcn_mov_dpp((int)(unsigned)b, ctrl, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(unsigned)(b >> 32), ctrl, 0xf, 0xf, false);
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) |
+                                            (unsigned long long)(unsigned)lo));
+}
+
+__device__ __forceinline__ void rotate16(double (&r)[16]) {
+    #pragma unroll
+    for (int m = 0; m < 15; ++m) r[m] = r[m + 1];
+    r[15] = 0.0;
+}
+
+__device__ __forceinline__ double4 lds4(const double* p) {
+    const double2 a = *reinterpret_cast<const double2*>(p);
+    const double2 b = *reinterpret_cast<const double2*>(p + 2);
+    return double4{a.x, a.y, b.x, b.y};
+}
+
+// Factorisation groups kk in [KK0, KK0 + 4).  P: 64 x 4 panel, Lp: 16 x LPR x 4.
+template <int KK0>
+__device__ __forceinline__ void chol_phase(double (&r)[16], int i, int g, double* P, double* Lp,
+                                           double* idg, bool& bad) {
+    constexpr int NL = 16 - KK0;
+    #pragma unroll 1
+    for (int kk = KK0; kk < KK0 + 4; ++kk) {
+        P[i * 4 + g] = r[0];
+        __syncthreads();
+        double q[4][4];
+        #pragma unroll
+        for (int a = 0; a < 4; ++a) {
+            const double4 v = lds4(P + (4 * kk + a) * 4);
+            q[a][0] = v.x; q[a][1] = v.y; q[a][2] = v.z; q[a][3] = v.w;
+        }
+        const double4 own = lds4(P + i * 4);
+        const double ai[4] = {own.x, own.y, own.z, own.w};
+        // 4x4 diagonal block, in every thread
+        double lb[4][4], isd[4];
+        #pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            double dd = q[s][s];
+            #pragma unroll
+            for (int t = 0; t < s; ++t) dd = fma(-lb[s][t], lb[s][t], dd);
+            if (!(dd > 0.0)) { bad = true; dd = 1.0; }
+            isd[s] = rsqrt_nr(dd);
+            #pragma unroll
+            for (int a = s + 1; a < 4; ++a) {
+                double v = q[a][s];
+                #pragma unroll
+                for (int t = 0; t < s; ++t) v = fma(-lb[a][t], lb[s][t], v);
+                lb[a][s] = v * isd[s];
+            }
+        }
+        // own row of the panel: L_i,4kk+s (zero above the diagonal)
+        double li[4];
+        #pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            double v = ai[s];
+            #pragma unroll
+            for (int t = 0; t < s; ++t) v = fma(-li[t], lb[s][t], v);
+            li[s] = i >= 4 * kk + s ? v * isd[s] : 0.0;
+        }
+        double* L = Lp + kk * LPR * 4;
+        if (g == 0) {
+            *reinterpret_cast<double2*>(L + i * 4) = double2{li[0], li[1]};
+            *reinterpret_cast<double2*>(L + i * 4 + 2) = double2{li[2], li[3]};
+        }
+        if (i == 0) idg[4 * kk + g] = g == 0 ? isd[0] : g == 1 ? isd[1] : g == 2 ? isd[2] : isd[3];
+        __syncthreads();
+        #pragma unroll
+        for (int m = 0; m < NL; ++m) {
+            const double4 lc = lds4(L + (g + 4 * (m + kk)) * 4);
+            double v = r[m];
+            v = fma(-li[0], lc.x, v);
+            v = fma(-li[1], lc.y, v);
+            v = fma(-li[2], lc.z, v);
+            r[m] = fma(-li[3], lc.w, v);
+        }
+        rotate16(r);
+    }
+}
+
+// Inverse groups kk in [KK0, KK0 + 4): column jc, rows g + 4 (m + kk).
+template <int KK0>
+__device__ __forceinline__ void inv_phase(double (&y)[16], int jc, int g, const double* Lp,
+                                          const double* idg, double* X, double* D, int npad) {
+    constexpr int NL = 16 - KK0;
+    #pragma unroll 1
+    for (int kk = KK0; kk < KK0 + 4; ++kk) {
+        const double* L = Lp + kk * LPR * 4;
+        const double4 lo = lds4(L + (g + 4 * kk) * 4);     // own holder row 4 kk + g
+        const double lrow[4] = {lo.x, lo.y, lo.z, lo.w};
+        double y0 = y[0], xs[4];
+        #pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            const double v = y0 * idg[4 * kk + s];
+            xs[s] = s == 0 ? quad_bcast<0>(v) : s == 1 ? quad_bcast<1>(v)
+                  : s == 2 ? quad_bcast<2>(v) : quad_bcast<3>(v);
+            if (g > s) y0 = fma(-lrow[s], xs[s], y0);
+        }
+        const double xg = g == 0 ? xs[0] : g == 1 ? xs[1] : g == 2 ? xs[2] : xs[3];
+        X[(size_t)(4 * kk + g) * npad + jc] = xg;
+        D[(4 * kk + g) * NB + jc] = xg;
+        #pragma unroll
+        for (int m = 1; m < NL; ++m) {
+            const double4 lr = lds4(L + (g + 4 * (m + kk)) * 4);
+            double v = y[m];
+            v = fma(-lr.x, xs[0], v);
+            v = fma(-lr.y, xs[1], v);
+            v = fma(-lr.z, xs[2], v);
+            y[m] = fma(-lr.w, xs[3], v);
+        }
+        rotate16(y);
+    }
+}
+
+// P: 4 * NB doubles, Lp: 16 * LPR * 4 doubles, idg: NB doubles of LDS (16-byte aligned).
 __device__ __forceinline__ void diag_body(double* __restrict__ a, double* __restrict__ x,
                                           double* __restrict__ linv_diag, int npad, int j,
-                                          int* __restrict__ fail, int f, double* col,
-                                          double* Ls, double* idg) {
-    const int lane = threadIdx.x;
+                                          int* __restrict__ fail, int f, double* P, double* Lp,
+                                          double* idg) {
+    const int t = threadIdx.x, i = t >> 2, g = t & 3;
     const int nblk = npad / NB;
     const size_t off = (size_t)f * npad * npad + (size_t)j * NB * npad + (size_t)j * NB;
     double* A = a + off;
     double* X = x + off;
     double* D = linv_diag + ((size_t)f * nblk + j) * NB * NB;
-    // This wave is on the critical path of the step loop and shares its CU with the column
-    // updates of other frequency groups: give it instruction-issue priority.
+    // These waves are on the critical path of the step loop and share their CU with the
+    // column updates of other frequency groups: give them instruction-issue priority.
     __builtin_amdgcn_s_setprio(3);
     bool bad = false;
-    {
-        double r[NB];
-        #pragma unroll
-        for (int c = 0; c < NB; ++c) r[c] = A[(size_t)lane * npad + c];
-        for (int k = 0; k < NB; ++k) {
-            const int slot = lane >= k ? lane - k : lane - k + NB;   // every slot written once
-            col[slot] = lane >= k ? r[0] : 0.0;
-            __syncthreads();
-            double d = col[0];
-            if (!(d > 0.0)) { bad = true; d = 1.0; }
-            const double isd = rsqrt_nr(d), sd = d * isd;
-            if (lane == k) idg[k] = isd;
-            const double l = lane > k ? r[0] * isd : sd;             // L[lane][k] for lane >= k
-            Ls[k * NB + slot] = lane >= k ? l : 0.0;
-            const double t = lane > k ? l * isd : 0.0;
-            #pragma unroll
-            for (int c = 1; c < NB; ++c) r[c - 1] = r[c] - t * col[c];
-            __syncthreads();
-        }
+    double r[16];
+    #pragma unroll
+    for (int m = 0; m < 16; ++m) r[m] = A[(size_t)i * npad + g + 4 * m];
+    chol_phase<0>(r, i, g, P, Lp, idg, bad);
+    chol_phase<4>(r, i, g, P, Lp, idg, bad);
+    chol_phase<8>(r, i, g, P, Lp, idg, bad);
+    chol_phase<12>(r, i, g, P, Lp, idg, bad);
+    __syncthreads();
+    #pragma unroll
+    for (int m = 0; m < 16; ++m) {
+        const int c = g + 4 * m;
+        A[(size_t)i * npad + c] = c <= i ? Lp[(m * LPR + i) * 4 + g] : 0.0;
     }
-    for (int q = 0; q < NB; ++q) A[(size_t)q * npad + lane] = q >= lane ? Ls[lane * NB + q - lane] : 0.0;
-    {
-        double y[NB];
-        #pragma unroll
-        for (int c = 0; c < NB; ++c) y[c] = c == lane ? 1.0 : 0.0;
-        for (int k = 0; k < NB; ++k) {
-            const double* Lk = Ls + k * NB;      // Lk[c] = L[k + c][k]
-            const double yk = y[0] * idg[k];     // Linv[k][lane]
-            X[(size_t)k * npad + lane] = yk;
-            D[k * NB + lane] = yk;
-            #pragma unroll
-            for (int c = 1; c < NB; ++c) y[c - 1] = y[c] - Lk[c] * yk;
-        }
-    }
-    if (__any(bad) && lane == 0) fail[f] = 1;
+    double y[16];
+    #pragma unroll
+    for (int m = 0; m < 16; ++m) y[m] = g + 4 * m == i ? 1.0 : 0.0;
+    inv_phase<0>(y, i, g, Lp, idg, X, D, npad);
+    inv_phase<4>(y, i, g, Lp, idg, X, D, npad);
+    inv_phase<8>(y, i, g, Lp, idg, X, D, npad);
+    inv_phase<12>(y, i, g, Lp, idg, X, D, npad);
+    if (__any(bad) && (t & 63) == 0) fail[f] = 1;
 }
 
-__global__ __launch_bounds__(64) void potrf_diag_kernel(double* __restrict__ a,
-                                                        double* __restrict__ x,
-                                                        double* __restrict__ linv_diag,
-                                                        int npad, int j, int* __restrict__ fail) {
-    __shared__ __attribute__((aligned(16))) double col[NB];
-    __shared__ __attribute__((aligned(16))) double Ls[NB * NB];
+__global__ __launch_bounds__(256) void potrf_diag_kernel(double* __restrict__ a,
+                                                         double* __restrict__ x,
+                                                         double* __restrict__ linv_diag,
+                                                         int npad, int j, int* __restrict__ fail) {
+    __shared__ __attribute__((aligned(16))) double P[4 * NB];
+    __shared__ __attribute__((aligned(16))) double Lp[16 * LPR * 4];
     __shared__ double idg[NB];
-    diag_body(a, x, linv_diag, npad, j, fail, blockIdx.x, col, Ls, idg);
+    diag_body(a, x, linv_diag, npad, j, fail, blockIdx.x, P, Lp, idg);
 }
 
 // After the diagonal factor of step j:
@@ -312,13 +427,13 @@ __global__ __launch_bounds__(256, 3) void potrf_colupdate_group_kernel(
     colupdate_tile(in.a, in.x, in.npad, j, local % nblk, local / nblk, As, Bs);
 }
 
-__global__ __launch_bounds__(64) void potrf_diag_group_kernel(const PotrfInst* __restrict__ inst,
-                                                              int nf, int j) {
-    __shared__ __attribute__((aligned(16))) double col[NB];
-    __shared__ __attribute__((aligned(16))) double Ls[NB * NB];
+__global__ __launch_bounds__(256) void potrf_diag_group_kernel(const PotrfInst* __restrict__ inst,
+                                                               int nf, int j) {
+    __shared__ __attribute__((aligned(16))) double P[4 * NB];
+    __shared__ __attribute__((aligned(16))) double Lp[16 * LPR * 4];
     __shared__ double idg[NB];
     const PotrfInst in = inst[blockIdx.x / nf];
-    diag_body(in.a, in.x, in.linv, in.npad, j, in.fail, blockIdx.x % nf, col, Ls, idg);
+    diag_body(in.a, in.x, in.linv, in.npad, j, in.fail, blockIdx.x % nf, P, Lp, idg);
 }
 
 __global__ __launch_bounds__(256) void potrf_rowpanel_group_kernel(
@@ -473,7 +588,7 @@ hipError_t launch_potrf_batched(double* a, double* x, double* linv_diag, int npa
             if (j > 0)
                 hipLaunchKernelGGL(potrf_colupdate_kernel, dim3(nblk, nfg), dim3(256), 0, sg[g],
                                    ag, xg, npad, j);
-            hipLaunchKernelGGL(potrf_diag_kernel, dim3(nfg), dim3(64), 0, sg[g], ag, xg, dg,
+            hipLaunchKernelGGL(potrf_diag_kernel, dim3(nfg), dim3(256), 0, sg[g], ag, xg, dg,
                                npad, j, fail + f0);
             if (nblk > 1)
                 hipLaunchKernelGGL(potrf_rowpanel_kernel, dim3(nblk - 1, nfg), dim3(256), 0,
@@ -499,7 +614,7 @@ hipError_t launch_potrf_group(const PotrfInst* inst_dev, const int* nblk_sorted,
         if (j > 0 && cu_off_host[nact] > 0)
             hipLaunchKernelGGL(potrf_colupdate_group_kernel, dim3(cu_off_host[nact]), dim3(256), 0,
                                st, inst_dev, cu_off_dev, nact, j);
-        hipLaunchKernelGGL(potrf_diag_group_kernel, dim3(nact * nf), dim3(64), 0, st, inst_dev,
+        hipLaunchKernelGGL(potrf_diag_group_kernel, dim3(nact * nf), dim3(256), 0, st, inst_dev,
                            nf, j);
         if (rp_off_host[nact] > 0)
             hipLaunchKernelGGL(potrf_rowpanel_group_kernel, dim3(rp_off_host[nact]), dim3(256), 0,
