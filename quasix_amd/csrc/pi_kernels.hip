@@ -281,6 +281,17 @@ __global__ void pi_prepare_mixed_kernel(const double* __restrict__ B, int nov, i
     }
 }
 
+// LDS row layout of a KB-wide chunk.  KB = 64: unpadded 128-byte rows with the 16-byte
+// chunk index XOR-swizzled by (row / 2) % 8; measured on gfx950 this streams MFMA fragment
+// reads (16 rows x 4 chunks) at the full LDS rate, where a padded 144-byte pitch reaches
+// about half of it.  KB = 32: rows padded to 40 bf16.
+template <int KB> struct PiLds {
+    static constexpr int pitch = KB == 64 ? 64 : KB + 8;
+    static __device__ __forceinline__ int col(int row, int chunk) {
+        return KB == 64 ? (chunk ^ ((row >> 1) & 7)) << 3 : chunk << 3;
+    }
+};
+
 // KB: N_ov rows per LDS chunk (32 or 64).  Staging map: each thread owns R = KB/16 tile
 // rows and 8 consecutive k of each; a row's KB values are read by KB/8 consecutive threads
 // (coalesced), and a thread needs only its 8 frequency weights per chunk.
@@ -289,7 +300,9 @@ __device__ __forceinline__ void pi_tile_bf16x3_body(
     const uint16_t* __restrict__ bt_hi, const uint16_t* __restrict__ bt_lo,
     const float* __restrict__ bt_f32, const float* __restrict__ s32, int kpad, int naux,
     int npad, double* __restrict__ out, PiOut mode, int I0, int J0, int f,
-    __bf16 (*Ah)[KB + 8], __bf16 (*Al)[KB + 8], __bf16 (*Ch)[KB + 8], __bf16 (*Cl)[KB + 8]) {
+    __bf16 (*Ah)[PiLds<KB>::pitch], __bf16 (*Al)[PiLds<KB>::pitch],
+    __bf16 (*Ch)[PiLds<KB>::pitch], __bf16 (*Cl)[PiLds<KB>::pitch]) {
+    typedef PiLds<KB> Lay;
     constexpr int TPR = KB / 8;                  // threads per row
     constexpr int RSTEP = 256 / TPR;             // row stride between a thread's rows
     constexpr int R = TILE / RSTEP;              // rows per thread
@@ -334,23 +347,26 @@ __device__ __forceinline__ void pi_tile_bf16x3_body(
         #pragma unroll
         for (int q = 0; q < R; ++q) {
             const int row = row0 + q * RSTEP;
-            *reinterpret_cast<u16x8*>(&Ah[row][kq0]) = rh[q];
-            *reinterpret_cast<u16x8*>(&Al[row][kq0]) = rl[q];
+            const int kc = Lay::col(row, kq0 >> 3);
+            *reinterpret_cast<u16x8*>(&Ah[row][kc]) = rh[q];
+            *reinterpret_cast<u16x8*>(&Al[row][kc]) = rl[q];
             // C = s * B split into hi + lo on the fly (packed hardware conversions)
             #pragma unroll
             for (int hq = 0; hq < 2; ++hq) {
                 const f4 v = rc[2 * q + hq] * rs[hq];
                 const bf16x4 h = __builtin_convertvector(v, bf16x4);
                 const bf16x4 l = __builtin_convertvector(v - __builtin_convertvector(h, f4), bf16x4);
-                *reinterpret_cast<bf16x4*>(&Ch[row][kq0 + 4 * hq]) = h;
-                *reinterpret_cast<bf16x4*>(&Cl[row][kq0 + 4 * hq]) = l;
+                *reinterpret_cast<bf16x4*>(&Ch[row][kc + 4 * hq]) = h;
+                *reinterpret_cast<bf16x4*>(&Cl[row][kc + 4 * hq]) = l;
             }
         }
         __syncthreads();
         if (k0 + KB < kpad) fetch(k0 + KB);      // in flight during the MFMA phase
         #pragma unroll
         for (int ks = 0; ks < KB; ks += 32) {
-            const int kq = ks + (lane >> 4) * 8, r16 = lane & 15;
+            const int cq = ks / 8 + (lane >> 4), r16 = lane & 15;
+            // the swizzle of rows wr*64 + q*16 + r16 depends on r16 only
+            const int kq = Lay::col(r16, cq);
             bf16x8 ah[4], al[4], ch[4], cl[4];
             #pragma unroll
             for (int q = 0; q < 4; ++q) {
@@ -493,7 +509,7 @@ __global__ __launch_bounds__(256, 2) void pi_syrk_bf16x3_kernel(
     const uint16_t* __restrict__ bt_hi, const uint16_t* __restrict__ bt_lo,
     const float* __restrict__ bt_f32, const float* __restrict__ s32, int kpad, int naux,
     int npad, int nf, int ntiles, double* __restrict__ out, PiOut mode) {
-    constexpr int KBP = KB + 8;                  // padded LDS pitch: conflict-free rows
+    constexpr int KBP = PiLds<KB>::pitch;
     __shared__ __attribute__((aligned(16))) __bf16 Ah[TILE][KBP];
     __shared__ __attribute__((aligned(16))) __bf16 Al[TILE][KBP];
     __shared__ __attribute__((aligned(16))) __bf16 Ch[TILE][KBP];
@@ -537,7 +553,7 @@ __device__ __forceinline__ bool group_item(const PiInst* __restrict__ inst,
 template <int KB>
 __global__ __launch_bounds__(256, 2) void pi_group_bf16x3_kernel(
     const PiInst* __restrict__ inst, const int* __restrict__ off, int ninst, int nf, int total) {
-    constexpr int KBP = KB + 8;
+    constexpr int KBP = PiLds<KB>::pitch;
     __shared__ __attribute__((aligned(16))) __bf16 Ah[TILE][KBP];
     __shared__ __attribute__((aligned(16))) __bf16 Al[TILE][KBP];
     __shared__ __attribute__((aligned(16))) __bf16 Ch[TILE][KBP];
