@@ -225,6 +225,17 @@ __global__ __launch_bounds__(256, 2) void pi_syrk_kernel(
 // chunk, so each lane fetches its 8 consecutive k values with one 16-byte read.
 typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 
+// hi + lo BF16 split of an FP32 quad (round to nearest even): hi by the hardware
+// conversion, its FP32 value by shifting the packed halves back, lo from the remainder.
+__device__ __forceinline__ void split_bf16x4(const f4 v, bf16x4& h, bf16x4& l) {
+    typedef unsigned int u2 __attribute__((ext_vector_type(2)));
+    h = __builtin_convertvector(v, bf16x4);
+    const u2 hb = __builtin_bit_cast(u2, h);
+    const f4 hf = f4{__uint_as_float(hb.x << 16), __uint_as_float(hb.x & 0xffff0000u),
+                     __uint_as_float(hb.y << 16), __uint_as_float(hb.y & 0xffff0000u)};
+    l = __builtin_convertvector(v - hf, bf16x4);
+}
+
 // The three split products over a wave's 4x4 blocks, product-outer so that 16 independent
 // MFMAs separate dependent accumulations.  Guarded: blocks outside the triangle/naux are
 // skipped (scalar branches; the wave index is scalar).  Tiles whose blocks are all live take
@@ -381,6 +392,176 @@ __device__ __forceinline__ void pi_tile_bf16x3_body(
         __syncthreads();
     }
     store_tile<float, false>(acc, I0, J0, wr, wc, lane, npad, naux, f, mode, out, 1.0);
+}
+
+// ---- 32x32x16 MFMA form ------------------------------------------------------------------
+// A wave's 64x64 block as 2x2 blocks of 32x32: per 16-wide k-step one
+// v_mfma_f32_32x32x16_bf16 per block and product, half the instructions of the 16x16x32
+// form for the same work.  Measured on MI355X, one wave issuing four independent 32x32x16
+// chains runs the matrix pipe near its peak, where 16x16x32 chains need two waves.
+// Fragments: lane l holds row l % 32 and the 8 consecutive k at (l / 32) * 8 of the step.
+// Output: lane l, register v -> row 8 (v / 4) + 4 (l / 32) + v % 4, column l % 32.
+typedef float f16v __attribute__((ext_vector_type(16)));
+
+__device__ __forceinline__ int mfma32_row(int lane, int v) {
+    return 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+}
+
+__device__ __forceinline__ bool block32_live(int I0, int J0, int wr, int wc, int bi, int bj,
+                                             int naux) {
+    const int r = I0 + wr * 64 + bi * 32, c = J0 + wc * 64 + bj * 32;
+    return r < naux && c < naux && (I0 != J0 || r + 31 >= c);
+}
+
+template <bool Guarded>
+__device__ __forceinline__ void mfma32_bf16x3(f16v (&acc)[2][2], const bf16x8 (&ah)[2],
+                                              const bf16x8 (&al)[2], const bf16x8 (&ch)[2],
+                                              const bf16x8 (&cl)[2], const bool (&live)[2][2]) {
+    #pragma unroll
+    for (int bi = 0; bi < 2; ++bi)
+        #pragma unroll
+        for (int bj = 0; bj < 2; ++bj)
+            if (!Guarded || live[bi][bj])
+                acc[bi][bj] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[bi], ch[bj], acc[bi][bj], 0, 0, 0);
+    #pragma unroll
+    for (int bi = 0; bi < 2; ++bi)
+        #pragma unroll
+        for (int bj = 0; bj < 2; ++bj)
+            if (!Guarded || live[bi][bj])
+                acc[bi][bj] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[bi], cl[bj], acc[bi][bj], 0, 0, 0);
+    #pragma unroll
+    for (int bi = 0; bi < 2; ++bi)
+        #pragma unroll
+        for (int bj = 0; bj < 2; ++bj)
+            if (!Guarded || live[bi][bj])
+                acc[bi][bj] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[bi], ch[bj], acc[bi][bj], 0, 0, 0);
+}
+
+// Same output conventions as store_tile (EpsLower: I - Pi over the tile; full: mirrored).
+__device__ __forceinline__ void store_tile32(const f16v (&acc)[2][2], int I0, int J0, int wr,
+                                             int wc, int lane, int npad, int naux, int f,
+                                             PiOut mode, double* __restrict__ out) {
+    #pragma unroll
+    for (int bi = 0; bi < 2; ++bi)
+        #pragma unroll
+        for (int bj = 0; bj < 2; ++bj)
+            #pragma unroll
+            for (int v = 0; v < 16; ++v) {
+                const int row = I0 + wr * 64 + bi * 32 + mfma32_row(lane, v);
+                const int col = J0 + wc * 64 + bj * 32 + (lane & 31);
+                const double p = (double)acc[bi][bj][v];
+                if (mode == PiOut::EpsLower) {
+                    if (row < npad && col < npad)
+                        out[((size_t)f * npad + row) * npad + col] = (row == col ? 1.0 : 0.0) - p;
+                } else if (row < naux && col < naux && (I0 != J0 || row >= col)) {
+                    out[((size_t)f * npad + row) * npad + col] = p;
+                    out[((size_t)f * npad + col) * npad + row] = p;
+                }
+            }
+}
+
+// The tile body of pi_tile_bf16x3_body (same staging and LDS layout) on 32x32x16 MFMAs.
+template <int KB, bool Guarded>
+__device__ __forceinline__ void pi_tile_bf16x3_m32_body(
+    const uint16_t* __restrict__ bt_hi, const uint16_t* __restrict__ bt_lo,
+    const float* __restrict__ bt_f32, const float* __restrict__ s32, int kpad, int naux,
+    int npad, double* __restrict__ out, PiOut mode, int I0, int J0, int f,
+    __bf16 (*Ah)[PiLds<KB>::pitch], __bf16 (*Al)[PiLds<KB>::pitch],
+    __bf16 (*Ch)[PiLds<KB>::pitch], __bf16 (*Cl)[PiLds<KB>::pitch]) {
+    typedef PiLds<KB> Lay;
+    constexpr int TPR = KB / 8, RSTEP = 256 / TPR, R = TILE / RSTEP;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int wr = wave >> 1, wc = wave & 1;
+    const float* sf = s32 + (size_t)f * kpad;
+    const int row0 = tid / TPR, kq0 = (tid % TPR) * 8;
+    u16x8 rh[R], rl[R];
+    f4 rc[2 * R], rs[2];
+    auto fetch = [&](int k0) {
+        #pragma unroll
+        for (int q = 0; q < R; ++q) {
+            const int row = row0 + q * RSTEP;
+            // operand rows are allocated and zero-filled up to a multiple of 256 >= the
+            // tile grid, so tile rows past npad load zeros without a bounds select
+            const size_t oa = (size_t)(I0 + row) * kpad + k0 + kq0;
+            const size_t oc = (size_t)(J0 + row) * kpad + k0 + kq0;
+            rh[q] = *reinterpret_cast<const u16x8*>(bt_hi + oa);
+            rl[q] = *reinterpret_cast<const u16x8*>(bt_lo + oa);
+            rc[2 * q] = *reinterpret_cast<const f4*>(bt_f32 + oc);
+            rc[2 * q + 1] = *reinterpret_cast<const f4*>(bt_f32 + oc + 4);
+        }
+        rs[0] = *reinterpret_cast<const f4*>(sf + k0 + kq0);
+        rs[1] = *reinterpret_cast<const f4*>(sf + k0 + kq0 + 4);
+    };
+    f16v acc[2][2];
+    bool live[2][2];
+    #pragma unroll
+    for (int bi = 0; bi < 2; ++bi)
+        #pragma unroll
+        for (int bj = 0; bj < 2; ++bj) {
+            #pragma unroll
+            for (int v = 0; v < 16; ++v) acc[bi][bj][v] = 0.f;
+            live[bi][bj] = block32_live(I0, J0, wr, wc, bi, bj, naux);
+        }
+    fetch(0);
+    for (int k0 = 0; k0 < kpad; k0 += KB) {
+        #pragma unroll
+        for (int q = 0; q < R; ++q) {
+            const int row = row0 + q * RSTEP;
+            const int kc = Lay::col(row, kq0 >> 3);
+            *reinterpret_cast<u16x8*>(&Ah[row][kc]) = rh[q];
+            *reinterpret_cast<u16x8*>(&Al[row][kc]) = rl[q];
+            #pragma unroll
+            for (int hq = 0; hq < 2; ++hq) {
+                bf16x4 h, l;
+                split_bf16x4(rc[2 * q + hq] * rs[hq], h, l);
+                *reinterpret_cast<bf16x4*>(&Ch[row][kc + 4 * hq]) = h;
+                *reinterpret_cast<bf16x4*>(&Cl[row][kc + 4 * hq]) = l;
+            }
+        }
+        __syncthreads();
+        if (k0 + KB < kpad) fetch(k0 + KB);      // in flight during the MFMA phase
+        #pragma unroll
+        for (int ks = 0; ks < KB; ks += 16) {
+            const int r32 = lane & 31;
+            // rows w*64 + q*32 + r32: the swizzle depends on r32 only
+            const int kq = Lay::col(r32, ks / 8 + (lane >> 5));
+            bf16x8 ah[2], al[2], ch[2], cl[2];
+            #pragma unroll
+            for (int q = 0; q < 2; ++q) {
+                ah[q] = *reinterpret_cast<const bf16x8*>(&Ah[wr * 64 + q * 32 + r32][kq]);
+                al[q] = *reinterpret_cast<const bf16x8*>(&Al[wr * 64 + q * 32 + r32][kq]);
+                ch[q] = *reinterpret_cast<const bf16x8*>(&Ch[wc * 64 + q * 32 + r32][kq]);
+                cl[q] = *reinterpret_cast<const bf16x8*>(&Cl[wc * 64 + q * 32 + r32][kq]);
+            }
+            mfma32_bf16x3<Guarded>(acc, ah, al, ch, cl, live);
+        }
+        __syncthreads();
+    }
+    store_tile32(acc, I0, J0, wr, wc, lane, npad, naux, f, mode, out);
+}
+
+// KB = 32 fits three workgroups per CU (40 KB of LDS, at most 168 VGPRs).
+template <int KB>
+__global__ __launch_bounds__(256, KB == 32 ? 3 : 2) void pi_syrk_bf16x3_m32_kernel(
+    const uint16_t* __restrict__ bt_hi, const uint16_t* __restrict__ bt_lo,
+    const float* __restrict__ bt_f32, const float* __restrict__ s32, int kpad, int naux,
+    int npad, int nf, int ntiles, double* __restrict__ out, PiOut mode) {
+    constexpr int KBP = PiLds<KB>::pitch;
+    __shared__ __attribute__((aligned(16))) __bf16 Ah[TILE][KBP];
+    __shared__ __attribute__((aligned(16))) __bf16 Al[TILE][KBP];
+    __shared__ __attribute__((aligned(16))) __bf16 Ch[TILE][KBP];
+    __shared__ __attribute__((aligned(16))) __bf16 Cl[TILE][KBP];
+    const int work = xcd_remap(blockIdx.x, ntiles * nf);
+    const int tile = work / nf, f = work % nf;
+    int I, J;
+    tri_index(tile, I, J);
+    if (tile_full(I * TILE, J * TILE, naux))
+        pi_tile_bf16x3_m32_body<KB, false>(bt_hi, bt_lo, bt_f32, s32, kpad, naux, npad, out,
+                                           mode, I * TILE, J * TILE, f, Ah, Al, Ch, Cl);
+    else
+        pi_tile_bf16x3_m32_body<KB, true>(bt_hi, bt_lo, bt_f32, s32, kpad, naux, npad, out,
+                                          mode, I * TILE, J * TILE, f, Ah, Al, Ch, Cl);
 }
 
 // Double-buffered variants: two LDS buffers, one barrier per chunk.  While the MFMAs of
@@ -569,6 +750,26 @@ __global__ __launch_bounds__(256, 2) void pi_group_bf16x3_kernel(
             pi_tile_bf16x3_body<KB, true>(in.bt_hi, in.bt_lo, in.bt_f32, in.s32, in.kpad,
                                           in.naux, in.npad, in.out, PiOut::EpsLower, I0, J0, f,
                                           Ah, Al, Ch, Cl);
+}
+
+__global__ __launch_bounds__(256, 2) void pi_group_bf16x3_m32_kernel(
+    const PiInst* __restrict__ inst, const int* __restrict__ off, int ninst, int nf, int total) {
+    constexpr int KBP = PiLds<64>::pitch;
+    __shared__ __attribute__((aligned(16))) __bf16 Ah[TILE][KBP];
+    __shared__ __attribute__((aligned(16))) __bf16 Al[TILE][KBP];
+    __shared__ __attribute__((aligned(16))) __bf16 Ch[TILE][KBP];
+    __shared__ __attribute__((aligned(16))) __bf16 Cl[TILE][KBP];
+    PiInst in;
+    int I0, J0, f;
+    for (int it = 0; group_item(inst, off, ninst, nf, total, it, in, I0, J0, f); ++it)
+        if (tile_full(I0, J0, in.naux))
+            pi_tile_bf16x3_m32_body<64, false>(in.bt_hi, in.bt_lo, in.bt_f32, in.s32, in.kpad,
+                                               in.naux, in.npad, in.out, PiOut::EpsLower, I0, J0,
+                                               f, Ah, Al, Ch, Cl);
+        else
+            pi_tile_bf16x3_m32_body<64, true>(in.bt_hi, in.bt_lo, in.bt_f32, in.s32, in.kpad,
+                                              in.naux, in.npad, in.out, PiOut::EpsLower, I0, J0,
+                                              f, Ah, Al, Ch, Cl);
 }
 
 __global__ __launch_bounds__(256, 2) void pi_group_bf16x3_db_kernel(
@@ -789,6 +990,14 @@ hipError_t launch_pi_syrk(int precision, const double* B, const PiOperands& ops,
             hipLaunchKernelGGL(pi_syrk_bf16x3_db_kernel<64>, grid, block, 0, st, ops.bt_hi,
                                ops.bt_lo, ops.bt_f32, ops.s32, ops.kpad, naux, npad, nf, ntiles,
                                out, mode);
+        else if (variant == 5)
+            hipLaunchKernelGGL(pi_syrk_bf16x3_m32_kernel<64>, grid, block, 0, st, ops.bt_hi,
+                               ops.bt_lo, ops.bt_f32, ops.s32, ops.kpad, naux, npad, nf, ntiles,
+                               out, mode);
+        else if (variant == 6)
+            hipLaunchKernelGGL(pi_syrk_bf16x3_m32_kernel<32>, grid, block, 0, st, ops.bt_hi,
+                               ops.bt_lo, ops.bt_f32, ops.s32, ops.kpad, naux, npad, nf, ntiles,
+                               out, mode);
         else if (variant == 1)
             hipLaunchKernelGGL(pi_syrk_bf16x3_kernel<64>, grid, block, 0, st, ops.bt_hi, ops.bt_lo,
                                ops.bt_f32, ops.s32, ops.kpad, naux, npad, nf, ntiles, out, mode);
@@ -799,7 +1008,7 @@ hipError_t launch_pi_syrk(int precision, const double* B, const PiOperands& ops,
     return hipGetLastError();
 }
 
-int pi_variant_count(int precision) { return precision == 2 ? 5 : 1; }
+int pi_variant_count(int precision) { return precision == 2 ? 7 : 1; }
 
 hipError_t launch_pi_group(int precision, const PiInst* inst_dev, const int* off_dev, int ninst,
                            int nf, int total, int grid, hipStream_t st, int variant) {
@@ -808,6 +1017,9 @@ hipError_t launch_pi_group(int precision, const PiInst* inst_dev, const int* off
     if (precision == 0)
         hipLaunchKernelGGL(pi_group_f64_kernel, dim3(grid), dim3(256), 0, st, inst_dev, off_dev,
                            ninst, nf, total);
+    else if (precision == 2 && variant == 3)
+        hipLaunchKernelGGL(pi_group_bf16x3_m32_kernel, dim3(grid), dim3(256), 0, st, inst_dev,
+                           off_dev, ninst, nf, total);
     else if (precision == 2 && variant == 2)
         hipLaunchKernelGGL(pi_group_bf16x3_db_kernel, dim3(grid), dim3(256), 0, st, inst_dev,
                            off_dev, ninst, nf, total);
@@ -822,13 +1034,15 @@ hipError_t launch_pi_group(int precision, const PiInst* inst_dev, const int* off
     return hipGetLastError();
 }
 
-int pi_group_variant_count(int precision) { return precision == 2 ? 3 : 1; }
+int pi_group_variant_count(int precision) { return precision == 2 ? 4 : 1; }
 
 int pi_group_occupancy(int precision, int variant) {
     int n = 0;
     hipError_t e;
     if (precision == 0)
         e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, pi_group_f64_kernel, 256, 0);
+    else if (variant == 3)
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, pi_group_bf16x3_m32_kernel, 256, 0);
     else if (variant == 2)
         e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, pi_group_bf16x3_db_kernel, 256, 0);
     else if (variant == 1)

@@ -66,9 +66,10 @@ hipError_t launch_pi_prepare_mixed(const double* B, int nov, int naux, int nrows
                                    hipStream_t st);
 // precision: 0 = FP64 MFMA, 1 = FP32 MFMA, 2 = BF16x3 MFMA (FP32 accumulation).
 // variant selects the kernel instance: BF16x3 has 128x128 tiles with K chunks 32 (0) and
-// 64 (1), 256x128 tiles with a swizzled LDS layout (2) and double-buffered 128x128 tiles
-// with K chunks of 32 (3) and 64 (4); FP64/FP32 have one instance.  pi_variant_count() tells how many
-// exist per precision.
+// 64 (1), 256x128 tiles with a swizzled LDS layout (2), double-buffered 128x128 tiles with
+// K chunks of 32 (3) and 64 (4), and 128x128 tiles on 32x32x16 MFMAs with K chunks of 64
+// (5) and 32 (6); FP64/FP32 have one instance.  pi_variant_count() tells how many exist
+// per precision.
 hipError_t launch_pi_syrk(int precision, const double* B, const PiOperands& ops,
                           const double* s, int nov, int naux, int npad, int nf,
                           double* out, PiOut mode, hipStream_t st, int variant = 0);
@@ -89,7 +90,7 @@ struct PiInst {
 // Persistent grouped Pi over all instances; off_dev: [ninst + 1] prefix sums of
 // ntiles_q * nf with ntiles_q = T (T + 1) / 2, T = ceil(npad_q / 128).  grid is rounded
 // down to a multiple of 8.  precision 0 (FP64) or 2 (BF16x3, variants 0 = K32, 1 = K64,
-// 2 = double-buffered K32).
+// 2 = double-buffered K32, 3 = K64 on 32x32x16 MFMAs).
 hipError_t launch_pi_group(int precision, const PiInst* inst_dev, const int* off_dev, int ninst,
                            int nf, int total, int grid, hipStream_t st, int variant = 0);
 int pi_group_variant_count(int precision);

@@ -122,9 +122,10 @@ def test_batch_waves_and_fallback(gpu, monkeypatch):
         _close(f"waves inst {q}", got[q]["qp_energies"], ref[q]["qp_energies"], 1e-10)
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3])
 def test_batch_every_grouped_pi_variant(gpu, variant, monkeypatch):
-    """Each grouped BF16x3 Pi kernel instance (K32, K64, double-buffered K32), forced."""
+    """Each grouped BF16x3 Pi kernel instance (K32, K64, double-buffered K32, K64 on 32x32x16
+    MFMAs), forced."""
     import torch
     from quasix_amd import gw
     inst = _instances(torch, SHAPES[:4], seed0=100)
