@@ -399,13 +399,9 @@ __device__ __forceinline__ void pi_tile_bf16x3_body(
 // v_mfma_f32_32x32x16_bf16 per block and product, half the instructions of the 16x16x32
 // form for the same work.  Measured on MI355X, one wave issuing four independent 32x32x16
 // chains runs the matrix pipe near its peak, where 16x16x32 chains need two waves.
-// Fragments: lane l holds row l % 32 and the 8 consecutive k at (l / 32) * 8 of the step.
-// Output: lane l, register v -> row 8 (v / 4) + 4 (l / 32) + v % 4, column l % 32.
+// Fragments: lane l holds row l % 32 and the 8 consecutive k at (l / 32) * 8 of the step;
+// outputs as mfma_row_32.
 typedef float f16v __attribute__((ext_vector_type(16)));
-
-__device__ __forceinline__ int mfma32_row(int lane, int v) {
-    return 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
-}
 
 __device__ __forceinline__ bool block32_live(int I0, int J0, int wr, int wc, int bi, int bj,
                                              int naux) {
@@ -447,7 +443,7 @@ __device__ __forceinline__ void store_tile32(const f16v (&acc)[2][2], int I0, in
         for (int bj = 0; bj < 2; ++bj)
             #pragma unroll
             for (int v = 0; v < 16; ++v) {
-                const int row = I0 + wr * 64 + bi * 32 + mfma32_row(lane, v);
+                const int row = I0 + wr * 64 + bi * 32 + mfma_row_32(lane, v);
                 const int col = J0 + wc * 64 + bj * 32 + (lane & 31);
                 const double p = (double)acc[bi][bj][v];
                 if (mode == PiOut::EpsLower) {

@@ -29,6 +29,11 @@ inline int ceil_div(int x, int m) { return (x + m - 1) / m; }
 //   v_mfma_f64_16x16x4f64:                            row = 4 * v + lane / 16
 __device__ __forceinline__ int mfma_row_f32(int lane, int v) { return 4 * (lane >> 4) + v; }
 __device__ __forceinline__ int mfma_row_f64(int lane, int v) { return 4 * v + (lane >> 4); }
+// 32x32 output block of v_mfma_f32_32x32x16_bf16 (16 registers, column lane % 32), measured
+// the same way: row = 8 * (v / 4) + 4 * (lane / 32) + v % 4.
+__device__ __forceinline__ int mfma_row_32(int lane, int v) {
+    return 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+}
 
 // Work tables of grouped (batched) launches: off[0..n] are the first work items of the n
 // instances; returns the instance that owns item w (off[0] <= w < off[n]).  The search
