@@ -347,10 +347,13 @@ struct WnmResult {
     int pi_variant = 0;
 };
 
-// Returns the variant to use for `kind` at this shape, timing all candidates on first use.
+// Returns the variant to use for `kind` at this shape, timing the candidates on first use
+// (every variant when `candidates` is empty; QXGW_PI_VARIANT can force any of the
+// nvariants).  Unmeasured variants record -1 ms in the tuning file.
 template <class Launch>
 int choose_variant(qx_handle h, const char* kind, int precision, int nvariants,
-                   std::initializer_list<int> shape, Launch launch) {
+                   std::initializer_list<int> shape, Launch launch,
+                   std::initializer_list<int> candidates = {}) {
     if (nvariants <= 1) return 0;
     if (const char* f = std::getenv("QXGW_PI_VARIANT")) {
         const int v = std::atoi(f);
@@ -363,23 +366,41 @@ int choose_variant(qx_handle h, const char* kind, int precision, int nvariants,
     tune_load(h);
     auto it = h->tuned.find(key);
     if (it != h->tuned.end() && it->second < nvariants) return it->second;
-    std::vector<float> ms(nvariants, 0.f);
+    std::vector<int> cand(candidates);
+    if (cand.empty())
+        for (int v = 0; v < nvariants; ++v) cand.push_back(v);
+    std::vector<float> ms(nvariants, -1.f);
     hipEvent_t a = take_event(h), b = take_event(h);
-    for (int v = 0; v < nvariants; ++v) {
+    int best = cand.front();
+    for (int v : cand) {
+        if (v < 0 || v >= nvariants) continue;
         launch(v);                                   // warm-up (code object load, caches)
         HIPCK(h, hipEventRecord(a, h->stream));
         launch(v);
         HIPCK(h, hipEventRecord(b, h->stream));
         HIPCK(h, hipEventSynchronize(b));
         HIPCK(h, hipEventElapsedTime(&ms[v], a, b));
+        if (ms[v] < ms[best] || ms[best] < 0.f) best = v;
     }
     h->ev_free.push_back(a);
     h->ev_free.push_back(b);
-    const int best = (int)(std::min_element(ms.begin(), ms.end()) - ms.begin());
     h->tuned[key] = best;
     tune_store(key, best, ms);
     return best;
 }
+
+// Frequencies used when timing a Pi variant: enough work items (tiles x frequencies) to
+// fill the chip several times over, so a large problem is not tuned at full cost.
+int tune_nf(int naux_or_npad, int nf) {
+    const int T = ceil_div(naux_or_npad, 128);
+    const int tiles = T * (T + 1) / 2;
+    return std::max(1, std::min(nf, ceil_div(4096, std::max(tiles, 1))));
+}
+
+// Autotuner candidates of the BF16x3 Pi kernels: the instances that win at some measured
+// shape (K32, K64, and both on 32x32x16 MFMAs); the others stay selectable by
+// QXGW_PI_VARIANT.
+#define QX_PI_CANDIDATES {0, 1, 5, 6}
 
 // Side streams and the group count for the factorisation step loop (QXGW_POTRF_GROUPS,
 // default 3; 1 runs everything on the handle's stream).
@@ -453,9 +474,9 @@ WnmResult build_wnm(qx_handle h, int precision, int nocc, int nvir, int naux,
     }
     const int pv = choose_variant(h, "pi", precision, pi_variant_count(precision),
                                   {naux, nov, nf}, [&](int v) {
-        HIPCK(h, launch_pi_syrk(precision, B_ov, ops, s, nov, naux, npad, nf, eps_mat,
-                                PiOut::EpsLower, st, v));
-    });
+        HIPCK(h, launch_pi_syrk(precision, B_ov, ops, s, nov, naux, npad, tune_nf(npad, nf),
+                                eps_mat, PiOut::EpsLower, st, v));
+    }, QX_PI_CANDIDATES);
     r.pi_variant = pv;
     {
         Phase ph(h, QX_PHASE_PI);
@@ -953,9 +974,9 @@ qx_status qx_rho_response_dev(qx_handle h, int precision, int nocc, int nvir, in
         }
         const int pv = choose_variant(h, "pi_full", precision, pi_variant_count(precision),
                                       {naux, nov, nfreq}, [&](int v) {
-            HIPCK(h, launch_pi_syrk(precision, B_ov, ops, s, nov, naux, naux, nfreq, pi_out,
-                                    PiOut::PiFull, h->stream, v));
-        });
+            HIPCK(h, launch_pi_syrk(precision, B_ov, ops, s, nov, naux, naux,
+                                    tune_nf(naux, nfreq), pi_out, PiOut::PiFull, h->stream, v));
+        }, QX_PI_CANDIDATES);
         HIPCK(h, launch_pi_syrk(precision, B_ov, ops, s, nov, naux, naux, nfreq, pi_out,
                                 PiOut::PiFull, h->stream, pv));
         HIPCK(h, hipStreamSynchronize(h->stream));
