@@ -560,6 +560,152 @@ __global__ __launch_bounds__(256, KB == 32 ? 3 : 2) void pi_syrk_bf16x3_m32_kern
                                           mode, I * TILE, J * TILE, f, Ah, Al, Ch, Cl);
 }
 
+// ---- Producer/consumer form ---------------------------------------------------------------
+// 512 threads per 128x128 tile and frequency: waves 0-3 only compute (each a 64x64 block on
+// 32x32x16 MFMAs, reading the fragments of the next k-step ahead of the current MFMAs),
+// waves 4-7 only stage (global loads, s_f scaling and hi/lo split of the C rows, LDS
+// writes).  The LDS holds two K64 chunks (128 KB, one workgroup per CU, two waves per
+// SIMD: one of each kind): while the consumers multiply chunk i, the producers stage chunk
+// i + 1 into the other buffer and load chunk i + 2 into registers; one barrier per chunk
+// hands the buffers over.  The matrix pipe of each SIMD is fed by one wave that does
+// nothing else.
+constexpr int WSK = 64;                           // K chunk
+constexpr int WSBUF = 4 * TILE * WSK;             // bf16 per buffer (Ah, Al, Ch, Cl)
+
+template <bool Guarded>
+__device__ __forceinline__ void pi_tile_bf16x3_ws_body(
+    const uint16_t* __restrict__ bt_hi, const uint16_t* __restrict__ bt_lo,
+    const float* __restrict__ bt_f32, const float* __restrict__ s32, int kpad, int naux,
+    int npad, double* __restrict__ out, PiOut mode, int I0, int J0, int f, __bf16* L) {
+    typedef PiLds<WSK> Lay;
+    typedef __bf16 Row[WSK];
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int nch = kpad / WSK;
+    auto arr = [&](int buf, int a) { return reinterpret_cast<Row*>(L + buf * WSBUF + a * TILE * WSK); };
+    if (wave < 4) {
+        // ---- consumer ----
+        const int wr = wave >> 1, wc = wave & 1, r32 = lane & 31;
+        f16v acc[2][2];
+        bool live[2][2];
+        #pragma unroll
+        for (int bi = 0; bi < 2; ++bi)
+            #pragma unroll
+            for (int bj = 0; bj < 2; ++bj) {
+                #pragma unroll
+                for (int v = 0; v < 16; ++v) acc[bi][bj][v] = 0.f;
+                live[bi][bj] = block32_live(I0, J0, wr, wc, bi, bj, naux);
+            }
+        __syncthreads();                          // chunk 0 staged
+        for (int i = 0; i < nch; ++i) {
+            Row* Ah = arr(i & 1, 0);
+            Row* Al = arr(i & 1, 1);
+            Row* Ch = arr(i & 1, 2);
+            Row* Cl = arr(i & 1, 3);
+            bf16x8 fa[2][4][2];
+            auto read_step = [&](int set, int ks) {
+                // rows w*64 + q*32 + r32: the swizzle depends on r32 only
+                const int kq = Lay::col(r32, ks / 8 + (lane >> 5));
+                #pragma unroll
+                for (int q = 0; q < 2; ++q) {
+                    fa[set][0][q] = *reinterpret_cast<const bf16x8*>(&Ah[wr * 64 + q * 32 + r32][kq]);
+                    fa[set][1][q] = *reinterpret_cast<const bf16x8*>(&Al[wr * 64 + q * 32 + r32][kq]);
+                    fa[set][2][q] = *reinterpret_cast<const bf16x8*>(&Ch[wc * 64 + q * 32 + r32][kq]);
+                    fa[set][3][q] = *reinterpret_cast<const bf16x8*>(&Cl[wc * 64 + q * 32 + r32][kq]);
+                }
+            };
+            read_step(0, 0);
+            #pragma unroll
+            for (int ks = 0; ks < WSK; ks += 16) {
+                const int cur = (ks / 16) & 1;
+                if (ks + 16 < WSK) read_step(cur ^ 1, ks + 16);
+                __builtin_amdgcn_sched_barrier(0);
+                mfma32_bf16x3<Guarded>(acc, fa[cur][0], fa[cur][1], fa[cur][2], fa[cur][3], live);
+            }
+            __syncthreads();                      // buffer i & 1 free, chunk i + 1 staged
+        }
+        store_tile32(acc, I0, J0, wr, wc, lane, npad, naux, f, mode, out);
+    } else {
+        // ---- producer ----
+        // Two register sets: chunk c is loaded into set c % 2 two iterations before it is
+        // staged, which covers the latency of loads that miss the L2.
+        constexpr int TPR = WSK / 8, RSTEP = 256 / TPR, R = TILE / RSTEP;
+        const int ptid = tid - 256;
+        const float* sf = s32 + (size_t)f * kpad;
+        const int row0 = ptid / TPR, kq0 = (ptid % TPR) * 8;
+        u16x8 rh[2][R], rl[2][R];
+        f4 rc[2][2 * R], rs[2][2];
+        auto fetch = [&](int set, int k0) {
+            #pragma unroll
+            for (int q = 0; q < R; ++q) {
+                const int row = row0 + q * RSTEP;
+                // operand rows are allocated and zero-filled up to a multiple of 256 >= the
+                // tile grid, so tile rows past npad load zeros without a bounds select
+                const size_t oa = (size_t)(I0 + row) * kpad + k0 + kq0;
+                const size_t oc = (size_t)(J0 + row) * kpad + k0 + kq0;
+                rh[set][q] = *reinterpret_cast<const u16x8*>(bt_hi + oa);
+                rl[set][q] = *reinterpret_cast<const u16x8*>(bt_lo + oa);
+                rc[set][2 * q] = *reinterpret_cast<const f4*>(bt_f32 + oc);
+                rc[set][2 * q + 1] = *reinterpret_cast<const f4*>(bt_f32 + oc + 4);
+            }
+            rs[set][0] = *reinterpret_cast<const f4*>(sf + k0 + kq0);
+            rs[set][1] = *reinterpret_cast<const f4*>(sf + k0 + kq0 + 4);
+        };
+        auto stage = [&](int set, int buf) {
+            Row* Ah = arr(buf, 0);
+            Row* Al = arr(buf, 1);
+            Row* Ch = arr(buf, 2);
+            Row* Cl = arr(buf, 3);
+            #pragma unroll
+            for (int q = 0; q < R; ++q) {
+                const int row = row0 + q * RSTEP;
+                const int kc = Lay::col(row, kq0 >> 3);
+                *reinterpret_cast<u16x8*>(&Ah[row][kc]) = rh[set][q];
+                *reinterpret_cast<u16x8*>(&Al[row][kc]) = rl[set][q];
+                #pragma unroll
+                for (int hq = 0; hq < 2; ++hq) {
+                    bf16x4 h, l;
+                    split_bf16x4(rc[set][2 * q + hq] * rs[set][hq], h, l);
+                    *reinterpret_cast<bf16x4*>(&Ch[row][kc + 4 * hq]) = h;
+                    *reinterpret_cast<bf16x4*>(&Cl[row][kc + 4 * hq]) = l;
+                }
+            }
+        };
+        fetch(0, 0);
+        stage(0, 0);
+        if (nch > 1) fetch(1, WSK);
+        if (nch > 2) fetch(0, 2 * WSK);
+        __syncthreads();                          // chunk 0 staged
+        // one barrier per chunk, as the consumers: iterations i (even) and i + 1
+        for (int i = 0; i < nch; i += 2) {
+            if (i + 1 < nch) stage(1, 1);         // chunk i + 1 from set 1 into buffer 1
+            if (i + 3 < nch) fetch(1, (i + 3) * WSK);
+            __syncthreads();
+            if (i + 1 >= nch) break;
+            if (i + 2 < nch) stage(0, 0);         // chunk i + 2 from set 0 into buffer 0
+            if (i + 4 < nch) fetch(0, (i + 4) * WSK);
+            __syncthreads();
+        }
+    }
+}
+
+__global__ __launch_bounds__(512, 1) void pi_syrk_bf16x3_ws_kernel(
+    const uint16_t* __restrict__ bt_hi, const uint16_t* __restrict__ bt_lo,
+    const float* __restrict__ bt_f32, const float* __restrict__ s32, int kpad, int naux,
+    int npad, int nf, int ntiles, double* __restrict__ out, PiOut mode) {
+    __shared__ __attribute__((aligned(16))) __bf16 L[2 * WSBUF];
+    const int work = xcd_remap(blockIdx.x, ntiles * nf);
+    const int tile = work / nf, f = work % nf;
+    int I, J;
+    tri_index(tile, I, J);
+    if (tile_full(I * TILE, J * TILE, naux))
+        pi_tile_bf16x3_ws_body<false>(bt_hi, bt_lo, bt_f32, s32, kpad, naux, npad, out, mode,
+                                      I * TILE, J * TILE, f, L);
+    else
+        pi_tile_bf16x3_ws_body<true>(bt_hi, bt_lo, bt_f32, s32, kpad, naux, npad, out, mode,
+                                     I * TILE, J * TILE, f, L);
+}
+
 // Double-buffered variants: two LDS buffers, one barrier per chunk.  While the MFMAs of
 // chunk i read buffer i % 2, the same waves split chunk i + 1 (already in registers) into the
 // other buffer and issue the global loads of chunk i + 2, so the split work and the load
@@ -994,6 +1140,12 @@ hipError_t launch_pi_syrk(int precision, const double* B, const PiOperands& ops,
             hipLaunchKernelGGL(pi_syrk_bf16x3_m32_kernel<32>, grid, block, 0, st, ops.bt_hi,
                                ops.bt_lo, ops.bt_f32, ops.s32, ops.kpad, naux, npad, nf, ntiles,
                                out, mode);
+        else if (variant == 7) {
+            if (ops.kpad % WSK) return hipErrorInvalidValue;
+            hipLaunchKernelGGL(pi_syrk_bf16x3_ws_kernel, grid, dim3(512), 0, st, ops.bt_hi,
+                               ops.bt_lo, ops.bt_f32, ops.s32, ops.kpad, naux, npad, nf, ntiles,
+                               out, mode);
+        }
         else if (variant == 1)
             hipLaunchKernelGGL(pi_syrk_bf16x3_kernel<64>, grid, block, 0, st, ops.bt_hi, ops.bt_lo,
                                ops.bt_f32, ops.s32, ops.kpad, naux, npad, nf, ntiles, out, mode);
@@ -1004,7 +1156,7 @@ hipError_t launch_pi_syrk(int precision, const double* B, const PiOperands& ops,
     return hipGetLastError();
 }
 
-int pi_variant_count(int precision) { return precision == 2 ? 7 : 1; }
+int pi_variant_count(int precision) { return precision == 2 ? 8 : 1; }
 
 hipError_t launch_pi_group(int precision, const PiInst* inst_dev, const int* off_dev, int ninst,
                            int nf, int total, int grid, hipStream_t st, int variant) {
