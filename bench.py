@@ -41,7 +41,8 @@ def measured_traffic(kernel):
         rec = json.load(open(TRAFFIC))
     except (OSError, ValueError):
         return None
-    return rec["bytes_per_launch"] if rec.get("kernel") == kernel else None
+    k = rec.get("kernels", {}).get(kernel)
+    return k["bytes_per_launch"] if k else None
 
 
 def pi_flops(nov, naux, nf):
