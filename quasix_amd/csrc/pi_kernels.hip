@@ -706,6 +706,148 @@ __global__ __launch_bounds__(512, 1) void pi_syrk_bf16x3_ws_kernel(
                                      I * TILE, J * TILE, f, L);
 }
 
+// Producer/consumer form with two frequencies per tile (K32 chunks): the A operand and the
+// FP32 C rows of a chunk are loaded and staged once and serve both frequencies, which halves
+// the global and L2 traffic per MFMA; the producers split C for each frequency, the
+// consumers keep two sets of accumulators.  Buffers: Ah, Al, Ch[2], Cl[2] of 128 x 40 bf16
+// (60 KB), two of them.  f1 == f0 (odd frequency count) computes one twice, stores it once.
+constexpr int WS2K = 32;
+constexpr int WS2P = PiLds<WS2K>::pitch;
+constexpr int WS2A = TILE * WS2P;                 // bf16 per array
+constexpr int WS2BUF = 6 * WS2A;                  // bf16 per buffer
+
+template <bool Guarded>
+__device__ __forceinline__ void pi_tile_bf16x3_ws2_body(
+    const uint16_t* __restrict__ bt_hi, const uint16_t* __restrict__ bt_lo,
+    const float* __restrict__ bt_f32, const float* __restrict__ s32, int kpad, int naux,
+    int npad, double* __restrict__ out, PiOut mode, int I0, int J0, int f0, int f1,
+    __bf16* L) {
+    typedef __bf16 Row[WS2P];
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int nch = kpad / WS2K;
+    // arrays: 0 Ah, 1 Al, 2 + fi Ch[fi], 4 + fi Cl[fi]
+    auto arr = [&](int buf, int a) { return reinterpret_cast<Row*>(L + buf * WS2BUF + a * WS2A); };
+    if (wave < 4) {
+        // ---- consumer ----
+        const int wr = wave >> 1, wc = wave & 1, r32 = lane & 31;
+        f16v acc[2][2][2];
+        bool live[2][2];
+        #pragma unroll
+        for (int bi = 0; bi < 2; ++bi)
+            #pragma unroll
+            for (int bj = 0; bj < 2; ++bj) {
+                #pragma unroll
+                for (int v = 0; v < 16; ++v) acc[0][bi][bj][v] = acc[1][bi][bj][v] = 0.f;
+                live[bi][bj] = block32_live(I0, J0, wr, wc, bi, bj, naux);
+            }
+        __syncthreads();                          // chunk 0 staged
+        for (int i = 0; i < nch; ++i) {
+            const int b = i & 1;
+            #pragma unroll
+            for (int ks = 0; ks < WS2K; ks += 16) {
+                const int kq = (ks / 8 + (lane >> 5)) * 8;
+                bf16x8 ah[2], al[2];
+                #pragma unroll
+                for (int q = 0; q < 2; ++q) {
+                    ah[q] = *reinterpret_cast<const bf16x8*>(&arr(b, 0)[wr * 64 + q * 32 + r32][kq]);
+                    al[q] = *reinterpret_cast<const bf16x8*>(&arr(b, 1)[wr * 64 + q * 32 + r32][kq]);
+                }
+                #pragma unroll
+                for (int fi = 0; fi < 2; ++fi) {
+                    bf16x8 ch[2], cl[2];
+                    #pragma unroll
+                    for (int q = 0; q < 2; ++q) {
+                        ch[q] = *reinterpret_cast<const bf16x8*>(&arr(b, 2 + fi)[wc * 64 + q * 32 + r32][kq]);
+                        cl[q] = *reinterpret_cast<const bf16x8*>(&arr(b, 4 + fi)[wc * 64 + q * 32 + r32][kq]);
+                    }
+                    mfma32_bf16x3<Guarded>(acc[fi], ah, al, ch, cl, live);
+                }
+            }
+            __syncthreads();                      // buffer i & 1 free, chunk i + 1 staged
+        }
+        store_tile32(acc[0], I0, J0, wr, wc, lane, npad, naux, f0, mode, out);
+        if (f1 != f0) store_tile32(acc[1], I0, J0, wr, wc, lane, npad, naux, f1, mode, out);
+    } else {
+        // ---- producer: two register sets, chunk c loaded into set c % 2 two iterations
+        // before it is staged ----
+        constexpr int TPR = WS2K / 8, RSTEP = 256 / TPR, R = TILE / RSTEP;
+        const int ptid = tid - 256;
+        const float* sf0 = s32 + (size_t)f0 * kpad;
+        const float* sf1 = s32 + (size_t)f1 * kpad;
+        const int row0 = ptid / TPR, kq0 = (ptid % TPR) * 8;
+        u16x8 rh[2][R], rl[2][R];
+        f4 rc[2][2 * R], rs[2][2][2];             // rs[set][frequency][half]
+        auto fetch = [&](int set, int k0) {
+            #pragma unroll
+            for (int q = 0; q < R; ++q) {
+                const int row = row0 + q * RSTEP;
+                // operand rows are allocated and zero-filled up to a multiple of 256 >= the
+                // tile grid, so tile rows past npad load zeros without a bounds select
+                const size_t oa = (size_t)(I0 + row) * kpad + k0 + kq0;
+                const size_t oc = (size_t)(J0 + row) * kpad + k0 + kq0;
+                rh[set][q] = *reinterpret_cast<const u16x8*>(bt_hi + oa);
+                rl[set][q] = *reinterpret_cast<const u16x8*>(bt_lo + oa);
+                rc[set][2 * q] = *reinterpret_cast<const f4*>(bt_f32 + oc);
+                rc[set][2 * q + 1] = *reinterpret_cast<const f4*>(bt_f32 + oc + 4);
+            }
+            rs[set][0][0] = *reinterpret_cast<const f4*>(sf0 + k0 + kq0);
+            rs[set][0][1] = *reinterpret_cast<const f4*>(sf0 + k0 + kq0 + 4);
+            rs[set][1][0] = *reinterpret_cast<const f4*>(sf1 + k0 + kq0);
+            rs[set][1][1] = *reinterpret_cast<const f4*>(sf1 + k0 + kq0 + 4);
+        };
+        auto stage = [&](int set, int buf) {
+            #pragma unroll
+            for (int q = 0; q < R; ++q) {
+                const int row = row0 + q * RSTEP;
+                *reinterpret_cast<u16x8*>(&arr(buf, 0)[row][kq0]) = rh[set][q];
+                *reinterpret_cast<u16x8*>(&arr(buf, 1)[row][kq0]) = rl[set][q];
+                #pragma unroll
+                for (int fi = 0; fi < 2; ++fi)
+                    #pragma unroll
+                    for (int hq = 0; hq < 2; ++hq) {
+                        bf16x4 h, l;
+                        split_bf16x4(rc[set][2 * q + hq] * rs[set][fi][hq], h, l);
+                        *reinterpret_cast<bf16x4*>(&arr(buf, 2 + fi)[row][kq0 + 4 * hq]) = h;
+                        *reinterpret_cast<bf16x4*>(&arr(buf, 4 + fi)[row][kq0 + 4 * hq]) = l;
+                    }
+            }
+        };
+        fetch(0, 0);
+        stage(0, 0);
+        if (nch > 1) fetch(1, WS2K);
+        if (nch > 2) fetch(0, 2 * WS2K);
+        __syncthreads();                          // chunk 0 staged
+        for (int i = 0; i < nch; i += 2) {
+            if (i + 1 < nch) stage(1, 1);
+            if (i + 3 < nch) fetch(1, (i + 3) * WS2K);
+            __syncthreads();
+            if (i + 1 >= nch) break;
+            if (i + 2 < nch) stage(0, 0);
+            if (i + 4 < nch) fetch(0, (i + 4) * WS2K);
+            __syncthreads();
+        }
+    }
+}
+
+__global__ __launch_bounds__(512, 1) void pi_syrk_bf16x3_ws2_kernel(
+    const uint16_t* __restrict__ bt_hi, const uint16_t* __restrict__ bt_lo,
+    const float* __restrict__ bt_f32, const float* __restrict__ s32, int kpad, int naux,
+    int npad, int nf, int ntiles, double* __restrict__ out, PiOut mode) {
+    __shared__ __attribute__((aligned(16))) __bf16 L[2 * WS2BUF];
+    const int nfp = (nf + 1) / 2;
+    const int work = xcd_remap(blockIdx.x, ntiles * nfp);
+    const int tile = work / nfp, f0 = 2 * (work % nfp), f1 = min(f0 + 1, nf - 1);
+    int I, J;
+    tri_index(tile, I, J);
+    if (tile_full(I * TILE, J * TILE, naux))
+        pi_tile_bf16x3_ws2_body<false>(bt_hi, bt_lo, bt_f32, s32, kpad, naux, npad, out, mode,
+                                       I * TILE, J * TILE, f0, f1, L);
+    else
+        pi_tile_bf16x3_ws2_body<true>(bt_hi, bt_lo, bt_f32, s32, kpad, naux, npad, out, mode,
+                                      I * TILE, J * TILE, f0, f1, L);
+}
+
 // Double-buffered variants: two LDS buffers, one barrier per chunk.  While the MFMAs of
 // chunk i read buffer i % 2, the same waves split chunk i + 1 (already in registers) into the
 // other buffer and issue the global loads of chunk i + 2, so the split work and the load
@@ -1145,6 +1287,11 @@ hipError_t launch_pi_syrk(int precision, const double* B, const PiOperands& ops,
             hipLaunchKernelGGL(pi_syrk_bf16x3_ws_kernel, grid, dim3(512), 0, st, ops.bt_hi,
                                ops.bt_lo, ops.bt_f32, ops.s32, ops.kpad, naux, npad, nf, ntiles,
                                out, mode);
+        } else if (variant == 8) {
+            if (ops.kpad % WS2K) return hipErrorInvalidValue;
+            hipLaunchKernelGGL(pi_syrk_bf16x3_ws2_kernel, dim3(ntiles * ((nf + 1) / 2)), dim3(512),
+                               0, st, ops.bt_hi, ops.bt_lo, ops.bt_f32, ops.s32, ops.kpad, naux,
+                               npad, nf, ntiles, out, mode);
         }
         else if (variant == 1)
             hipLaunchKernelGGL(pi_syrk_bf16x3_kernel<64>, grid, block, 0, st, ops.bt_hi, ops.bt_lo,
@@ -1156,7 +1303,7 @@ hipError_t launch_pi_syrk(int precision, const double* B, const PiOperands& ops,
     return hipGetLastError();
 }
 
-int pi_variant_count(int precision) { return precision == 2 ? 8 : 1; }
+int pi_variant_count(int precision) { return precision == 2 ? 9 : 1; }
 
 hipError_t launch_pi_group(int precision, const PiInst* inst_dev, const int* off_dev, int ninst,
                            int nf, int total, int grid, hipStream_t st, int variant) {
