@@ -569,20 +569,27 @@ __global__ __launch_bounds__(256, KB == 32 ? 3 : 2) void pi_syrk_bf16x3_m32_kern
 // i + 1 into the other buffer and load chunk i + 2 into registers; one barrier per chunk
 // hands the buffers over.  The matrix pipe of each SIMD is fed by one wave that does
 // nothing else.
-constexpr int WSK = 64;                           // K chunk
-constexpr int WSBUF = 4 * TILE * WSK;             // bf16 per buffer (Ah, Al, Ch, Cl)
+// WSK = 64: 128 KB of LDS, one workgroup per CU, consumers read the next k-step ahead.
+// WSK = 32: 80 KB, two workgroups per CU (four waves per SIMD, at most 128 VGPRs), no
+// read-ahead; a stalled workgroup leaves the CU to the other one.
+template <int WSK> struct WsCfg {
+    static constexpr int pitch = PiLds<WSK>::pitch;
+    static constexpr int buf = 4 * TILE * pitch;  // bf16 per buffer (Ah, Al, Ch, Cl)
+};
 
-template <bool Guarded>
+template <int WSK, bool Guarded>
 __device__ __forceinline__ void pi_tile_bf16x3_ws_body(
     const uint16_t* __restrict__ bt_hi, const uint16_t* __restrict__ bt_lo,
     const float* __restrict__ bt_f32, const float* __restrict__ s32, int kpad, int naux,
     int npad, double* __restrict__ out, PiOut mode, int I0, int J0, int f, __bf16* L) {
     typedef PiLds<WSK> Lay;
-    typedef __bf16 Row[WSK];
+    constexpr int P = WsCfg<WSK>::pitch, WSBUF = WsCfg<WSK>::buf;
+    constexpr bool Ahead = WSK == 64;
+    typedef __bf16 Row[P];
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int nch = kpad / WSK;
-    auto arr = [&](int buf, int a) { return reinterpret_cast<Row*>(L + buf * WSBUF + a * TILE * WSK); };
+    auto arr = [&](int buf, int a) { return reinterpret_cast<Row*>(L + buf * WSBUF + a * TILE * P); };
     if (wave < 4) {
         // ---- consumer ----
         const int wr = wave >> 1, wc = wave & 1, r32 = lane & 31;
@@ -614,12 +621,12 @@ __device__ __forceinline__ void pi_tile_bf16x3_ws_body(
                     fa[set][3][q] = *reinterpret_cast<const bf16x8*>(&Cl[wc * 64 + q * 32 + r32][kq]);
                 }
             };
-            read_step(0, 0);
             #pragma unroll
             for (int ks = 0; ks < WSK; ks += 16) {
-                const int cur = (ks / 16) & 1;
-                if (ks + 16 < WSK) read_step(cur ^ 1, ks + 16);
-                __builtin_amdgcn_sched_barrier(0);
+                const int cur = Ahead ? (ks / 16) & 1 : 0;
+                if (!Ahead || ks == 0) read_step(cur, ks);
+                if (Ahead && ks + 16 < WSK) read_step(cur ^ 1, ks + 16);
+                if (Ahead) __builtin_amdgcn_sched_barrier(0);
                 mfma32_bf16x3<Guarded>(acc, fa[cur][0], fa[cur][1], fa[cur][2], fa[cur][3], live);
             }
             __syncthreads();                      // buffer i & 1 free, chunk i + 1 staged
@@ -689,21 +696,22 @@ __device__ __forceinline__ void pi_tile_bf16x3_ws_body(
     }
 }
 
-__global__ __launch_bounds__(512, 1) void pi_syrk_bf16x3_ws_kernel(
+template <int WSK>
+__global__ __launch_bounds__(512, WSK == 32 ? 2 : 1) void pi_syrk_bf16x3_ws_kernel(
     const uint16_t* __restrict__ bt_hi, const uint16_t* __restrict__ bt_lo,
     const float* __restrict__ bt_f32, const float* __restrict__ s32, int kpad, int naux,
     int npad, int nf, int ntiles, double* __restrict__ out, PiOut mode) {
-    __shared__ __attribute__((aligned(16))) __bf16 L[2 * WSBUF];
+    __shared__ __attribute__((aligned(16))) __bf16 L[2 * WsCfg<WSK>::buf];
     const int work = xcd_remap(blockIdx.x, ntiles * nf);
     const int tile = work / nf, f = work % nf;
     int I, J;
     tri_index(tile, I, J);
     if (tile_full(I * TILE, J * TILE, naux))
-        pi_tile_bf16x3_ws_body<false>(bt_hi, bt_lo, bt_f32, s32, kpad, naux, npad, out, mode,
-                                      I * TILE, J * TILE, f, L);
+        pi_tile_bf16x3_ws_body<WSK, false>(bt_hi, bt_lo, bt_f32, s32, kpad, naux, npad, out,
+                                           mode, I * TILE, J * TILE, f, L);
     else
-        pi_tile_bf16x3_ws_body<true>(bt_hi, bt_lo, bt_f32, s32, kpad, naux, npad, out, mode,
-                                     I * TILE, J * TILE, f, L);
+        pi_tile_bf16x3_ws_body<WSK, true>(bt_hi, bt_lo, bt_f32, s32, kpad, naux, npad, out,
+                                          mode, I * TILE, J * TILE, f, L);
 }
 
 // Producer/consumer form with two frequencies per tile (K32 chunks): the A operand and the
@@ -1282,11 +1290,12 @@ hipError_t launch_pi_syrk(int precision, const double* B, const PiOperands& ops,
             hipLaunchKernelGGL(pi_syrk_bf16x3_m32_kernel<32>, grid, block, 0, st, ops.bt_hi,
                                ops.bt_lo, ops.bt_f32, ops.s32, ops.kpad, naux, npad, nf, ntiles,
                                out, mode);
-        else if (variant == 7) {
-            if (ops.kpad % WSK) return hipErrorInvalidValue;
-            hipLaunchKernelGGL(pi_syrk_bf16x3_ws_kernel, grid, dim3(512), 0, st, ops.bt_hi,
-                               ops.bt_lo, ops.bt_f32, ops.s32, ops.kpad, naux, npad, nf, ntiles,
-                               out, mode);
+        else if (variant == 7 || variant == 9) {
+            if (ops.kpad % 64) return hipErrorInvalidValue;
+            hipLaunchKernelGGL(variant == 7 ? pi_syrk_bf16x3_ws_kernel<64>
+                                            : pi_syrk_bf16x3_ws_kernel<32>,
+                               grid, dim3(512), 0, st, ops.bt_hi, ops.bt_lo, ops.bt_f32, ops.s32,
+                               ops.kpad, naux, npad, nf, ntiles, out, mode);
         } else if (variant == 8) {
             if (ops.kpad % WS2K) return hipErrorInvalidValue;
             hipLaunchKernelGGL(pi_syrk_bf16x3_ws2_kernel, dim3(ntiles * ((nf + 1) / 2)), dim3(512),
@@ -1303,7 +1312,7 @@ hipError_t launch_pi_syrk(int precision, const double* B, const PiOperands& ops,
     return hipGetLastError();
 }
 
-int pi_variant_count(int precision) { return precision == 2 ? 9 : 1; }
+int pi_variant_count(int precision) { return precision == 2 ? 10 : 1; }
 
 hipError_t launch_pi_group(int precision, const PiInst* inst_dev, const int* off_dev, int ninst,
                            int nf, int total, int grid, hipStream_t st, int variant) {

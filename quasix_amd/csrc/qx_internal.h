@@ -73,8 +73,9 @@ hipError_t launch_pi_prepare_mixed(const double* B, int nov, int naux, int nrows
 // variant selects the kernel instance: BF16x3 has 128x128 tiles with K chunks 32 (0) and
 // 64 (1), 256x128 tiles with a swizzled LDS layout (2), double-buffered 128x128 tiles with
 // K chunks of 32 (3) and 64 (4), 128x128 tiles on 32x32x16 MFMAs with K chunks of 64 (5)
-// and 32 (6), and the producer/consumer form with 512 threads, one frequency and K64 chunks
-// (7) or two frequencies and K32 chunks (8); FP64/FP32 have one instance.
+// and 32 (6), and the producer/consumer form with 512 threads: one frequency and K64 chunks
+// (7), two frequencies and K32 chunks (8), one frequency and K32 chunks with two workgroups
+// per CU (9); FP64/FP32 have one instance.
 // pi_variant_count() tells how many exist per precision.
 hipError_t launch_pi_syrk(int precision, const double* B, const PiOperands& ops,
                           const double* s, int nov, int naux, int npad, int nf,
