@@ -29,8 +29,9 @@ MEV = 1.0 / 27211.386245988
 PI_KERNELS = {0: "pi_syrk_bf16x3_kernel<32>", 1: "pi_syrk_bf16x3_kernel<64>",
               2: "pi_syrk_bf16x3_wide_kernel", 3: "pi_syrk_bf16x3_db_kernel<32>",
               4: "pi_syrk_bf16x3_db_kernel<64>", 5: "pi_syrk_bf16x3_m32_kernel<64>",
-              6: "pi_syrk_bf16x3_m32_kernel<32>", 7: "pi_syrk_bf16x3_ws_kernel<64>",
-              8: "pi_syrk_bf16x3_ws2_kernel", 9: "pi_syrk_bf16x3_ws_kernel<32>"}
+              6: "pi_syrk_bf16x3_m32_kernel<32>", 7: "pi_syrk_bf16x3_ws_kernel<64, 4>",
+              8: "pi_syrk_bf16x3_ws2_kernel", 9: "pi_syrk_bf16x3_ws_kernel<32, 4>",
+              10: "pi_syrk_bf16x3_ws_kernel<64, 8>"}
 TRAFFIC = os.path.join(ROOT, "profiles", "pi_hbm_traffic.json")
 
 

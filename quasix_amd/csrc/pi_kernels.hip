@@ -577,7 +577,7 @@ template <int WSK> struct WsCfg {
     static constexpr int buf = 4 * TILE * pitch;  // bf16 per buffer (Ah, Al, Ch, Cl)
 };
 
-template <int WSK, bool Guarded>
+template <int WSK, int NP, bool Guarded>
 __device__ __forceinline__ void pi_tile_bf16x3_ws_body(
     const uint16_t* __restrict__ bt_hi, const uint16_t* __restrict__ bt_lo,
     const float* __restrict__ bt_f32, const float* __restrict__ s32, int kpad, int naux,
@@ -633,10 +633,10 @@ __device__ __forceinline__ void pi_tile_bf16x3_ws_body(
         }
         store_tile32(acc, I0, J0, wr, wc, lane, npad, naux, f, mode, out);
     } else {
-        // ---- producer ----
+        // ---- producer (NP waves) ----
         // Two register sets: chunk c is loaded into set c % 2 two iterations before it is
         // staged, which covers the latency of loads that miss the L2.
-        constexpr int TPR = WSK / 8, RSTEP = 256 / TPR, R = TILE / RSTEP;
+        constexpr int TPR = WSK / 8, RSTEP = 64 * NP / TPR, R = TILE / RSTEP;
         const int ptid = tid - 256;
         const float* sf = s32 + (size_t)f * kpad;
         const int row0 = ptid / TPR, kq0 = (ptid % TPR) * 8;
@@ -696,8 +696,9 @@ __device__ __forceinline__ void pi_tile_bf16x3_ws_body(
     }
 }
 
-template <int WSK>
-__global__ __launch_bounds__(512, WSK == 32 ? 2 : 1) void pi_syrk_bf16x3_ws_kernel(
+// NP: producer waves (4, or 8 for twice the loads in flight per CU).
+template <int WSK, int NP>
+__global__ __launch_bounds__(256 + 64 * NP, WSK == 32 ? 2 : 1) void pi_syrk_bf16x3_ws_kernel(
     const uint16_t* __restrict__ bt_hi, const uint16_t* __restrict__ bt_lo,
     const float* __restrict__ bt_f32, const float* __restrict__ s32, int kpad, int naux,
     int npad, int nf, int ntiles, double* __restrict__ out, PiOut mode) {
@@ -707,11 +708,11 @@ __global__ __launch_bounds__(512, WSK == 32 ? 2 : 1) void pi_syrk_bf16x3_ws_kern
     int I, J;
     tri_index(tile, I, J);
     if (tile_full(I * TILE, J * TILE, naux))
-        pi_tile_bf16x3_ws_body<WSK, false>(bt_hi, bt_lo, bt_f32, s32, kpad, naux, npad, out,
-                                           mode, I * TILE, J * TILE, f, L);
+        pi_tile_bf16x3_ws_body<WSK, NP, false>(bt_hi, bt_lo, bt_f32, s32, kpad, naux, npad,
+                                               out, mode, I * TILE, J * TILE, f, L);
     else
-        pi_tile_bf16x3_ws_body<WSK, true>(bt_hi, bt_lo, bt_f32, s32, kpad, naux, npad, out,
-                                          mode, I * TILE, J * TILE, f, L);
+        pi_tile_bf16x3_ws_body<WSK, NP, true>(bt_hi, bt_lo, bt_f32, s32, kpad, naux, npad,
+                                              out, mode, I * TILE, J * TILE, f, L);
 }
 
 // Producer/consumer form with two frequencies per tile (K32 chunks): the A operand and the
@@ -1290,12 +1291,17 @@ hipError_t launch_pi_syrk(int precision, const double* B, const PiOperands& ops,
             hipLaunchKernelGGL(pi_syrk_bf16x3_m32_kernel<32>, grid, block, 0, st, ops.bt_hi,
                                ops.bt_lo, ops.bt_f32, ops.s32, ops.kpad, naux, npad, nf, ntiles,
                                out, mode);
-        else if (variant == 7 || variant == 9) {
+        else if (variant == 7 || variant == 9 || variant == 10) {
             if (ops.kpad % 64) return hipErrorInvalidValue;
-            hipLaunchKernelGGL(variant == 7 ? pi_syrk_bf16x3_ws_kernel<64>
-                                            : pi_syrk_bf16x3_ws_kernel<32>,
-                               grid, dim3(512), 0, st, ops.bt_hi, ops.bt_lo, ops.bt_f32, ops.s32,
-                               ops.kpad, naux, npad, nf, ntiles, out, mode);
+            if (variant == 10)
+                hipLaunchKernelGGL((pi_syrk_bf16x3_ws_kernel<64, 8>), grid, dim3(768), 0, st,
+                                   ops.bt_hi, ops.bt_lo, ops.bt_f32, ops.s32, ops.kpad, naux,
+                                   npad, nf, ntiles, out, mode);
+            else
+                hipLaunchKernelGGL(variant == 7 ? (pi_syrk_bf16x3_ws_kernel<64, 4>)
+                                                : (pi_syrk_bf16x3_ws_kernel<32, 4>),
+                                   grid, dim3(512), 0, st, ops.bt_hi, ops.bt_lo, ops.bt_f32,
+                                   ops.s32, ops.kpad, naux, npad, nf, ntiles, out, mode);
         } else if (variant == 8) {
             if (ops.kpad % WS2K) return hipErrorInvalidValue;
             hipLaunchKernelGGL(pi_syrk_bf16x3_ws2_kernel, dim3(ntiles * ((nf + 1) / 2)), dim3(512),
@@ -1312,7 +1318,7 @@ hipError_t launch_pi_syrk(int precision, const double* B, const PiOperands& ops,
     return hipGetLastError();
 }
 
-int pi_variant_count(int precision) { return precision == 2 ? 10 : 1; }
+int pi_variant_count(int precision) { return precision == 2 ? 11 : 1; }
 
 hipError_t launch_pi_group(int precision, const PiInst* inst_dev, const int* off_dev, int ninst,
                            int nf, int total, int grid, hipStream_t st, int variant) {

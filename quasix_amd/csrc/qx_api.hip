@@ -400,7 +400,7 @@ int tune_nf(int naux_or_npad, int nf) {
 // Autotuner candidates of the BF16x3 Pi kernels: the instances that win at some measured
 // shape (K32, K64, and both on 32x32x16 MFMAs); the others stay selectable by
 // QXGW_PI_VARIANT.
-#define QX_PI_CANDIDATES {0, 1, 5, 6, 7, 8, 9}
+#define QX_PI_CANDIDATES {0, 1, 5, 6, 7, 8, 9, 10}
 
 // Side streams and the group count for the factorisation step loop (QXGW_POTRF_GROUPS,
 // default 3; 1 runs everything on the handle's stream).

@@ -158,7 +158,7 @@ def test_wnm_mixed_precision(gpu, prec, tol):
     _close(f"wnm {prec}", got, ref, tol, np.max(np.abs(ref)))
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5, 6, 7, 8, 9])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10])
 def test_pi_bf16x3_every_variant(gpu, variant, monkeypatch):
     """Each BF16x3 Pi kernel instance the autotuner can pick, forced in turn."""
     from quasix_amd import core
@@ -175,7 +175,7 @@ def test_pi_bf16x3_every_variant(gpu, variant, monkeypatch):
         _close(f"pi bf16x3 v{variant} k={k}", p0[k], ref, 3e-5, np.max(np.abs(ref)))
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5, 6, 7, 8, 9])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10])
 def test_g0w0_bf16x3_every_variant(gpu, variant, monkeypatch):
     from quasix_amd import gw
     monkeypatch.setenv("QXGW_PI_VARIANT", str(variant))
