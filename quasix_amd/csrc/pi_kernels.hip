@@ -1065,6 +1065,44 @@ __global__ __launch_bounds__(256, 2) void pi_group_bf16x3_m32_kernel(
                                               f, Ah, Al, Ch, Cl);
 }
 
+// Pairs of frequencies (f, f + 1) of one tile and instance, for the two-frequency
+// producer/consumer body; valid when nf is even (every instance's item count is even, so
+// items 2 w and 2 w + 1 are such a pair).
+__device__ __forceinline__ bool group_pair_item(const PiInst* __restrict__ inst,
+                                                const int* __restrict__ off, int ninst, int nf,
+                                                int total, int it, PiInst& in, int& I0, int& J0,
+                                                int& f) {
+    const int G = gridDim.x;
+    const int w2 = it * G + (blockIdx.x & 7) * (G >> 3) + (blockIdx.x >> 3);
+    if (2 * w2 >= total) return false;
+    const int w = 2 * w2;
+    const int q = find_inst(off, ninst, w);
+    in = inst[q];
+    const int local = w - off[q];
+    f = local % nf;
+    int I, J;
+    tri_index(local / nf, I, J);
+    I0 = I * TILE;
+    J0 = J * TILE;
+    return true;
+}
+
+__global__ __launch_bounds__(512, 1) void pi_group_bf16x3_ws2_kernel(
+    const PiInst* __restrict__ inst, const int* __restrict__ off, int ninst, int nf, int total) {
+    __shared__ __attribute__((aligned(16))) __bf16 L[2 * WS2BUF];
+    PiInst in;
+    int I0, J0, f;
+    for (int it = 0; group_pair_item(inst, off, ninst, nf, total, it, in, I0, J0, f); ++it)
+        if (tile_full(I0, J0, in.naux))
+            pi_tile_bf16x3_ws2_body<false>(in.bt_hi, in.bt_lo, in.bt_f32, in.s32, in.kpad,
+                                           in.naux, in.npad, in.out, PiOut::EpsLower, I0, J0, f,
+                                           f + 1, L);
+        else
+            pi_tile_bf16x3_ws2_body<true>(in.bt_hi, in.bt_lo, in.bt_f32, in.s32, in.kpad,
+                                          in.naux, in.npad, in.out, PiOut::EpsLower, I0, J0, f,
+                                          f + 1, L);
+}
+
 __global__ __launch_bounds__(256, 2) void pi_group_bf16x3_db_kernel(
     const PiInst* __restrict__ inst, const int* __restrict__ off, int ninst, int nf, int total) {
     __shared__ __attribute__((aligned(16))) __bf16 L[2 * 4 * TILE * 32];
@@ -1327,7 +1365,11 @@ hipError_t launch_pi_group(int precision, const PiInst* inst_dev, const int* off
     if (precision == 0)
         hipLaunchKernelGGL(pi_group_f64_kernel, dim3(grid), dim3(256), 0, st, inst_dev, off_dev,
                            ninst, nf, total);
-    else if (precision == 2 && variant == 3)
+    else if (precision == 2 && variant == 4) {
+        if (nf % 2) return hipErrorInvalidValue;
+        hipLaunchKernelGGL(pi_group_bf16x3_ws2_kernel, dim3(grid), dim3(512), 0, st, inst_dev,
+                           off_dev, ninst, nf, total);
+    } else if (precision == 2 && variant == 3)
         hipLaunchKernelGGL(pi_group_bf16x3_m32_kernel, dim3(grid), dim3(256), 0, st, inst_dev,
                            off_dev, ninst, nf, total);
     else if (precision == 2 && variant == 2)
@@ -1344,13 +1386,15 @@ hipError_t launch_pi_group(int precision, const PiInst* inst_dev, const int* off
     return hipGetLastError();
 }
 
-int pi_group_variant_count(int precision) { return precision == 2 ? 4 : 1; }
+int pi_group_variant_count(int precision) { return precision == 2 ? 5 : 1; }
 
 int pi_group_occupancy(int precision, int variant) {
     int n = 0;
     hipError_t e;
     if (precision == 0)
         e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, pi_group_f64_kernel, 256, 0);
+    else if (variant == 4)
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, pi_group_bf16x3_ws2_kernel, 512, 0);
     else if (variant == 3)
         e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, pi_group_bf16x3_m32_kernel, 256, 0);
     else if (variant == 2)

@@ -1724,8 +1724,13 @@ void enqueue_batch(qx_handle h, BatchPlan& P, bool direct) {
                                      P.pi_grid * pi_group_occupancy(prec, var), st, var));
         };
         if (direct) {
-            w.pi_variant = choose_variant(h, "pi_group", prec, pi_group_variant_count(prec),
-                                          {n, w.pi_total, P.inst[w.b].kpad}, pi_launch);
+            // the two-frequency instance (4) needs an even frequency count
+            w.pi_variant =
+                nf % 2 ? choose_variant(h, "pi_group", prec, pi_group_variant_count(prec),
+                                        {n, w.pi_total, P.inst[w.b].kpad}, pi_launch,
+                                        {0, 1, 2, 3})
+                       : choose_variant(h, "pi_group", prec, pi_group_variant_count(prec),
+                                        {n, w.pi_total, P.inst[w.b].kpad}, pi_launch);
             w.pi_grid = P.pi_grid * pi_group_occupancy(prec, w.pi_variant);
         }
         phase(QX_PHASE_PI, [&] {

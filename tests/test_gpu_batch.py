@@ -122,10 +122,10 @@ def test_batch_waves_and_fallback(gpu, monkeypatch):
         _close(f"waves inst {q}", got[q]["qp_energies"], ref[q]["qp_energies"], 1e-10)
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2, 3])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4])
 def test_batch_every_grouped_pi_variant(gpu, variant, monkeypatch):
     """Each grouped BF16x3 Pi kernel instance (K32, K64, double-buffered K32, K64 on 32x32x16
-    MFMAs), forced."""
+    MFMAs, producer/consumer with frequency pairs), forced."""
     import torch
     from quasix_amd import gw
     inst = _instances(torch, SHAPES[:4], seed0=100)
@@ -136,3 +136,17 @@ def test_batch_every_grouped_pi_variant(gpu, variant, monkeypatch):
     assert bs.stats.pi_variant == variant
     for q, (a, b) in enumerate(zip(bs.results(), r64)):
         _close(f"grouped v{variant} inst {q}", a["qp_energies"], b["qp_energies"], MEV / 100)
+
+
+def test_batch_odd_frequency_count(gpu):
+    """An odd number of frequencies: the autotuner must leave out the frequency-pair
+    instance, and the mixed batch still agrees with FP64 within 1 meV."""
+    import torch
+    from quasix_amd import gw
+    inst = _instances(torch, SHAPES[:3], seed0=200)
+    r64 = gw.g0w0_batch(inst, nfreq=15, precision="fp64")
+    bs = gw.G0W0BatchSolver(inst, nfreq=15, precision="bf16x3")
+    bs.run()
+    assert bs.stats.pi_variant != 4
+    for q, (a, b) in enumerate(zip(bs.results(), r64)):
+        _close(f"odd nf inst {q}", a["qp_energies"], b["qp_energies"], MEV)
