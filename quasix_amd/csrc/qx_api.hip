@@ -397,10 +397,10 @@ int tune_nf(int naux_or_npad, int nf) {
     return std::max(1, std::min(nf, ceil_div(4096, std::max(tiles, 1))));
 }
 
-// Autotuner candidates of the BF16x3 Pi kernels: the instances that win at some measured
-// shape (K32, K64, and both on 32x32x16 MFMAs); the others stay selectable by
-// QXGW_PI_VARIANT.
-#define QX_PI_CANDIDATES {0, 1, 5, 6, 7, 8, 9, 10}
+// Autotuner candidates of the BF16x3 Pi kernels: the instances that win or come within a few
+// per cent at the measured shapes (the producer/consumer forms, K64 on 32x32x16 MFMAs, K64 on
+// 16x16x32); the others stay selectable by QXGW_PI_VARIANT.
+#define QX_PI_CANDIDATES {8, 5, 7, 9, 1}
 
 // Side streams and the group count for the factorisation step loop (QXGW_POTRF_GROUPS,
 // default 3; 1 runs everything on the handle's stream).
