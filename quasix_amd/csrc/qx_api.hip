@@ -389,15 +389,15 @@ int choose_variant(qx_handle h, const char* kind, int precision, int nvariants,
     return best;
 }
 
-// Frequencies used when timing a Pi variant: all of them up to 16384 work items (tiles x
-// frequencies; a few launch-quantisation rounds of the chip), an even subset of that size
-// beyond, so a large problem is not tuned at full cost.  Even, because the frequency-pair
-// instances would otherwise time a duplicated frequency.
+// Frequencies used when timing a Pi variant: all of them up to 8192 work items (tiles x
+// frequencies), beyond that an even subset of about 4096 items (the chip filled several
+// times over), so a large problem is not tuned at full cost.  Even, because the
+// frequency-pair instances would otherwise time a duplicated frequency.
 int tune_nf(int naux_or_npad, int nf) {
     const int T = ceil_div(naux_or_npad, 128);
     const int tiles = std::max(T * (T + 1) / 2, 1);
-    if (tiles * nf <= 16384) return nf;
-    const int n = ceil_div(16384, tiles);
+    if (tiles * nf <= 8192) return nf;
+    const int n = ceil_div(4096, tiles);
     return std::max(1, std::min(nf, n + (n & 1)));
 }
 
