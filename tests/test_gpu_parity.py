@@ -175,6 +175,25 @@ def test_pi_bf16x3_every_variant(gpu, variant, monkeypatch):
         _close(f"pi bf16x3 v{variant} k={k}", p0[k], ref, 3e-5, np.max(np.abs(ref)))
 
 
+@pytest.mark.parametrize("variant", [1, 5, 7, 8, 9, 10])
+@pytest.mark.parametrize("nocc,nvir", [(4, 15), (4, 45)])
+def test_pi_bf16x3_odd_chunk_counts(gpu, variant, nocc, nvir, monkeypatch):
+    """N_ov padded to 64 and 192: one and three K64 chunks (the producer loops of the
+    double-buffered instances advance two chunks per pass), with full and partial tiles, and
+    an odd frequency count (the frequency-pair instance duplicates the last one)."""
+    from quasix_amd import core
+    monkeypatch.setenv("QXGW_PI_VARIANT", str(variant))
+    naux = 300
+    e, B = O.make_inputs(nocc, nvir, naux, seed=31)
+    fr, _ = O.get_scaled_legendre_roots(5, 0.5)
+    occ = np.array([2.0] * nocc + [0.0] * nvir)
+    p0 = core.compute_polarizability_p0(B.reshape(nocc, nvir, naux), e, occ, fr,
+                                        precision="bf16x3")
+    for k, w in enumerate(fr):
+        ref = O.get_rho_response(w, e, B, nocc)
+        _close(f"pi v{variant} nov {nocc * nvir} k={k}", p0[k], ref, 3e-5, np.max(np.abs(ref)))
+
+
 @pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10])
 def test_g0w0_bf16x3_every_variant(gpu, variant, monkeypatch):
     from quasix_amd import gw
