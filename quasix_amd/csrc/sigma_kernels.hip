@@ -20,6 +20,7 @@
 #include "qx_newton.h"
 
 #include <algorithm>
+#include <cstdlib>
 
 namespace qx {
 namespace {
@@ -354,6 +355,135 @@ __global__ __launch_bounds__(256, 2) void wnm_bf16x3_kernel(
                 const int j = j0 + wr * 64 + bi * 16 + mfma_row_f32(lane, v);
                 if (j < npair) wpart[((size_t)f * nblk + pb) * npair + j] = rs[v];
             }
+    }
+}
+
+// Two frequencies (f0, f1) per workgroup on 32x32x16 MFMAs: the pair-row operand Bp of a
+// chunk is loaded and staged once for both, which cuts the operand traffic per MFMA by a
+// quarter.  f1 == f0 (odd frequency count) computes one twice and writes it once.
+__global__ __launch_bounds__(256, 2) void wnm_bf16x3_f2_kernel(
+    const __bf16* __restrict__ bh, const __bf16* __restrict__ bl, const __bf16* __restrict__ mh,
+    const __bf16* __restrict__ ml, const double* __restrict__ bp, int npair, int npad, int nf,
+    double* __restrict__ wpart) {
+    typedef float f16v __attribute__((ext_vector_type(16)));
+    __shared__ __attribute__((aligned(16))) __bf16 Ah[WT][WKP];
+    __shared__ __attribute__((aligned(16))) __bf16 Al[WT][WKP];
+    __shared__ __attribute__((aligned(16))) __bf16 Mh[2][WT][WKP];
+    __shared__ __attribute__((aligned(16))) __bf16 Ml[2][WT][WKP];
+    const int j0 = blockIdx.x * WT, P0 = blockIdx.y * WT;
+    const int f0 = 2 * blockIdx.z, f1 = min(f0 + 1, nf - 1);
+    const int nblk = npad / NB;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wr = wave >> 1, wc = wave & 1, r32 = lane & 31;
+    const int row = tid >> 1, kq0 = (tid & 1) * 16;
+    const bool vm = P0 + row < npad;
+    const __bf16* pah = bh + (size_t)(j0 + row) * npad + kq0;
+    const __bf16* pal = bl + (size_t)(j0 + row) * npad + kq0;
+    const size_t moff = (size_t)(P0 + row) * npad + kq0;
+    const size_t fstride[2] = {(size_t)f0 * npad * npad, (size_t)f1 * npad * npad};
+    bf16x8 rah[2], ral[2], rmh[2][2], rml[2][2];
+    const bf16x8 z8 = bf16x8{0, 0, 0, 0, 0, 0, 0, 0};
+    auto fetch = [&](int k0) {
+        #pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            rah[q] = *reinterpret_cast<const bf16x8*>(pah + k0 + 8 * q);
+            ral[q] = *reinterpret_cast<const bf16x8*>(pal + k0 + 8 * q);
+            #pragma unroll
+            for (int fi = 0; fi < 2; ++fi) {
+                const size_t o = fstride[fi] + moff + k0 + 8 * q;
+                rmh[fi][q] = vm ? *reinterpret_cast<const bf16x8*>(mh + o) : z8;
+                rml[fi][q] = vm ? *reinterpret_cast<const bf16x8*>(ml + o) : z8;
+            }
+        }
+    };
+    f16v acc[2][2][2];
+    #pragma unroll
+    for (int fi = 0; fi < 2; ++fi)
+        #pragma unroll
+        for (int bi = 0; bi < 2; ++bi)
+            #pragma unroll
+            for (int bj = 0; bj < 2; ++bj)
+                #pragma unroll
+                for (int v = 0; v < 16; ++v) acc[fi][bi][bj][v] = 0.f;
+    const int kend = P0 + WT < npad ? P0 + WT : npad;
+    fetch(0);
+    for (int k0 = 0; k0 < kend; k0 += WK) {
+        #pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            *reinterpret_cast<bf16x8*>(&Ah[row][kq0 + 8 * q]) = rah[q];
+            *reinterpret_cast<bf16x8*>(&Al[row][kq0 + 8 * q]) = ral[q];
+            #pragma unroll
+            for (int fi = 0; fi < 2; ++fi) {
+                *reinterpret_cast<bf16x8*>(&Mh[fi][row][kq0 + 8 * q]) = rmh[fi][q];
+                *reinterpret_cast<bf16x8*>(&Ml[fi][row][kq0 + 8 * q]) = rml[fi][q];
+            }
+        }
+        __syncthreads();
+        if (k0 + WK < kend) fetch(k0 + WK);
+        #pragma unroll
+        for (int ks = 0; ks < WK; ks += 16) {
+            const int kq = ks + (lane >> 5) * 8;
+            bf16x8 ah[2], al[2];
+            #pragma unroll
+            for (int q = 0; q < 2; ++q) {
+                ah[q] = *reinterpret_cast<const bf16x8*>(&Ah[wr * 64 + q * 32 + r32][kq]);
+                al[q] = *reinterpret_cast<const bf16x8*>(&Al[wr * 64 + q * 32 + r32][kq]);
+            }
+            #pragma unroll
+            for (int fi = 0; fi < 2; ++fi) {
+                bf16x8 xh[2], xl[2];
+                #pragma unroll
+                for (int q = 0; q < 2; ++q) {
+                    xh[q] = *reinterpret_cast<const bf16x8*>(&Mh[fi][wc * 64 + q * 32 + r32][kq]);
+                    xl[q] = *reinterpret_cast<const bf16x8*>(&Ml[fi][wc * 64 + q * 32 + r32][kq]);
+                }
+                #pragma unroll
+                for (int bi = 0; bi < 2; ++bi)
+                    #pragma unroll
+                    for (int bj = 0; bj < 2; ++bj)
+                        acc[fi][bi][bj] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[bi], xh[bj], acc[fi][bi][bj], 0, 0, 0);
+                #pragma unroll
+                for (int bi = 0; bi < 2; ++bi)
+                    #pragma unroll
+                    for (int bj = 0; bj < 2; ++bj)
+                        acc[fi][bi][bj] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[bi], xl[bj], acc[fi][bi][bj], 0, 0, 0);
+                #pragma unroll
+                for (int bi = 0; bi < 2; ++bi)
+                    #pragma unroll
+                    for (int bj = 0; bj < 2; ++bj)
+                        acc[fi][bi][bj] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[bi], xh[bj], acc[fi][bi][bj], 0, 0, 0);
+            }
+        }
+        __syncthreads();
+    }
+    const int pb = P0 / NB + wc;
+    #pragma unroll
+    for (int fi = 0; fi < 2; ++fi) {
+        if (fi == 1 && f1 == f0) break;
+        const int f = fi ? f1 : f0;
+        #pragma unroll
+        for (int bi = 0; bi < 2; ++bi) {
+            double rs[16];
+            #pragma unroll
+            for (int v = 0; v < 16; ++v) {
+                const int j = j0 + wr * 64 + bi * 32 + mfma_row_32(lane, v);
+                rs[v] = 0.0;
+                #pragma unroll
+                for (int bj = 0; bj < 2; ++bj) {
+                    const int Pc = P0 + wc * 64 + bj * 32 + r32;
+                    const double y = (double)acc[fi][bi][bj][v];
+                    const double b = (j < npair && Pc < npad) ? bp[(size_t)j * npad + Pc] : 0.0;
+                    rs[v] += 2.0 * b * y + y * y;
+                }
+                for (int off = 1; off < 32; off <<= 1) rs[v] += __shfl_xor(rs[v], off, 64);
+            }
+            if (r32 == 0 && pb < nblk)
+                #pragma unroll
+                for (int v = 0; v < 16; ++v) {
+                    const int j = j0 + wr * 64 + bi * 32 + mfma_row_32(lane, v);
+                    if (j < npair) wpart[((size_t)f * nblk + pb) * npair + j] = rs[v];
+                }
+        }
     }
 }
 
@@ -744,9 +874,18 @@ hipError_t launch_wnm(int precision, const double* bpairs, const double* x, int 
                                npad, mixed->bh, mixed->bl);
         hipLaunchKernelGGL(split_m_kernel, dim3(8192), dim3(256), 0, st, x, npad, nf, mixed->mh,
                            mixed->ml);
-        dim3 grid(npp / WT, ceil_div(npad, WT), nf);
-        hipLaunchKernelGGL(wnm_bf16x3_kernel, grid, dim3(256), 0, st, mixed->bh, mixed->bl,
-                           mixed->mh, mixed->ml, bpairs, npair, npad, wpart);
+        // QXGW_WNM_F2=0 selects the one-frequency kernel
+        const char* fv = std::getenv("QXGW_WNM_F2");
+        const bool f2 = !(fv && std::atoi(fv) == 0);
+        if (f2) {
+            dim3 grid(npp / WT, ceil_div(npad, WT), (nf + 1) / 2);
+            hipLaunchKernelGGL(wnm_bf16x3_f2_kernel, grid, dim3(256), 0, st, mixed->bh,
+                               mixed->bl, mixed->mh, mixed->ml, bpairs, npair, npad, nf, wpart);
+        } else {
+            dim3 grid(npp / WT, ceil_div(npad, WT), nf);
+            hipLaunchKernelGGL(wnm_bf16x3_kernel, grid, dim3(256), 0, st, mixed->bh, mixed->bl,
+                               mixed->mh, mixed->ml, bpairs, npair, npad, wpart);
+        }
     } else {
         dim3 grid(ceil_div(npair, NB), nblk, nf);
         hipLaunchKernelGGL(wnm_linv_kernel, grid, dim3(256), 0, st, bpairs, x, npair, npad, wpart);

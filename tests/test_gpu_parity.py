@@ -143,6 +143,25 @@ def test_g0w0_c3_fp64_and_mixed(gpu):
         assert dE <= 1.0 * MEV
 
 
+@pytest.mark.parametrize("f2", ["0", "1"])
+@pytest.mark.parametrize("nf", [7, 8])
+def test_wnm_bf16x3_both_kernels(gpu, f2, nf, monkeypatch):
+    """The one- and two-frequency mixed Wc_nm kernels (QXGW_WNM_F2), with an odd frequency
+    count (the two-frequency kernel computes the last one twice and writes it once) and
+    pair/auxiliary counts off the 128 tiles."""
+    from quasix_amd import core
+    monkeypatch.setenv("QXGW_WNM_F2", f2)
+    nocc, nvir, naux = 7, 45, 200
+    e, B = O.make_inputs(nocc, nvir, naux, seed=9)
+    targets = O.frontier_targets(nocc, nocc + nvir, 3, 3)
+    mn = O.assemble_mnP(B, *O.ov_only_blocks(B, nocc, nvir), nocc, nvir, naux)
+    pt, pm, Bp = O.pair_list(mn, targets)
+    fr, _ = O.get_scaled_legendre_roots(nf, 0.5)
+    ref = O.wmn_imag(e, B, nocc, Bp, fr)
+    got = core.compute_wmn_imag(e, B, nocc, Bp, fr, precision="bf16x3")
+    _close(f"wnm bf16x3 f2={f2} nf={nf}", got, ref, 2e-5, np.max(np.abs(ref)))
+
+
 @pytest.mark.parametrize("prec,tol", [("fp32", 2e-5), ("bf16x3", 2e-5)])
 def test_wnm_mixed_precision(gpu, prec, tol):
     """Wc_nm through the mixed kernels against the FP64 oracle (ragged naux = 150)."""
