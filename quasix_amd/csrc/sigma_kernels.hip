@@ -572,10 +572,11 @@ __global__ __launch_bounds__(64) void sigma_imag_eval_kernel(SigmaProblem p,
         eval_imag<5>(p, t, E, cnt, s);
         if ((threadIdx.x & 63) == 0)
             #pragma unroll
-            for (int q = 0; q < cnt; ++q) {
-                out[((size_t)t * nq + q0 + q) * 2] = s[q].re;
-                out[((size_t)t * nq + q0 + q) * 2 + 1] = s[q].im;
-            }
+            for (int q = 0; q < 5; ++q)
+                if (q < cnt) {
+                    out[((size_t)t * nq + q0 + q) * 2] = s[q].re;
+                    out[((size_t)t * nq + q0 + q) * 2 + 1] = s[q].im;
+                }
     }
 }
 
