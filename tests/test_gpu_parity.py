@@ -65,6 +65,25 @@ def test_screened_wc(gpu):
         _close(f"wc k={k}", wc[k], ref, 1e-11, max(1.0, np.max(np.abs(ref))))
 
 
+def test_screened_wc_five_blocks(gpu):
+    """Five 64-blocks (n = 300, padded to 320) exercise every column-update tile kind (the
+    Cholesky NT tiles and the inverse NN sums) with a ragged last block."""
+    from quasix_amd import core
+    rng = np.random.default_rng(21)
+    n = 300
+    pis = []
+    for lo in (-3.0, -0.2):
+        Q = np.linalg.qr(rng.standard_normal((n, n)))[0]
+        pi = (Q * rng.uniform(lo, 0.0, n)) @ Q.T
+        pis.append(0.5 * (pi + pi.T))
+    pis = np.stack(pis)
+    wc, nfb = core.compute_screening_wc(pis)
+    assert nfb == 0
+    for k in range(pis.shape[0]):
+        ref = O.screened_wc(pis[k])
+        _close(f"wc n300 k={k}", wc[k], ref, 1e-11, max(1.0, np.max(np.abs(ref))))
+
+
 def test_screened_wc_lu_fallback(gpu):
     """A Pi with an eigenvalue above 1 makes eps indefinite: Cholesky fails, LU is used."""
     from quasix_amd import core
