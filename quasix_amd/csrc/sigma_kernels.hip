@@ -529,16 +529,19 @@ __device__ int eval_imag(const SigmaProblem& p, int t, const double (&E)[NQ], in
         const double em = p.eps_g[m];
         const double c = -p.eta * (en_ref > p.eps_ref[m] ? 1.0 : -1.0);
         #pragma unroll
-        for (int q = 0; q < nq; ++q) {
+        for (int q = 0; q < NQ; ++q) {
             const double Eq = E[q];
-            if ((Eq > p.ef && p.ef < em && em < Eq) || (Eq <= p.ef && Eq < em && em < p.ef))
+            if (q < nq &&
+                ((Eq > p.ef && p.ef < em && em < Eq) || (Eq <= p.ef && Eq < em && em < p.ef)))
                 pole = 1;
         }
+        // all NQ slots are evaluated (slots past nq hold a placeholder energy and are
+        // never returned), so the accumulators stay in registers
         for (int k = 0; k < p.nf; ++k) {
             const double w = p.wnm[(size_t)k * p.npair + j];
             const double wk = p.wts[k], w2 = p.freqs[k] * p.freqs[k];
             #pragma unroll
-            for (int q = 0; q < nq; ++q) {
+            for (int q = 0; q < NQ; ++q) {
                 const double a = E[q] - em;
                 // emo = a + i c ; den = emo^2 + w2 ; g = wk emo / den
                 const double dre = a * a - c * c + w2, dim = 2.0 * a * c;
@@ -551,10 +554,11 @@ __device__ int eval_imag(const SigmaProblem& p, int t, const double (&E)[NQ], in
         }
     }
     #pragma unroll
-    for (int q = 0; q < nq; ++q) {
-        out[q].re = -wave_sum(sre[q]) / kPi;
-        out[q].im = -wave_sum(sim[q]) / kPi;
-    }
+    for (int q = 0; q < NQ; ++q)
+        if (q < nq) {
+            out[q].re = -wave_sum(sre[q]) / kPi;
+            out[q].im = -wave_sum(sim[q]) / kPi;
+        }
     for (int off = 32; off > 0; off >>= 1) pole |= __shfl_xor(pole, off, 64);
     return pole;
 }
@@ -637,7 +641,8 @@ struct DeviceEval {
         for (int q = 0; q < 5; ++q) Ev[q] = q < nq ? E[q] : 0.0;
         const int pole = eval_imag<5>(*p, t, Ev, nq, s);
         #pragma unroll
-        for (int q = 0; q < nq; ++q) out[q] = s[q];
+        for (int q = 0; q < 5; ++q)
+            if (q < nq) out[q] = s[q];
         return pole != 0;
     }
 };
