@@ -9,7 +9,12 @@
 //     inverse (long K, pipelined); a one-workgroup kernel factors and inverts A_jj; one
 //     launch forms the panel L_ij = A_ij Linv_jj^T and X_jk = -Linv_jj S_jk.  Only block
 //     column j of A and block row j of X are written per step, so the HBM traffic is the
-//     left-looking reads, not a rewrite of the trailing matrix every step;
+//     left-looking reads, not a rewrite of the trailing matrix every step.  The single-
+//     instance path runs the steps in pairs (j, j + 1): one wide update of both block
+//     columns (and both block rows of X) over m < j reads the operand rows once for two
+//     steps, and a K = 64 update adds block column j to column j + 1 after panel j; the
+//     diagonal factor runs inside the update that finishes its block (first-dispatched
+//     workgroups), so a step needs no separate diagonal launch;
 //   * Gauss-Jordan with partial pivoting (one workgroup per matrix) for any frequency
 //     whose Cholesky hit a non-positive pivot; a singular matrix gives Wc = 0.
 // The downstream contraction needs only X (Wc_nm = |X b|^2 - |b|^2), so DPOTRI's
@@ -17,6 +22,7 @@
 #include "qx_internal.h"
 
 #include <algorithm>
+#include <cstdlib>
 
 namespace qx {
 namespace {
@@ -324,92 +330,180 @@ constexpr int GK = 32;
 __device__ __forceinline__ int swz_rk(int row, int k) { return row * GK + (k ^ (4 * (row & 7))); }
 __device__ __forceinline__ int swz_kc(int k, int c) { return k * NB + (c ^ (16 * (k & 1))); }
 
-// As, Bs: NB * GK doubles of LDS each.  Chol: Cholesky tile (NT product) or inverse sum (NN);
-// the choice is uniform per workgroup, so each instance carries only its own addressing.
-// Staging uses 16-byte loads: a thread moves 4 pairs of consecutive doubles per operand,
-// and the swizzles keep each pair adjacent and 16-byte aligned in LDS.
-template <bool Chol>
+// Chol: Cholesky tile (NT product) or inverse sum (NN); the choice is uniform per
+// workgroup, so each instance carries only its own addressing.  Wide: the tile spans two
+// block columns j, j + 1 (Chol: 64 x 128 output, B operand L rows of both blocks) or two
+// block rows j, j + 1 of X (NN: 128 x 64 output, A operand L rows of both blocks), so the
+// operand that is unique to the tile is read once for two steps.  kmin: the contraction
+// starts at max(kbeg, kmin); an NN tile whose K range was cut this way adds to the partial
+// sum already stored (the rest of its range came from the wide update before it).
+// LDS: As = (Wide NN ? 2 : 1) * NB * GK doubles, then Bs = (Wide Chol ? 2 : 1) * NB * GK.
+// Staging uses 16-byte loads: a thread moves 4 pairs of consecutive doubles per 64 operand
+// rows, and the swizzles keep each pair adjacent and 16-byte aligned in LDS.
+template <bool Chol, bool Wide>
 __device__ __forceinline__ void colupdate_body(double* __restrict__ a, double* __restrict__ x,
-                                               int npad, int j, int bx, int f, double* As,
-                                               double* Bs) {
+                                               int npad, int j, int bx, int f, int kmin,
+                                               double* S) {
+    constexpr int RA = (Wide && !Chol) ? 2 : 1;    // 64-row groups of the output / A operand
+    constexpr int RB = (Wide && Chol) ? 2 : 1;     // 64-column groups of the output / B operand
+    double* As = S;
+    double* Bs = S + RA * NB * GK;
     const int nblk = npad / NB;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     double* Af = a + (size_t)f * npad * npad;
     double* Xf = x + (size_t)f * npad * npad;
     const int i = Chol ? j + bx : j;                 // output block row
     const int k = Chol ? j : bx - (nblk - j);        // output block column
-    const int kbeg = Chol ? 0 : k * NB, kend = j * NB;
-    // A operand rows: L[i-block][kbeg:kend]; B operand: NT -> L[j-block][kbeg:kend] (rows),
-    // NN -> X[kbeg:kend][k-block].
+    const int kbeg = Chol ? kmin : max(k * NB, kmin), kend = j * NB;
+    const bool accum = !Chol && k * NB < kmin;
+    if (Chol && kbeg >= kend) return;
+    // A operand rows: L[i-block(s)][kbeg:kend]; B operand: NT -> L[j-block(s)][kbeg:kend]
+    // (rows), NN -> X[kbeg:kend][k-block].
     const double* arow = Af + (size_t)i * NB * npad;
     const double* brow = Chol ? Af + (size_t)j * NB * npad : Xf + (size_t)k * NB;
     // [row][k] staging: pair p = tid + 256 q -> row p / 16, k 2 (p % 16)
     // [k][col] staging (NN B): pair p -> k p / 32, col 2 (p % 32)
     const int rr0 = tid >> 4, kk2 = (tid & 15) * 2;
     const int kr0 = tid >> 5, cc2 = (tid & 31) * 2;
-    double ax[4], ay[4], bxv[4], byv[4];     // pairs held as scalars (kept in registers)
+    constexpr int NPA = 4 * RA, NPB = 4 * RB;
+    double ax[NPA], ay[NPA], bxv[NPB], byv[NPB];   // pairs held as scalars (kept in registers)
     auto fetch = [&](int k0) {
         #pragma unroll
-        for (int q = 0; q < 4; ++q) {
+        for (int q = 0; q < NPA; ++q) {
             const double2 va =
                 *reinterpret_cast<const double2*>(arow + (size_t)(rr0 + 16 * q) * npad + k0 + kk2);
+            ax[q] = va.x;
+            ay[q] = va.y;
+        }
+        #pragma unroll
+        for (int q = 0; q < NPB; ++q) {
             const double2 vb =
                 Chol ? *reinterpret_cast<const double2*>(brow + (size_t)(rr0 + 16 * q) * npad + k0 + kk2)
                      : *reinterpret_cast<const double2*>(brow + (size_t)(k0 + kr0 + 8 * q) * npad + cc2);
-            ax[q] = va.x;
-            ay[q] = va.y;
             bxv[q] = vb.x;
             byv[q] = vb.y;
         }
     };
-    d4 acc[4] = {d4{0, 0, 0, 0}, d4{0, 0, 0, 0}, d4{0, 0, 0, 0}, d4{0, 0, 0, 0}};
+    d4 acc[RA][4 * RB];
+    #pragma unroll
+    for (int ra = 0; ra < RA; ++ra)
+        #pragma unroll
+        for (int c = 0; c < 4 * RB; ++c) acc[ra][c] = d4{0, 0, 0, 0};
     if (kbeg < kend) fetch(kbeg);
     for (int k0 = kbeg; k0 < kend; k0 += GK) {
         #pragma unroll
-        for (int q = 0; q < 4; ++q) {
+        for (int q = 0; q < NPA; ++q)
             *reinterpret_cast<double2*>(As + swz_rk(rr0 + 16 * q, kk2)) = double2{ax[q], ay[q]};
+        #pragma unroll
+        for (int q = 0; q < NPB; ++q)
             *reinterpret_cast<double2*>(Bs + (Chol ? swz_rk(rr0 + 16 * q, kk2)
                                                    : swz_kc(kr0 + 8 * q, cc2))) =
                 double2{bxv[q], byv[q]};
-        }
         __syncthreads();
         if (k0 + GK < kend) fetch(k0 + GK);
-        const int rr = wave * 16 + (lane & 15);
         #pragma unroll
         for (int kk = 0; kk < GK; kk += 4) {
             const int q = kk + (lane >> 4);
-            const double av = As[swz_rk(rr, q)];
+            double av[RA];
             #pragma unroll
-            for (int bj = 0; bj < 4; ++bj) {
-                const int c = bj * 16 + (lane & 15);
+            for (int ra = 0; ra < RA; ++ra) av[ra] = As[swz_rk(ra * NB + wave * 16 + (lane & 15), q)];
+            #pragma unroll
+            for (int cb = 0; cb < 4 * RB; ++cb) {
+                const int c = cb * 16 + (lane & 15);
                 const double bv = Bs[Chol ? swz_rk(c, q) : swz_kc(q, c)];
-                acc[bj] = mfma64(av, bv, acc[bj]);
+                #pragma unroll
+                for (int ra = 0; ra < RA; ++ra) acc[ra][cb] = mfma64(av[ra], bv, acc[ra][cb]);
             }
         }
         __syncthreads();
     }
-    if (Chol) {
-        if (j == 0) return;
-        double* C = Af + (size_t)i * NB * npad + (size_t)j * NB;
-        for_acc(acc, wave, lane, [&](int r, int c, double v) { C[(size_t)r * npad + c] -= v; });
-    } else {
-        double* C = Xf + (size_t)j * NB * npad + (size_t)k * NB;
-        for_acc(acc, wave, lane, [&](int r, int c, double v) { C[(size_t)r * npad + c] = v; });
-    }
+    double* C = Chol ? Af + (size_t)i * NB * npad + (size_t)j * NB
+                     : Xf + (size_t)j * NB * npad + (size_t)k * NB;
+    #pragma unroll
+    for (int ra = 0; ra < RA; ++ra)
+        #pragma unroll
+        for (int rb = 0; rb < RB; ++rb) {
+            if (Chol && rb == 1 && i == j) continue;     // block (j, j + 1) is above the diagonal
+            const d4 t[4] = {acc[ra][4 * rb], acc[ra][4 * rb + 1], acc[ra][4 * rb + 2],
+                             acc[ra][4 * rb + 3]};
+            double* Cs = C + (size_t)ra * NB * npad + rb * NB;
+            for_acc(t, wave, lane, [&](int r, int c, double v) {
+                double& o = Cs[(size_t)r * npad + c];
+                if (Chol) o -= v;
+                else if (accum) o += v;
+                else o = v;
+            });
+        }
 }
 
 __device__ __forceinline__ void colupdate_tile(double* a, double* x, int npad, int j, int bx,
-                                               int f, double* As, double* Bs) {
-    if (bx < npad / NB - j) colupdate_body<true>(a, x, npad, j, bx, f, As, Bs);
-    else colupdate_body<false>(a, x, npad, j, bx, f, As, Bs);
+                                               int f, int kmin, double* S) {
+    if (bx < npad / NB - j) colupdate_body<true, false>(a, x, npad, j, bx, f, kmin, S);
+    else colupdate_body<false, false>(a, x, npad, j, bx, f, kmin, S);
+}
+
+// Diagonal factor fused into the update that finishes the diagonal block: with linv_diag
+// set, the workgroups of the diagonal tiles (bx = 0, one per frequency) are dispatched
+// first and, once their tile is written, factor and invert it (diag_body) while the other
+// tiles are still being updated, so the step needs no separate diagonal launch.
+constexpr int DIAG_LDS = 4 * NB + 16 * LPR * 4 + NB;     // P, Lp, idg of diag_body (doubles)
+
+__device__ __forceinline__ void diag_first(int nblk, int& bx, int& f) {
+    const int w = blockIdx.x + gridDim.x * blockIdx.y, nfg = gridDim.y;
+    if (w < nfg) {
+        bx = 0;
+        f = w;
+    } else {
+        const int r = w - nfg;
+        f = r / (nblk - 1);
+        bx = 1 + r % (nblk - 1);
+    }
+}
+
+__device__ __forceinline__ void fused_diag(double* a, double* x, double* linv_diag, int npad,
+                                           int j, int* fail, int f, double* S) {
+    __threadfence();                          // the tile's stores, before any thread reads it
+    __syncthreads();
+    diag_body(a, x, linv_diag, npad, j, fail, f, S, S + 4 * NB, S + 4 * NB + 16 * LPR * 4);
 }
 
 __global__ __launch_bounds__(256, 3) void potrf_colupdate_kernel(double* __restrict__ a,
                                                               double* __restrict__ x, int npad,
-                                                              int j) {
-    __shared__ __attribute__((aligned(16))) double As[NB * GK];
-    __shared__ __attribute__((aligned(16))) double Bs[NB * GK];
-    colupdate_tile(a, x, npad, j, blockIdx.x, blockIdx.y, As, Bs);
+                                                              int j, int kmin,
+                                                              double* __restrict__ linv_diag,
+                                                              int* __restrict__ fail) {
+    __shared__ __attribute__((aligned(16))) double S[2 * NB * GK > DIAG_LDS ? 2 * NB * GK : DIAG_LDS];
+    int bx = blockIdx.x, f = blockIdx.y;
+    if (linv_diag) diag_first(npad / NB, bx, f);
+    colupdate_tile(a, x, npad, j, bx, f, kmin, S);
+    if (linv_diag && bx == 0) fused_diag(a, x, linv_diag, npad, j, fail, f, S);
+}
+
+// Wide update for the step pair (j, j + 1): block columns j and j + 1 of A (tiles i >= j)
+// and block rows j and j + 1 of the inverse sums (k < j), all over m < j.
+__global__ __launch_bounds__(256, 2) void potrf_colupdate2_kernel(double* __restrict__ a,
+                                                               double* __restrict__ x, int npad,
+                                                               int j,
+                                                               double* __restrict__ linv_diag,
+                                                               int* __restrict__ fail) {
+    __shared__ __attribute__((aligned(16))) double S[3 * NB * GK > DIAG_LDS ? 3 * NB * GK : DIAG_LDS];
+    int bx = blockIdx.x, f = blockIdx.y;
+    if (linv_diag) diag_first(npad / NB, bx, f);
+    if (bx < npad / NB - j) colupdate_body<true, true>(a, x, npad, j, bx, f, 0, S);
+    else colupdate_body<false, true>(a, x, npad, j, bx, f, 0, S);
+    if (linv_diag && bx == 0) fused_diag(a, x, linv_diag, npad, j, fail, f, S);
+}
+
+// QXGW_POTRF_FUSE=0: separate diagonal launches.
+bool potrf_fuse_diag() {
+    const char* v = std::getenv("QXGW_POTRF_FUSE");
+    return !(v && std::atoi(v) == 0);
+}
+
+// QXGW_POTRF_PAIR=0: one full column update per step instead of step pairs.
+bool potrf_pair_steps() {
+    const char* v = std::getenv("QXGW_POTRF_PAIR");
+    return !(v && std::atoi(v) == 0);
 }
 
 // ---- Grouped forms: one launch per step covers every active instance of a batch --------
@@ -418,13 +512,12 @@ __global__ __launch_bounds__(256, 3) void potrf_colupdate_kernel(double* __restr
 // whose per-instance grid is (items per frequency) x nf.
 __global__ __launch_bounds__(256, 3) void potrf_colupdate_group_kernel(
     const PotrfInst* __restrict__ inst, const int* __restrict__ off, int nact, int j) {
-    __shared__ __attribute__((aligned(16))) double As[NB * GK];
-    __shared__ __attribute__((aligned(16))) double Bs[NB * GK];
+    __shared__ __attribute__((aligned(16))) double S[2 * NB * GK];
     const int w = blockIdx.x;
     const int q = find_inst(off, nact, w);
     const PotrfInst in = inst[q];
     const int nblk = in.npad / NB, local = w - off[q];
-    colupdate_tile(in.a, in.x, in.npad, j, local % nblk, local / nblk, As, Bs);
+    colupdate_tile(in.a, in.x, in.npad, j, local % nblk, local / nblk, 0, S);
 }
 
 __global__ __launch_bounds__(256) void potrf_diag_group_kernel(const PotrfInst* __restrict__ inst,
@@ -579,21 +672,37 @@ hipError_t launch_potrf_batched(double* a, double* x, double* linv_diag, int npa
         if (e != hipSuccess) return e;
     }
     const size_t mat = (size_t)npad * npad, dblk = (size_t)nblk * NB * NB;
-    for (int j = 0; j < nblk; ++j) {
+    // Step pairs (j, j + 1): one wide update of both block columns over m < j, the diagonal
+    // and panel kernels of j, a K = 64 update of column j + 1 by block column j (kmin = 64 j),
+    // then the diagonal and panel kernels of j + 1.  A last odd step runs alone.
+    // With fusion the diagonal factor of every step but the first runs inside its update.
+    const bool pairs = potrf_pair_steps(), fuse = potrf_fuse_diag();
+    for (int j = 0; j < nblk;) {
+        const bool pair = pairs && j + 1 < nblk;
         for (int g = 0; g < ng; ++g) {
             const int f0 = g * nf / ng, nfg = (g + 1) * nf / ng - f0;
             double* ag = a + f0 * mat;
             double* xg = x + f0 * mat;
             double* dg = linv_diag + f0 * dblk;
-            if (j > 0)
-                hipLaunchKernelGGL(potrf_colupdate_kernel, dim3(nblk, nfg), dim3(256), 0, sg[g],
-                                   ag, xg, npad, j);
-            hipLaunchKernelGGL(potrf_diag_kernel, dim3(nfg), dim3(256), 0, sg[g], ag, xg, dg,
-                               npad, j, fail + f0);
-            if (nblk > 1)
-                hipLaunchKernelGGL(potrf_rowpanel_kernel, dim3(nblk - 1, nfg), dim3(256), 0,
-                                   sg[g], ag, xg, dg, npad, j);
+            for (int s = 0; s < (pair ? 2 : 1); ++s) {
+                const int js = j + s;
+                double* dfuse = fuse && js > 0 ? dg : nullptr;
+                if (pair && s == 0 && js > 0)
+                    hipLaunchKernelGGL(potrf_colupdate2_kernel, dim3(nblk, nfg), dim3(256), 0,
+                                       sg[g], ag, xg, npad, js, dfuse, fail + f0);
+                else if (js > 0)
+                    hipLaunchKernelGGL(potrf_colupdate_kernel, dim3(nblk, nfg), dim3(256), 0,
+                                       sg[g], ag, xg, npad, js, pair ? j * NB : 0, dfuse,
+                                       fail + f0);
+                if (!dfuse)
+                    hipLaunchKernelGGL(potrf_diag_kernel, dim3(nfg), dim3(256), 0, sg[g], ag,
+                                       xg, dg, npad, js, fail + f0);
+                if (nblk > 1)
+                    hipLaunchKernelGGL(potrf_rowpanel_kernel, dim3(nblk - 1, nfg), dim3(256), 0,
+                                       sg[g], ag, xg, dg, npad, js);
+            }
         }
+        j += pair ? 2 : 1;
     }
     for (int g = 1; g < ng; ++g) {
         hipError_t e = hipEventRecord(fan->join[g - 1], sg[g]);

@@ -65,12 +65,18 @@ def test_screened_wc(gpu):
         _close(f"wc k={k}", wc[k], ref, 1e-11, max(1.0, np.max(np.abs(ref))))
 
 
-def test_screened_wc_five_blocks(gpu):
-    """Five 64-blocks (n = 300, padded to 320) exercise every column-update tile kind (the
-    Cholesky NT tiles and the inverse NN sums) with a ragged last block."""
+@pytest.mark.parametrize("pair,fuse,n", [("1", "1", 300), ("0", "1", 300), ("1", "0", 300),
+                                         ("0", "0", 300), ("1", "1", 250)])
+def test_screened_wc_multi_block(gpu, pair, fuse, n, monkeypatch):
+    """Five (n = 300, padded to 320) or four (n = 250) 64-blocks exercise every column-update
+    tile kind (Cholesky NT tiles, inverse NN sums) with a ragged last block, in step pairs
+    (wide update of two block columns, then the K = 64 update; an odd block count ends with a
+    single step) or single steps (QXGW_POTRF_PAIR=0), with the diagonal factor fused into
+    the update or launched on its own (QXGW_POTRF_FUSE=0)."""
     from quasix_amd import core
+    monkeypatch.setenv("QXGW_POTRF_PAIR", pair)
+    monkeypatch.setenv("QXGW_POTRF_FUSE", fuse)
     rng = np.random.default_rng(21)
-    n = 300
     pis = []
     for lo in (-3.0, -0.2):
         Q = np.linalg.qr(rng.standard_normal((n, n)))[0]
@@ -81,7 +87,7 @@ def test_screened_wc_five_blocks(gpu):
     assert nfb == 0
     for k in range(pis.shape[0]):
         ref = O.screened_wc(pis[k])
-        _close(f"wc n300 k={k}", wc[k], ref, 1e-11, max(1.0, np.max(np.abs(ref))))
+        _close(f"wc n={n} pair={pair} fuse={fuse} k={k}", wc[k], ref, 1e-11, max(1.0, np.max(np.abs(ref))))
 
 
 def test_screened_wc_lu_fallback(gpu):
