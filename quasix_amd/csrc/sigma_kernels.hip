@@ -512,9 +512,11 @@ __global__ __launch_bounds__(256) void exchange_diag_kernel(const double* __rest
 // ---- CD quadrature ------------------------------------------------------------------------
 
 
-// I_n(E_q) for nq energies of target t (whole wavefront); returns 1 if any E_q has a
-// residue pole among the target's partners.
-template <int NQ>
+// I_n(E_q) for nq energies of target t (whole workgroup of NW wavefronts: the pairs are
+// spread over all its lanes); returns 1 if any E_q has a residue pole among the target's
+// partners.  Every thread returns the same values (the NW partial sums are added in one
+// fixed order), so callers may branch on them uniformly.
+template <int NQ, int NW = 1>
 __device__ int eval_imag(const SigmaProblem& p, int t, const double (&E)[NQ], int nq,
                          C2 (&out)[NQ]) {
     const int lane = threadIdx.x & 63;
@@ -524,7 +526,7 @@ __device__ int eval_imag(const SigmaProblem& p, int t, const double (&E)[NQ], in
     #pragma unroll
     for (int q = 0; q < NQ; ++q) { sre[q] = 0.0; sim[q] = 0.0; }
     int pole = 0;
-    for (int j = p.pair_off[t] + lane; j < p.pair_off[t + 1]; j += 64) {
+    for (int j = p.pair_off[t] + (int)threadIdx.x; j < p.pair_off[t + 1]; j += 64 * NW) {
         const int m = p.pair_m[j];
         const double em = p.eps_g[m];
         const double c = -p.eta * (en_ref > p.eps_ref[m] ? 1.0 : -1.0);
@@ -553,14 +555,46 @@ __device__ int eval_imag(const SigmaProblem& p, int t, const double (&E)[NQ], in
             }
         }
     }
-    #pragma unroll
-    for (int q = 0; q < NQ; ++q)
-        if (q < nq) {
-            out[q].re = -wave_sum(sre[q]) / kPi;
-            out[q].im = -wave_sum(sim[q]) / kPi;
-        }
     for (int off = 32; off > 0; off >>= 1) pole |= __shfl_xor(pole, off, 64);
-    return pole;
+    if (NW == 1) {
+        #pragma unroll
+        for (int q = 0; q < NQ; ++q)
+            if (q < nq) {
+                out[q].re = -wave_sum(sre[q]) / kPi;
+                out[q].im = -wave_sum(sim[q]) / kPi;
+            }
+        return pole;
+    }
+    __shared__ double red[NW][2 * NQ + 1];
+    const int w = threadIdx.x >> 6;
+    #pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        const double r = wave_sum(sre[q]), i = wave_sum(sim[q]);
+        if (lane == 0) {
+            red[w][2 * q] = r;
+            red[w][2 * q + 1] = i;
+        }
+    }
+    if (lane == 0) red[w][2 * NQ] = pole;
+    __syncthreads();
+    #pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        double r = 0.0, i = 0.0;
+        #pragma unroll
+        for (int v = 0; v < NW; ++v) {
+            r += red[v][2 * q];
+            i += red[v][2 * q + 1];
+        }
+        if (q < nq) {
+            out[q].re = -r / kPi;
+            out[q].im = -i / kPi;
+        }
+    }
+    int any = 0;
+    #pragma unroll
+    for (int v = 0; v < NW; ++v) any |= red[v][2 * NQ] != 0.0;
+    __syncthreads();                          // red is reused by the next call
+    return any;
 }
 
 __global__ __launch_bounds__(64) void sigma_imag_eval_kernel(SigmaProblem p,
@@ -631,6 +665,7 @@ __global__ __launch_bounds__(64) void sigma_ac_kernel(SigmaProblem p,
     }
 }
 
+template <int NW = 1>
 struct DeviceEval {
     const SigmaProblem* p;
     int t;
@@ -639,7 +674,7 @@ struct DeviceEval {
         C2 s[5];
         #pragma unroll
         for (int q = 0; q < 5; ++q) Ev[q] = q < nq ? E[q] : 0.0;
-        const int pole = eval_imag<5>(*p, t, Ev, nq, s);
+        const int pole = eval_imag<5, NW>(*p, t, Ev, nq, s);
         #pragma unroll
         for (int q = 0; q < 5; ++q)
             if (q < nq) out[q] = s[q];
@@ -647,20 +682,22 @@ struct DeviceEval {
     }
 };
 
-__global__ __launch_bounds__(64) void newton_cd_kernel(SigmaProblem p, NewtonParams np,
-                                                       const double* __restrict__ guess,
-                                                       double* e_out, double* z_out,
-                                                       double* sig_re, double* sig_im,
-                                                       int* state, int* iters) {
+// One workgroup of NEWTON_NW wavefronts per target: every thread runs the same Newton
+// iteration on the same reduced values, the pair sums are split over all lanes.
+constexpr int NEWTON_NW = 4;
+
+__global__ __launch_bounds__(64 * NEWTON_NW) void newton_cd_kernel(
+    SigmaProblem p, NewtonParams np, const double* __restrict__ guess, double* e_out,
+    double* z_out, double* sig_re, double* sig_im, int* state, int* iters) {
     const int t = blockIdx.x;
     const int n = p.targets[t];
-    DeviceEval ev{&p, t};
+    DeviceEval<NEWTON_NW> ev{&p, t};
     double E = 0.0, Z = 0.0;
     C2 s = {0.0, 0.0};
     int it = 0;
     const int stt = newton_core(ev, np, p.eps_ref[n], p.static_terms[t], p.ef, guess != nullptr,
                                 guess ? guess[t] : 0.0, E, Z, s, it);
-    if ((threadIdx.x & 63) == 0) {
+    if (threadIdx.x == 0) {
         e_out[t] = E;
         z_out[t] = Z;
         sig_re[t] = s.re;
@@ -685,7 +722,7 @@ __global__ __launch_bounds__(64) void newton_group_kernel(const NewtonInst* __re
     p.ef = 0.5 * (p.eps_g[p.nocc - 1] + p.eps_g[p.nocc]);
     const int t = g - t_off[q], o = inst[q].out_off + t;
     const int n = p.targets[t];
-    DeviceEval ev{&p, t};
+    DeviceEval<> ev{&p, t};
     double E = 0.0, Z = 0.0;
     C2 s = {0.0, 0.0};
     int it = 0;
@@ -937,7 +974,7 @@ hipError_t launch_newton_cd(const SigmaProblem& p, const NewtonParams& np, const
                             double* e_out, double* z_out, double* sig_re, double* sig_im,
                             int* state, int* iters, hipStream_t st) {
     if (p.nt <= 0) return hipSuccess;
-    hipLaunchKernelGGL(newton_cd_kernel, dim3(p.nt), dim3(64), 0, st, p, np, guess, e_out, z_out,
+    hipLaunchKernelGGL(newton_cd_kernel, dim3(p.nt), dim3(64 * NEWTON_NW), 0, st, p, np, guess, e_out, z_out,
                        sig_re, sig_im, state, iters);
     return hipGetLastError();
 }
