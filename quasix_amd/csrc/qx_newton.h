@@ -1,5 +1,6 @@
-// Newton-Raphson quasiparticle solve, shared by the device kernel (one wavefront per
-// target) and the host-driven path used when residue poles appear.
+// Newton-Raphson quasiparticle solve, shared by the device kernels (a workgroup per target:
+// every thread runs the same iteration on the same reduced values) and the host-driven path
+// used when residue poles appear.
 //
 // Restates quasix_core/src/qp/newton_solver.rs:174-391:
 //   f(E) = E - eps_n - static - Re Sigma_c(E)      (static = Sigma_x - V_xc)
