@@ -361,6 +361,9 @@ __global__ __launch_bounds__(256, 2) void wnm_bf16x3_kernel(
 // Two frequencies (f0, f1) per workgroup on 32x32x16 MFMAs: the pair-row operand Bp of a
 // chunk is loaded and staged once for both, which cuts the operand traffic per MFMA by a
 // quarter.  f1 == f0 (odd frequency count) computes one twice and writes it once.
+// Grid (pair tiles, frequency pairs, auxiliary row tiles): the row tile P0 is the slowest
+// index, taken from the last one down, because a tile's K range (0 .. P0 + 128) grows with
+// P0: the longest workgroups are dispatched first and the short ones fill the tail.
 __global__ __launch_bounds__(256, 2) void wnm_bf16x3_f2_kernel(
     const __bf16* __restrict__ bh, const __bf16* __restrict__ bl, const __bf16* __restrict__ mh,
     const __bf16* __restrict__ ml, const double* __restrict__ bp, int npair, int npad, int nf,
@@ -370,8 +373,8 @@ __global__ __launch_bounds__(256, 2) void wnm_bf16x3_f2_kernel(
     __shared__ __attribute__((aligned(16))) __bf16 Al[WT][WKP];
     __shared__ __attribute__((aligned(16))) __bf16 Mh[2][WT][WKP];
     __shared__ __attribute__((aligned(16))) __bf16 Ml[2][WT][WKP];
-    const int j0 = blockIdx.x * WT, P0 = blockIdx.y * WT;
-    const int f0 = 2 * blockIdx.z, f1 = min(f0 + 1, nf - 1);
+    const int j0 = blockIdx.x * WT, P0 = (gridDim.z - 1 - blockIdx.z) * WT;
+    const int f0 = 2 * blockIdx.y, f1 = min(f0 + 1, nf - 1);
     const int nblk = npad / NB;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wr = wave >> 1, wc = wave & 1, r32 = lane & 31;
@@ -921,7 +924,7 @@ hipError_t launch_wnm(int precision, const double* bpairs, const double* x, int 
         const char* fv = std::getenv("QXGW_WNM_F2");
         const bool f2 = !(fv && std::atoi(fv) == 0);
         if (f2) {
-            dim3 grid(npp / WT, ceil_div(npad, WT), (nf + 1) / 2);
+            dim3 grid(npp / WT, (nf + 1) / 2, ceil_div(npad, WT));
             hipLaunchKernelGGL(wnm_bf16x3_f2_kernel, grid, dim3(256), 0, st, mixed->bh,
                                mixed->bl, mixed->mh, mixed->ml, bpairs, npair, npad, nf, wpart);
         } else {
