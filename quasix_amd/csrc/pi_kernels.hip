@@ -38,6 +38,20 @@ __device__ __forceinline__ int xcd_remap(int bid, int n) {
     return xcd < rem ? xcd * (per + 1) + slot : rem * (per + 1) + (xcd - rem) * per + slot;
 }
 
+// Tile order for the XCD ranges: logical tile w enumerates the tiles t = x, x + 8, x + 16, ...
+// of residue class x = 0..7 one class after another, so each XCD's contiguous range holds
+// every 8th tile of the triangle (a fair share of the cheaper diagonal and edge tiles)
+// instead of one band of rows.
+__device__ __forceinline__ int tile_interleave(int w, int ntiles) {
+    int x = 0, start = 0;
+    for (; x < 7; ++x) {
+        const int c = (ntiles - x + 7) / 8;
+        if (w < start + c) break;
+        start += c;
+    }
+    return x + 8 * (w - start);
+}
+
 __device__ __forceinline__ void tri_index(int t, int& I, int& J) {
     // t enumerates lower-triangle tiles (I >= J) row by row.
     int i = (int)((sqrtf(8.0f * (float)t + 1.0f) - 1.0f) * 0.5f);
@@ -846,7 +860,8 @@ __global__ __launch_bounds__(512, 1) void pi_syrk_bf16x3_ws2_kernel(
     __shared__ __attribute__((aligned(16))) __bf16 L[2 * WS2BUF];
     const int nfp = (nf + 1) / 2;
     const int work = xcd_remap(blockIdx.x, ntiles * nfp);
-    const int tile = work / nfp, f0 = 2 * (work % nfp), f1 = min(f0 + 1, nf - 1);
+    const int tile = tile_interleave(work / nfp, ntiles), f0 = 2 * (work % nfp);
+    const int f1 = min(f0 + 1, nf - 1);
     int I, J;
     tri_index(tile, I, J);
     if (tile_full(I * TILE, J * TILE, naux))
