@@ -18,6 +18,8 @@
 // or above the diagonal are skipped.
 #include "qx_internal.h"
 
+#include <cstdlib>
+
 #include <algorithm>
 
 namespace qx {
@@ -739,7 +741,9 @@ constexpr int WS2P = PiLds<WS2K>::pitch;
 constexpr int WS2A = TILE * WS2P;                 // bf16 per array
 constexpr int WS2BUF = 6 * WS2A;                  // bf16 per buffer
 
-template <bool Guarded>
+// NFR = 1: one frequency only (f1 is ignored): the second frequency's split, staging and
+// MFMAs are skipped (tail items, see pi_syrk_bf16x3_ws2_kernel).
+template <bool Guarded, int NFR = 2>
 __device__ __forceinline__ void pi_tile_bf16x3_ws2_body(
     const uint16_t* __restrict__ bt_hi, const uint16_t* __restrict__ bt_lo,
     const float* __restrict__ bt_f32, const float* __restrict__ s32, int kpad, int naux,
@@ -777,7 +781,7 @@ __device__ __forceinline__ void pi_tile_bf16x3_ws2_body(
                     al[q] = *reinterpret_cast<const bf16x8*>(&arr(b, 1)[wr * 64 + q * 32 + r32][kq]);
                 }
                 #pragma unroll
-                for (int fi = 0; fi < 2; ++fi) {
+                for (int fi = 0; fi < NFR; ++fi) {
                     bf16x8 ch[2], cl[2];
                     #pragma unroll
                     for (int q = 0; q < 2; ++q) {
@@ -790,7 +794,8 @@ __device__ __forceinline__ void pi_tile_bf16x3_ws2_body(
             __syncthreads();                      // buffer i & 1 free, chunk i + 1 staged
         }
         store_tile32(acc[0], I0, J0, wr, wc, lane, npad, naux, f0, mode, out);
-        if (f1 != f0) store_tile32(acc[1], I0, J0, wr, wc, lane, npad, naux, f1, mode, out);
+        if (NFR == 2 && f1 != f0)
+            store_tile32(acc[1], I0, J0, wr, wc, lane, npad, naux, f1, mode, out);
     } else {
         // ---- producer: two register sets, chunk c loaded into set c % 2 two iterations
         // before it is staged ----
@@ -826,7 +831,7 @@ __device__ __forceinline__ void pi_tile_bf16x3_ws2_body(
                 *reinterpret_cast<u16x8*>(&arr(buf, 0)[row][kq0]) = rh[set][q];
                 *reinterpret_cast<u16x8*>(&arr(buf, 1)[row][kq0]) = rl[set][q];
                 #pragma unroll
-                for (int fi = 0; fi < 2; ++fi)
+                for (int fi = 0; fi < NFR; ++fi)
                     #pragma unroll
                     for (int hq = 0; hq < 2; ++hq) {
                         bf16x4 h, l;
@@ -853,23 +858,46 @@ __device__ __forceinline__ void pi_tile_bf16x3_ws2_body(
     }
 }
 
+// Items are (tile, frequency pair), tile-major.  The first nfull items run as pairs; the
+// last ntiles * nfp - nfull items (the part of the last round of workgroups that would leave
+// CUs idle) run as two single-frequency workgroups each, dispatched last, so the final round
+// is shorter.  Grid: nfull + 2 (ntiles * nfp - nfull).
 __global__ __launch_bounds__(512, 1) void pi_syrk_bf16x3_ws2_kernel(
     const uint16_t* __restrict__ bt_hi, const uint16_t* __restrict__ bt_lo,
     const float* __restrict__ bt_f32, const float* __restrict__ s32, int kpad, int naux,
-    int npad, int nf, int ntiles, double* __restrict__ out, PiOut mode) {
+    int npad, int nf, int ntiles, int nfull, double* __restrict__ out, PiOut mode) {
     __shared__ __attribute__((aligned(16))) __bf16 L[2 * WS2BUF];
     const int nfp = (nf + 1) / 2;
-    const int work = xcd_remap(blockIdx.x, ntiles * nfp);
-    const int tile = tile_interleave(work / nfp, ntiles), f0 = 2 * (work % nfp);
+    const int bid = blockIdx.x;
+    const bool single = bid >= nfull;
+    int work, f0;
+    if (!single) {
+        work = xcd_remap(bid, nfull);
+        f0 = 2 * (work % nfp);
+    } else {
+        work = nfull + (bid - nfull) / 2;
+        f0 = 2 * (work % nfp) + (bid - nfull) % 2;
+        if (f0 >= nf) return;                     // odd nf: the pair's second slot is empty
+    }
+    const int tile = tile_interleave(work / nfp, ntiles);
     const int f1 = min(f0 + 1, nf - 1);
     int I, J;
     tri_index(tile, I, J);
-    if (tile_full(I * TILE, J * TILE, naux))
+    const bool full = tile_full(I * TILE, J * TILE, naux);
+    if (single) {
+        if (full)
+            pi_tile_bf16x3_ws2_body<false, 1>(bt_hi, bt_lo, bt_f32, s32, kpad, naux, npad, out,
+                                              mode, I * TILE, J * TILE, f0, f0, L);
+        else
+            pi_tile_bf16x3_ws2_body<true, 1>(bt_hi, bt_lo, bt_f32, s32, kpad, naux, npad, out,
+                                             mode, I * TILE, J * TILE, f0, f0, L);
+    } else if (full) {
         pi_tile_bf16x3_ws2_body<false>(bt_hi, bt_lo, bt_f32, s32, kpad, naux, npad, out, mode,
                                        I * TILE, J * TILE, f0, f1, L);
-    else
+    } else {
         pi_tile_bf16x3_ws2_body<true>(bt_hi, bt_lo, bt_f32, s32, kpad, naux, npad, out, mode,
                                       I * TILE, J * TILE, f0, f1, L);
+    }
 }
 
 // Double-buffered variants: two LDS buffers, one barrier per chunk.  While the MFMAs of
@@ -1303,6 +1331,20 @@ hipError_t launch_pi_prepare_mixed(const double* B, int nov, int naux, int npad,
     return hipGetLastError();
 }
 
+// Compute units of the current device (cached per device).
+static int device_cu_count() {
+    static int cached[64] = {};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 0;
+    if (cached[dev] == 0) {
+        int n = 0;
+        if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+            return 0;
+        cached[dev] = n;
+    }
+    return cached[dev];
+}
+
 hipError_t launch_pi_syrk(int precision, const double* B, const PiOperands& ops,
                           const double* s, int nov, int naux, int npad, int nf, double* out,
                           PiOut mode, hipStream_t st, int variant) {
@@ -1357,9 +1399,16 @@ hipError_t launch_pi_syrk(int precision, const double* B, const PiOperands& ops,
                                    ops.s32, ops.kpad, naux, npad, nf, ntiles, out, mode);
         } else if (variant == 8) {
             if (ops.kpad % WS2K) return hipErrorInvalidValue;
-            hipLaunchKernelGGL(pi_syrk_bf16x3_ws2_kernel, dim3(ntiles * ((nf + 1) / 2)), dim3(512),
-                               0, st, ops.bt_hi, ops.bt_lo, ops.bt_f32, ops.s32, ops.kpad, naux,
-                               npad, nf, ntiles, out, mode);
+            // one 120 KB workgroup per CU: the items past the last full round of CUs run as
+            // single-frequency halves when those fit in one round (QXGW_PI_TAIL=0: never)
+            const int nitems = ntiles * ((nf + 1) / 2), cus = device_cu_count();
+            const int ntail = cus > 0 ? nitems % cus : 0;
+            const char* tv = std::getenv("QXGW_PI_TAIL");
+            const bool tail = !(tv && std::atoi(tv) == 0) && ntail > 0 && 2 * ntail <= cus;
+            const int nfull = tail ? nitems - ntail : nitems;
+            hipLaunchKernelGGL(pi_syrk_bf16x3_ws2_kernel, dim3(nfull + 2 * (nitems - nfull)),
+                               dim3(512), 0, st, ops.bt_hi, ops.bt_lo, ops.bt_f32, ops.s32,
+                               ops.kpad, naux, npad, nf, ntiles, nfull, out, mode);
         }
         else if (variant == 1)
             hipLaunchKernelGGL(pi_syrk_bf16x3_kernel<64>, grid, block, 0, st, ops.bt_hi, ops.bt_lo,

@@ -238,6 +238,29 @@ def test_pi_bf16x3_odd_chunk_counts(gpu, variant, nocc, nvir, monkeypatch):
         _close(f"pi v{variant} nov {nocc * nvir} k={k}", p0[k], ref, 3e-5, np.max(np.abs(ref)))
 
 
+@pytest.mark.parametrize("naux,nf", [(300, 7), (1000, 16), (1000, 17)])
+def test_pi_pair_kernel_tail_items(gpu, naux, nf, monkeypatch):
+    """The frequency-pair instance runs the items past the last full round of CUs as
+    single-frequency workgroups (every item at naux 300 / 7 frequencies, 32 of 288 at
+    naux 1000 / 16, with the odd last pair at 17): the same MFMA sequence per frequency, so
+    Pi is bit-identical to the all-pairs launch (QXGW_PI_TAIL=0), and within 3e-5 of the
+    oracle."""
+    from quasix_amd import core
+    monkeypatch.setenv("QXGW_PI_VARIANT", "8")
+    nocc, nvir = 5, 40
+    e, B = O.make_inputs(nocc, nvir, naux, seed=41)
+    fr, _ = O.get_scaled_legendre_roots(nf, 0.5)
+    occ = np.array([2.0] * nocc + [0.0] * nvir)
+    args = (B.reshape(nocc, nvir, naux), e, occ, fr)
+    p_tail = core.compute_polarizability_p0(*args, precision="bf16x3")
+    monkeypatch.setenv("QXGW_PI_TAIL", "0")
+    p_pairs = core.compute_polarizability_p0(*args, precision="bf16x3")
+    _close(f"pi tail vs pairs naux {naux} nf {nf}", p_tail, p_pairs, 0.0)
+    for k in (0, nf - 1):
+        ref = O.get_rho_response(fr[k], e, B, nocc)
+        _close(f"pi tail naux {naux} k={k}", p_tail[k], ref, 3e-5, np.max(np.abs(ref)))
+
+
 @pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10])
 def test_g0w0_bf16x3_every_variant(gpu, variant, monkeypatch):
     from quasix_amd import gw
