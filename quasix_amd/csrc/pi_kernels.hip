@@ -40,18 +40,18 @@ __device__ __forceinline__ int xcd_remap(int bid, int n) {
     return xcd < rem ? xcd * (per + 1) + slot : rem * (per + 1) + (xcd - rem) * per + slot;
 }
 
-// Tile order for the XCD ranges: logical tile w enumerates the tiles t = x, x + 8, x + 16, ...
-// of residue class x = 0..7 one class after another, so each XCD's contiguous range holds
-// every 8th tile of the triangle (a fair share of the cheaper diagonal and edge tiles)
-// instead of one band of rows.
-__device__ __forceinline__ int tile_interleave(int w, int ntiles) {
-    int x = 0, start = 0;
-    for (; x < 7; ++x) {
-        const int c = (ntiles - x + 7) / 8;
-        if (w < start + c) break;
-        start += c;
+// Lower-triangle tiles (I >= J) of a T x T tile grid enumerated column by column (J outer,
+// I = J .. T - 1 inner): neighbours share the column operand rows J in L2, and the cheaper
+// diagonal (first of a column) and edge (last, I = T - 1) tiles are spread over the order,
+// so the contiguous range each XCD gets holds its share of them.
+__device__ __forceinline__ void tri_index_cols(int t, int T, int& I, int& J) {
+    int j = 0, start = 0;
+    while (j < T - 1 && t >= start + (T - j)) {
+        start += T - j;
+        ++j;
     }
-    return x + 8 * (w - start);
+    J = j;
+    I = j + (t - start);
 }
 
 __device__ __forceinline__ void tri_index(int t, int& I, int& J) {
@@ -879,10 +879,10 @@ __global__ __launch_bounds__(512, 1) void pi_syrk_bf16x3_ws2_kernel(
         f0 = 2 * (work % nfp) + (bid - nfull) % 2;
         if (f0 >= nf) return;                     // odd nf: the pair's second slot is empty
     }
-    const int tile = tile_interleave(work / nfp, ntiles);
+    const int T = (int)((sqrtf(8.0f * (float)ntiles + 1.0f) - 1.0f) * 0.5f + 0.5f);
     const int f1 = min(f0 + 1, nf - 1);
     int I, J;
-    tri_index(tile, I, J);
+    tri_index_cols(work / nfp, T, I, J);
     const bool full = tile_full(I * TILE, J * TILE, naux);
     if (single) {
         if (full)
