@@ -1124,7 +1124,7 @@ __device__ __forceinline__ bool group_pair_item(const PiInst* __restrict__ inst,
     const int local = w - off[q];
     f = local % nf;
     int I, J;
-    tri_index(local / nf, I, J);
+    tri_index_cols(local / nf, (in.npad + TILE - 1) / TILE, I, J);
     I0 = I * TILE;
     J0 = J * TILE;
     return true;
