@@ -35,10 +35,23 @@ Parity status per function (checks live in tests/test_oracle_*.py):
   wmn_imag, sigma_cd, residues                              pinned by Casida brute force
   get_sigma_diag, thiele, pade_thiele, ac_pade_thiele_diag  pinned (rational-function
       exactness, reference tests correlation_ac.rs:755-810)
-  qp_step_ac                                                pinned through its parts
-      (pade_thiele, NewtonQPSolver) and the closed-form linearised step
-  evgw_ac                                                   parity unpinned beyond its
-      parts (as evgw_cd)
+  qp_step_ac                                                pinned (Pade model of a known
+      rational Sigma: the Newton root solves the QP equation with the mean-field Fermi
+      level and not with the current one; linearised step in closed form,
+      tests/test_oracle_evgw_pins.py)
+  evgw_ac, evgw_cd, convergence_window                      pinned (closed-form evGW fixed
+      point of the one-occupied / one-virtual / one-auxiliary model, with and without the
+      mean-field Fermi-level shift of the AC QP step; fixed-point residuals of converged
+      runs evaluated independently of the driver loop; hand-counted frontier windows and a
+      case where the frontier and all-orbital stop rules part, tests/test_oracle_evgw_pins.py)
+  metric_sqrt, compute_screening_w                          pinned (scipy.linalg.sqrtm;
+      the Dyson identity W = v + v P0 W; identity metric gives (I - P0)^-1;
+      tests/test_oracle_stage.py)
+  sigma_c_pyscf_cd                                          pinned (equals the imaginary-axis
+      part of sigma_cd_eval at E = 0, itself pinned by Casida, when no residue pole lies
+      between the Fermi level and 0; zero P0 gives zero; tests/test_oracle_stage.py)
+  solve_qp_linearized                                       pinned (closed form, tests/
+      test_oracle_stage.py)
   NewtonQPSolver                                            pinned (reference tests
       newton_solver.rs:609-732 restated as known answers)
   transform_mo_3center                                      pinned (identity coefficients
@@ -50,8 +63,6 @@ Parity status per function (checks live in tests/test_oracle_*.py):
       zero-error case that the driver's call order produces, tests/test_convergence.py)
   g0w0_adjoint                                              pinned (central finite
       differences of g0w0_cd)
-  evgw_cd                                                   parity unpinned beyond its
-      parts (no reference evGW numbers exist without PySCF inputs)
 """
 from __future__ import annotations
 
@@ -68,7 +79,8 @@ __all__ = [
     "get_rho_response", "compute_polarizability_p0", "rho_response_complex",
     "screened_wc", "wmn_imag", "sigma_cd_eval", "NewtonQPSolver", "g0w0_cd",
     "evgw_cd", "get_sigma_diag", "thiele", "pade_thiele", "ac_pade_thiele_diag",
-    "linearized_qp_ac", "CONFIGS",
+    "linearized_qp_ac", "qp_step_ac", "evgw_ac", "convergence_window", "metric_sqrt",
+    "compute_screening_w", "sigma_c_pyscf_cd", "solve_qp_linearized", "CONFIGS",
 ]
 
 # --------------------------------------------------------------------------------------
@@ -269,6 +281,67 @@ def screened_wc(pi):
         except np.linalg.LinAlgError:
             inv = np.eye(n)
     return inv - np.eye(n)
+
+
+def metric_sqrt(v):
+    """Symmetric square root of the DF metric: eigendecomposition, eigenvalues floored at
+    1e-14, Q diag(sqrt(l)) Q^T, error if |vs vs^T - v|_max > 1e-10.
+    Restates quasix_core/src/dielectric/screening.rs:580-639."""
+    lam, Q = np.linalg.eigh(np.asarray(v, dtype=np.float64))
+    vs = (Q * np.sqrt(np.maximum(lam, 1e-14))) @ Q.T
+    err = np.max(np.abs(vs @ vs.T - v))
+    if err > 1e-10:
+        raise ArithmeticError(f"Eigendecomposition sqrt reconstruction error: {err:.3e}")
+    return vs
+
+
+def compute_screening_w(p0, metric, v_sqrt=None):
+    """W_f = vs eps_f^-1 vs^T with eps_f = sym(I - vs P0_f vs^T), Cholesky inverse (LU if it
+    fails), W symmetrised.  Restates quasix_core/src/dielectric/screening.rs:164-420."""
+    vs = metric_sqrt(metric) if v_sqrt is None else np.asarray(v_sqrt, dtype=np.float64)
+    p0 = np.asarray(p0, dtype=np.float64)
+    n = p0.shape[1]
+    out = np.empty_like(p0)
+    for f in range(p0.shape[0]):
+        m = (vs @ p0[f]) @ vs.T
+        eps = np.eye(n) - m
+        eps = 0.5 * (eps + eps.T)
+        try:
+            c = sla.cho_factor(eps, lower=True)
+            inv = sla.cho_solve(c, np.eye(n))
+        except (np.linalg.LinAlgError, sla.LinAlgError):
+            inv = np.linalg.inv(eps)
+        w = (vs @ inv) @ vs.T
+        out[f] = 0.5 * (w + w.T)
+    return out
+
+
+def sigma_c_pyscf_cd(mo_energy, eta, p0_iw, lpq, freqs, weights):
+    """Static CD self-energy of every orbital (no residues, E = 0).
+
+    Restates quasix_core/src/selfenergy/correlation_pyscf_cd.rs:60-298: Wc_f = inv(I - P0_f)
+    - I (general inverse); W_mn = sum_PQ lpq[P,n,m] Wc[P,Q] lpq[Q,m,n];
+    Sigma_n = -1/pi sum_f w_f sum_m W_mn emo / (emo^2 + w_f^2),
+    emo = -eps_m - i eta sign, sign = +1 if eps_n > eps_m else -1.
+    """
+    e = np.asarray(mo_energy, dtype=np.float64)
+    nmo = e.size
+    naux = p0_iw.shape[1]
+    sig = np.zeros(nmo, dtype=np.complex128)
+    for f, (w, wt) in enumerate(zip(freqs, weights)):
+        wc = np.linalg.inv(np.eye(naux) - p0_iw[f]) - np.eye(naux)
+        for n in range(nmo):
+            W = np.einsum("pm,pq,qm->m", lpq[:, n, :], wc, lpq[:, :, n])
+            sign = np.where(e[n] > e, 1.0, -1.0)
+            emo = -e - 1j * eta * sign
+            sig[n] += np.sum(W * wt * emo / (emo * emo + w * w))
+    return sig * (-1.0 / math.pi)
+
+
+def solve_qp_linearized(mo_energy, sigma_x, sigma_c, vxc):
+    """E = eps + Sigma_x + Sigma_c - V_xc with Z = 1 (quasix_core/src/qp/solver.rs:137-221)."""
+    e = np.asarray(mo_energy) + np.asarray(sigma_x) + np.asarray(sigma_c) - np.asarray(vxc)
+    return e, np.ones_like(e)
 
 
 def wmn_imag(mo_energy, B_ov, nocc, B_pair, freqs):
@@ -637,57 +710,116 @@ def evgw_step_mix(it, mon, e_old, z_old, e_new, z_new, damping, diis, z_min=0.1,
     return e, np.clip(z, z_min, z_max)
 
 
+def convergence_window(mo_energy, nocc, frontier_convergence=True, frontier_window=1.0):
+    """[start, end) of the orbitals whose change decides convergence.
+
+    Restates EvGWDriver::determine_convergence_window (quasix_core/src/qp/evgw.rs:1121-1160),
+    evaluated on the mean-field energies: every orbital when frontier_convergence is off;
+    otherwise the occupied orbitals found scanning down from the HOMO while within
+    frontier_window of it, and the virtual ones scanning up from the LUMO while within
+    frontier_window of it (HOMO and LUMO always included).
+    """
+    e = np.asarray(mo_energy, dtype=np.float64)
+    nmo = e.size
+    if not frontier_convergence:
+        return 0, nmo
+    start = max(nocc - 1, 0)
+    for i in range(nocc - 1, -1, -1):
+        if e[i] >= e[nocc - 1] - frontier_window:
+            start = i
+        else:
+            break
+    end = nocc + 1
+    for i in range(nocc, nmo):
+        if e[i] <= e[nocc] + frontier_window:
+            end = i + 1
+        else:
+            break
+    return start, end
+
+
+def _window_changes(new, old, window):
+    """(max, rms) of |new - old| over the window (evgw.rs:1002-1020)."""
+    lo, hi = window
+    d = np.abs(np.asarray(new) - np.asarray(old))[lo:hi]
+    return float(np.max(d)), float(np.sqrt(np.sum(d * d) / (hi - lo)))
+
+
 def evgw_cd(mo_energy, B_ov, nocc, targets, freqs, wts, *, n_iter=8, damping=0.5,
-            eta=0.01, tol=1e-4, static=None, diis=False, diis_space=6):
+            eta=0.01, tol=1e-4, static=None, diis=False, diis_space=6,
+            frontier_convergence=True, frontier_window=1.0, newton=None, pairs=None,
+            solver="newton"):
     """Eigenvalue-self-consistent GW on the CD self-energy.
 
-    Follows quasix_core/src/qp/evgw.rs:603-1027: iteration 0 is G0W0 on the mean-field
-    energies; iteration k > 0 rebuilds Pi, the Green's function poles and the residues
-    from the previous QP energies (:626-633); the (E, Z) update is ``evgw_step_mix``
-    (:832-927, damping and DIIS); Newton restarts from the previous QP energy (:742).
-    Orbitals outside ``targets`` keep their mean-field energies.  Convergence: max |dE| over
-    the targets below ``tol`` (:931-1027 with the frontier window equal to the target set).
+    Follows quasix_core/src/qp/evgw.rs:603-1027 with the AC self-energy replaced by the CD
+    one: iteration 0 is G0W0 on the mean-field energies; iteration k > 0 rebuilds Pi, the
+    Green's function poles and the residues from the previous QP energies (:626-633); the
+    (E, Z) update is ``evgw_step_mix`` (:832-927, damping and DIIS); Newton restarts from
+    the previous QP energy (:742).  Orbitals outside ``targets`` keep their mean-field
+    energies.  Stop rule (:929-1027): after every iteration, including the first, the max
+    |dE| over ``convergence_window`` of the mean-field energies is compared with ``tol``.
+    ``pairs`` = (pair_t, pair_m, B_pair) of the targets (default: the ov-only pair list);
+    ``solver`` "newton" or "linearized" (the linearised step of :786-820 at the mean-field
+    energy in iteration 0 and at the previous QP energy after it).
     """
     e0 = np.asarray(mo_energy, dtype=np.float64)
     targets = np.asarray(targets)
+    win = convergence_window(e0, nocc, frontier_convergence, frontier_window)
     cur = e0.copy()
     z = np.ones(len(targets))
     mon = DIISMonitor(diis_space)
     hist = []
     res = None
+    converged = False
     for it in range(n_iter):
         guesses = None if it == 0 else cur[targets]
-        res = _g0w0_iter(cur, e0, B_ov, nocc, targets, freqs, wts, static, eta, guesses)
+        res = _g0w0_iter(cur, e0, B_ov, nocc, targets, freqs, wts, static, eta, guesses,
+                         newton, pairs, solver)
         et, z = evgw_step_mix(it, mon, cur[targets], z, res["E"], res["Z"], damping, diis)
         new = cur.copy()
         new[targets] = et
-        dmax = float(np.max(np.abs(new[targets] - cur[targets])))
-        hist.append(dict(cycle=it, qp_energies=new.copy(), energy_change=dmax))
+        dmax, drms = _window_changes(new, cur, win)
+        zf = np.ones(e0.size)
+        zf[targets] = z
+        hist.append(dict(cycle=it, qp_energies=new.copy(), z_factors=zf, energy_change=dmax,
+                         rms_change=drms))
         cur = new
-        if it > 0 and dmax < tol:
+        if dmax < tol:
+            converged = True
             break
-    return dict(E=cur, Z=z, history=hist, last=res)
+    return dict(E=cur, Z=z, history=hist, last=res, converged=converged, n_cycles=len(hist),
+                window=win)
 
 
-def _g0w0_iter(cur, e0, B_ov, nocc, targets, freqs, wts, static, eta, guesses):
+def _g0w0_iter(cur, e0, B_ov, nocc, targets, freqs, wts, static, eta, guesses, newton=None,
+               pairs=None, solver="newton"):
     """One evGW step: Pi/poles from ``cur``; QP equation anchored at the mean-field e0."""
     nt = len(targets)
     static = np.zeros(nt) if static is None else np.asarray(static)
-    miP, abP = ov_only_blocks(B_ov, nocc, len(cur) - nocc)
-    mn = assemble_mnP(B_ov, miP, abP, nocc, len(cur) - nocc, B_ov.shape[1])
-    pt, pm, Bp = pair_list(mn, targets)
+    if pairs is None:
+        miP, abP = ov_only_blocks(B_ov, nocc, len(cur) - nocc)
+        mn = assemble_mnP(B_ov, miP, abP, nocc, len(cur) - nocc, B_ov.shape[1])
+        pairs = pair_list(mn, targets)
+    pt, pm, Bp = pairs
     wnm = wmn_imag(cur, B_ov, nocc, Bp, freqs)
     ctx = CDContext(cur, e0, nocc, B_ov, np.asarray(targets), pt, pm, Bp,
                     np.asarray(freqs), np.asarray(wts), wnm, eta)
     ef = ctx.ef
     E = np.empty(nt)
     Z = np.empty(nt)
+    S = np.empty(nt, dtype=np.complex128)
     for t, n in enumerate(targets):
         f = lambda om, t=t: sigma_cd_eval(ctx, t, om + ef)
+        if solver == "linearized":
+            ev = e0[n] if guesses is None else guesses[t]
+            s0, s1 = f(ev - ef), f(ev - ef + 1e-6).real
+            z = min(max(1.0 / (1.0 - (s1 - s0.real) / 1e-6), 0.1), 0.999)
+            E[t], Z[t], S[t] = e0[n] + z * (static[t] + s0.real), z, s0
+            continue
         g = linearized_guess(f, e0[n], static[t], ef) if guesses is None else guesses[t]
-        r = NewtonQPSolver(e0[n], static[t], 0.0, ef).solve(f, g)
-        E[t], Z[t] = r.qp_energy, r.z_factor
-    return dict(E=E, Z=Z, wnm=wnm)
+        r = NewtonQPSolver(e0[n], static[t], 0.0, ef, newton).solve(f, g)
+        E[t], Z[t], S[t] = r.qp_energy, r.z_factor, r.sigma_c
+    return dict(E=E, Z=Z, sigma=S, wnm=wnm)
 
 
 # --------------------------------------------------------------------------------------
@@ -806,7 +938,7 @@ def linearized_qp_ac(mo_energy, sigma_x, vxc, coeff, ofit, orbs, nocc, de=1e-6):
 
 
 def qp_step_ac(coeff, ofit, orbs, e_hf, nocc, sigma_x, vxc, solver, prev_qp, z_min=0.1,
-               z_max=0.999):
+               z_max=0.999, newton=None):
     """QP step of the evGW driver on the Pade model, quasix_core/src/qp/evgw.rs:705-820:
     linearised at prev_qp (or the mean-field energy at iteration 0) with a clamped forward-
     difference Z, or Newton from prev_qp / the linearised guess with the linearised value at
@@ -828,7 +960,7 @@ def qp_step_ac(coeff, ofit, orbs, e_hf, nocc, sigma_x, vxc, solver, prev_qp, z_m
             continue
         guess = lin(e[p])[0] if prev_qp is None else start
         try:
-            r = NewtonQPSolver(e[p], sigma_x[p], vxc[p], ef).solve(f, guess)
+            r = NewtonQPSolver(e[p], sigma_x[p], vxc[p], ef, newton).solve(f, guess)
             E[i], Z[i] = r.qp_energy, r.z_factor
         except RuntimeError:
             E[i], Z[i] = lin(guess)
@@ -837,35 +969,44 @@ def qp_step_ac(coeff, ofit, orbs, e_hf, nocc, sigma_x, vxc, solver, prev_qp, z_m
 
 def evgw_ac(mo_energy, lpq, nocc, orbs=None, *, nw=100, x0=0.5, iw_cutoff=5.0, n_iter=8,
             damping=0.5, tol=1e-4, diis=False, diis_space=6, solver="linearized",
-            sigma_x=None, vxc=None):
+            sigma_x=None, vxc=None, frontier_convergence=True, frontier_window=1.0,
+            newton=None):
     """evGW on the AC self-energy (the reference's live driver, quasix_core/src/qp/evgw.rs:
     420-1049): get_sigma_diag on the previous QP energies (:603-690), ac_pade_thiele_diag,
-    the QP step ``qp_step_ac`` and the mixing step ``evgw_step_mix``; ``orbs`` (default all)
-    are updated, the others keep their mean-field energies; stop when max |dE| over ``orbs``
-    falls below ``tol`` after iteration 0."""
+    the QP step ``qp_step_ac`` and the mixing step ``evgw_step_mix``; ``orbs`` (default all,
+    as the reference) are updated, others keep their mean-field energies.  Stop rule
+    (:929-1027): after every iteration, including the first, the max |dE| over
+    ``convergence_window`` of the mean-field energies below ``tol``; the history records
+    that max and the RMS over the same window, and each iteration's Z."""
     e0 = np.asarray(mo_energy, dtype=np.float64)
     nmo = e0.size
     orbs = np.arange(nmo) if orbs is None else np.asarray(orbs)
     sx = np.zeros(nmo) if sigma_x is None else np.asarray(sigma_x, dtype=np.float64)
     vx = np.zeros(nmo) if vxc is None else np.asarray(vxc, dtype=np.float64)
+    win = convergence_window(e0, nocc, frontier_convergence, frontier_window)
     cur = e0.copy()
     z = np.ones(len(orbs))
     mon = DIISMonitor(diis_space)
     hist = []
+    converged = False
     for it in range(n_iter):
         sig, om = get_sigma_diag(cur, lpq, orbs, nocc, nw, x0, iw_cutoff)
         coeff, ofit = ac_pade_thiele_diag(sig, om)
         prev = None if it == 0 else cur
-        En, Zn = qp_step_ac(coeff, ofit, orbs, e0, nocc, sx, vx, solver, prev)
+        En, Zn = qp_step_ac(coeff, ofit, orbs, e0, nocc, sx, vx, solver, prev, newton=newton)
         et, z = evgw_step_mix(it, mon, cur[orbs], z, En, Zn, damping, diis)
         new = cur.copy()
         new[orbs] = et
-        dmax = float(np.max(np.abs(new[orbs] - cur[orbs])))
-        hist.append(dict(cycle=it, qp_energies=new.copy(), energy_change=dmax))
+        dmax, drms = _window_changes(new, cur, win)
+        zf = np.ones(nmo)
+        zf[orbs] = z
+        hist.append(dict(cycle=it, qp_energies=new.copy(), z_factors=zf, energy_change=dmax,
+                         rms_change=drms))
         cur = new
-        if it > 0 and dmax < tol:
+        if dmax < tol:
+            converged = True
             break
-    return dict(E=cur, Z=z, history=hist)
+    return dict(E=cur, Z=z, history=hist, converged=converged, n_cycles=len(hist), window=win)
 
 
 # --------------------------------------------------------------------------------------

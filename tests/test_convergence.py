@@ -25,7 +25,7 @@ def _spd_border(n, rng):
     return bm
 
 
-@pytest.mark.parametrize("solve", [Q.solve_linear_system, O.diis_solve])
+@pytest.mark.parametrize("solve", [Q.pulay_solve, O.diis_solve])
 def test_solve_matches_numpy(solve):
     rng = np.random.default_rng(0)
     for n in (2, 3, 5, 7):
@@ -36,7 +36,7 @@ def test_solve_matches_numpy(solve):
         assert abs(x[:n].sum() - 1.0) < 1e-12
 
 
-@pytest.mark.parametrize("solve,exc", [(Q.solve_linear_system, Q.DIISError),
+@pytest.mark.parametrize("solve,exc", [(Q.pulay_solve, Q.DIISError),
                                        (O.diis_solve, ArithmeticError)])
 def test_solve_rejects_singular(solve, exc):
     bm = np.array([[0.0, 0.0, -1.0], [0.0, 0.0, -1.0], [-1.0, -1.0, 0.0]])
@@ -70,11 +70,11 @@ def test_diis_is_constrained_least_squares(impl):
     xs = _iterates(n, m)
     z = np.zeros(n)
     if impl == "product":
-        mon = Q.ConvergenceMonitor(diis_space=m)
-        mon.check_convergence(xs[0], z)
+        mon = Q.EvGWHistory(space=m)
+        mon.record(xs[0], z)
         for x in xs[1:]:
-            e, _ = mon.accelerate(x, z)          # error = x_k - x_{k-1}
-            mon.check_convergence(x, z)
+            e, _ = mon.extrapolate(x, z)         # error = x_k - x_{k-1}
+            mon.record(x, z)
     else:
         mon = O.DIISMonitor(space=m)
         mon.record(xs[0], z)
@@ -100,7 +100,7 @@ def test_driver_order_falls_back_to_damping_and_both_agree():
     seq = [(rng.uniform(-1, 1, 3), rng.uniform(0.6, 0.95, 3)) for _ in range(6)]
     prod = _drive(lambda it, m, e, z, en, zn, d: Q.evgw_mix(it, m, e, z, en, zn, d,
                                                             use_diis=True),
-                  lambda: Q.ConvergenceMonitor(diis_space=6), seq)
+                  lambda: Q.EvGWHistory(space=6), seq)
     orac = _drive(lambda it, m, e, z, en, zn, d: O.evgw_step_mix(it, m, e, z, en, zn, d, True),
                   lambda: O.DIISMonitor(6), seq)
     for (a, za), (b, zb) in zip(prod, orac):
@@ -110,3 +110,16 @@ def test_driver_order_falls_back_to_damping_and_both_agree():
     for it in range(3, 6):
         want = 0.5 * prod[it - 1][0] + 0.5 * seq[it][0]
         assert np.allclose(prod[it][0], want, atol=1e-15)
+
+
+def test_frontier_window_product_matches_oracle():
+    """The product's stop-rule window and the oracle's (hand-counted cases live in
+    tests/test_oracle_evgw_pins.py) on random spectra and windows."""
+    rng = np.random.default_rng(11)
+    for _ in range(200):
+        nocc, nvir = int(rng.integers(1, 8)), int(rng.integers(1, 8))
+        e = np.r_[np.sort(rng.uniform(-3, -0.1, nocc)), np.sort(rng.uniform(0.0, 3, nvir))]
+        w = float(rng.uniform(0.0, 2.5))
+        for fc in (True, False):
+            assert Q.frontier_window(e, nocc, fc, w) == O.convergence_window(e, nocc, fc, w)
+
