@@ -13,7 +13,10 @@
  *     host pointers and copy internally.
  *   - Row-major ("C order") arrays, FP64 unless stated.  B_ov is the reference's iaP block,
  *     [nocc*nvir][naux] (quasix/src/evgw.rs:105-114).
- *   - Status codes map one-to-one onto QuasixError (quasix_core/src/common/mod.rs:18-67);
+ *   - Status codes 1-5 carry the meaning of the QuasixError variants the path can raise
+ *     (InvalidInput, DimensionMismatch, NumericalError, ConvergenceError, NotImplemented;
+ *     quasix_core/src/common/mod.rs:15-70); the reference's other variants (I/O, HDF5,
+ *     basis-set, SCF, ...) belong to subsystems outside this path and have no code here.
  *     qx_last_error() returns the message of the last failure on the handle.
  *   - One handle per host thread; calls on one handle are serialised on its stream.
  */
@@ -26,7 +29,7 @@ extern "C" {
 
 #define QXGW_API __attribute__((visibility("default")))
 #define QXGW_VERSION_MAJOR 0
-#define QXGW_VERSION_MINOR 1
+#define QXGW_VERSION_MINOR 2
 
 typedef struct qx_context* qx_handle;
 
@@ -74,7 +77,13 @@ typedef struct {
     int ntarget;               /* number of target orbitals */
     int npair;                 /* number of (target, partner) rows of B_pair */
     qx_newton_config newton;
+    int qp_solver;             /* QX_SOLVER_NEWTON (default) or QX_SOLVER_LINEARIZED */
 } qx_g0w0_desc;
+
+/* QP solvers of qx_g0w0_desc.qp_solver (quasix_core/src/qp/evgw.rs:705-820). */
+#define QX_SOLVER_NEWTON 0      /* NewtonQPSolver::solve with the linearised fallback */
+#define QX_SOLVER_LINEARIZED 1  /* E = eps + Z (Sigma_x - V_xc + Re Sigma_c(E0)) at E0 = the
+                                   guess (else eps), forward-difference Z clamped */
 
 typedef struct {
     int chol_fallbacks;        /* frequencies whose Cholesky failed (Gauss-Jordan used) */
@@ -145,6 +154,18 @@ QXGW_API qx_status qx_rho_response_dev(qx_handle h, int precision, int nocc, int
 QXGW_API qx_status qx_screened_wc_dev(qx_handle h, int naux, int nfreq, const double* pi,
                                       double* wc_out, int* fallbacks);
 
+/* w_out[f] ([naux][naux], device) = vs (I - vs p0[f] vs^T)^-1 vs^T, symmetrised, with vs =
+ * v_sqrt when given, else the principal square root of the SPD Coulomb metric `metric`
+ * (Newton-Schulz on FP64 MFMA, accepted when |vs vs^T - v|_max <= 1e-10), else vs = I.
+ * eps = I - vs p0 vs^T is symmetrised and inverted by Cholesky, LU when that fails; a singular
+ * eps or a non-finite W is QX_ERR_NUMERICAL.  Replaces
+ * dielectric::screening::compute_screened_interaction{,_with_vsqrt} and
+ * compute_W_single_frequency (quasix_core/src/dielectric/screening.rs:164-420) behind the
+ * quasix_core.compute_screening_w stage function (quasix_core/src/python/g0w0.rs:102-130). */
+QXGW_API qx_status qx_screening_w_dev(qx_handle h, int naux, int nfreq, const double* p0,
+                                      const double* metric, const double* v_sqrt,
+                                      double* w_out);
+
 /* ---- A4: Wc_nm on the imaginary axis --------------------------------------------------- */
 /* wnm[f][j] = B_pair[j]^T Wc(i freqs[f]) B_pair[j] for every pair row; Pi from B_ov and
  * mo_energy.  Replaces the per-target contraction of correlation_ac::get_sigma_diag
@@ -166,6 +187,28 @@ QXGW_API qx_status qx_sigma_cd_dev(qx_handle h, const qx_g0w0_desc* d, const dou
                                    const int* pair_m, const int* targets, const double* freqs,
                                    const double* wts, const double* wnm, int nq,
                                    const double* energies, double* sigma);
+
+/* Static CD self-energy of every orbital at E = 0, without residues, from a given
+ * polarizability p0_iw [nfreq][naux][naux] and the full DF tensor lpq [naux][nmo][nmo]
+ * (device pointers): Wc_f = (I - p0_f)^-1 - I, W_nm = lpq[:, n, m]^T Wc_f lpq[:, m, n],
+ * sigma[n] = -1/pi sum_f w_f sum_m W_nm emo / (emo^2 + freqs_f^2), emo = -eps_m - i eta s,
+ * s = +1 if eps_n > eps_m else -1.  sigma: [nmo] complex (re, im), device.  A singular
+ * I - p0_f is QX_ERR_NUMERICAL.  Replaces selfenergy::correlation_pyscf_cd::
+ * compute_sigma_c_pyscf_cd (quasix_core/src/selfenergy/correlation_pyscf_cd.rs:60-298) and
+ * its stage function (quasix_core/src/python/g0w0.rs:597-631). */
+QXGW_API qx_status qx_sigma_c_pyscf_cd_dev(qx_handle h, int nmo, int naux, int nfreq, double eta,
+                                           const double* mo_energy, const double* p0_iw,
+                                           const double* lpq, const double* freqs,
+                                           const double* weights, double* sigma);
+
+/* Host-only linearised QP equation with Z = 1: qp = mo_energy + sigma_x + sigma_c - vxc,
+ * QX_ERR_NUMERICAL if a result is not finite.  Replaces qp::solver::
+ * solve_quasiparticle_linearized (quasix_core/src/qp/solver.rs:137-221) behind the
+ * quasix_core.solve_qp_equation stage function (quasix_core/src/python/g0w0.rs:423-450). */
+QXGW_API qx_status qx_solve_qp_linearized(int nmo, const double* mo_energy,
+                                          const double* sigma_x, const double* sigma_c,
+                                          const double* vxc, double* qp_energy,
+                                          double* z_factor);
 
 /* ---- A9/A10: fused G0W0 solve ---------------------------------------------------------- */
 /* One G0W0 solve: Pi at all frequencies -> FP64 Cholesky -> Wc_nm -> Newton on the CD

@@ -19,6 +19,7 @@ STATUS_NAMES = {
     6: "QX_ERR_DEVICE", 7: "QX_ERR_ALLOC",
 }
 PREC = {"fp64": 0, "fp32": 1, "bf16x3": 2, "mixed": 2}
+SOLVER = {"newton": 0, "linearized": 1}
 PHASES = ["prep", "pi", "potrf", "trtri", "wnm", "newton", "residue", "spare"]
 
 EXPORTED = [
@@ -28,7 +29,8 @@ EXPORTED = [
     "qx_screened_wc_dev", "qx_wmn_imag_dev", "qx_sigma_cd_dev", "qx_g0w0_cd_dev", "qx_g0w0_cd_ov_dev", "qx_g0w0_cd",
     "qx_set_profiling", "qx_get_profile", "qx_g0w0_adjoint_ov_dev", "qx_g0w0_batch_ov_dev",
     "qx_sigma_ac_dev", "qx_thiele", "qx_pade_thiele", "qx_ac_pade_thiele_diag", "qx_qp_solve_ac",
-    "qx_exchange_diag_dev", "qx_df_mo_transform_dev",
+    "qx_exchange_diag_dev", "qx_df_mo_transform_dev", "qx_screening_w_dev",
+    "qx_sigma_c_pyscf_cd_dev", "qx_solve_qp_linearized",
 ]
 
 
@@ -48,6 +50,7 @@ class G0W0Desc(C.Structure):
         ("nocc", C.c_int), ("nvir", C.c_int), ("naux", C.c_int), ("nfreq", C.c_int),
         ("x0", C.c_double), ("eta", C.c_double), ("precision", C.c_int),
         ("ntarget", C.c_int), ("npair", C.c_int), ("newton", NewtonConfig),
+        ("qp_solver", C.c_int),
     ]
 
 
@@ -131,6 +134,10 @@ def load():
         lib.qx_exchange_diag_dev.argtypes = [vp, C.c_int, C.c_int, C.c_int, dp, dp]
         lib.qx_df_mo_transform_dev.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, dp, dp,
                                                dp, dp]
+        lib.qx_screening_w_dev.argtypes = [vp, C.c_int, C.c_int, dp, dp, dp, dp]
+        lib.qx_sigma_c_pyscf_cd_dev.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_double, dp,
+                                                dp, dp, dp, dp, dp]
+        lib.qx_solve_qp_linearized.argtypes = [C.c_int, dp, dp, dp, dp, dp, dp]
         lib.qx_set_profiling.argtypes = [vp, C.c_int]
         lib.qx_get_profile.argtypes = [vp, dp, dp]
         non_status = ("qx_destroy", "qx_last_error", "qx_newton_config_default",

@@ -100,16 +100,103 @@ def compute_screening_wc(p0, *, device=0):
     return out.cpu().numpy(), nfb.value
 
 
-def compute_screening_w(p0, coulomb_matrix=None, freqs=None, v_sqrt=None, *, device=0):
-    """quasix_core.compute_screening_w (python/g0w0.rs:103 -> screening.rs:334-420).
+def compute_screening_w(p0, coulomb_matrix, freqs, v_sqrt=None, *, device=0):
+    """quasix_core.compute_screening_w (python/g0w0.rs:102-130 -> screening.rs:164-420).
 
-    With the Cholesky-normalised DF tensors of the live path the metric is the identity and
-    W = eps^-1 = Wc + I.  A non-identity metric is outside this path and raises."""
-    for m in (coulomb_matrix, v_sqrt):
-        if m is not None and not np.allclose(np.asarray(m), np.eye(np.asarray(m).shape[0])):
-            raise RuntimeError("only the identity metric (Cholesky-normalised B) is supported")
-    wc, _ = compute_screening_wc(p0, device=device)
-    return wc + np.eye(wc.shape[1])[None]
+    W_f = vs (I - vs P0_f vs^T)^-1 vs^T (eps and W symmetrised), with vs = v_sqrt when given,
+    else the square root of the Coulomb metric ``coulomb_matrix``; all of it on the GPU
+    (qx_screening_w_dev).  ``freqs`` is only checked for length, as the reference ignores
+    it.  Errors raise RuntimeError("W(iω) computation error: ...") as the PyO3 layer does."""
+    dev = _dev(device)
+    h = _lib.handle(device)
+    p = _f64(p0, dev)
+    try:
+        if p.dim() != 3 or p.shape[1] != p.shape[2]:
+            raise ValueError(f"P0 must be [nfreq, naux, naux], got {tuple(p.shape)}")
+        nf, naux, _ = p.shape
+        v = np.asarray(coulomb_matrix.cpu() if torch.is_tensor(coulomb_matrix)
+                       else coulomb_matrix, dtype=np.float64)
+        if v.shape != (naux, naux):
+            raise ValueError(f"Dimension mismatch: v_aux must be {naux}x{naux}, got {v.shape}")
+        vs = None
+        if v_sqrt is not None:
+            vs = _f64(v_sqrt, dev)
+            if tuple(vs.shape) != (naux, naux):
+                raise ValueError(f"Dimension mismatch: v_sqrt must be {naux}x{naux}, got "
+                                 f"{tuple(vs.shape)}")
+    except ValueError as e:
+        raise RuntimeError(f"W(iω) computation error: {e}") from e
+    vd = _f64(v, dev)
+    out = torch.empty((nf, naux, naux), dtype=torch.float64, device=dev)
+    torch.cuda.synchronize(dev)
+    try:
+        h.check(h.lib.qx_screening_w_dev(h.h, naux, nf, _ptr(p), _ptr(vd), _ptr(vs), _ptr(out)))
+    except _lib.QxError as e:
+        raise RuntimeError(f"W(iω) computation error: {e}") from e
+    return out.cpu().numpy()
+
+
+def compute_sigma_c_pyscf_cd(mo_energy, eta, p0_iw, lpq, freqs, weights, *, device=0):
+    """quasix_core.compute_sigma_c_pyscf_cd (python/g0w0.rs:597-631 ->
+    correlation_pyscf_cd.rs:60-298): the static CD self-energy Sigma_c_n(E = 0) of every
+    orbital, without residues, from a given P0 [nfreq, naux, naux] and the full DF tensor
+    lpq [naux, nmo, nmo], on the GPU (qx_sigma_c_pyscf_cd_dev).  Returns complex [nmo].
+    Shape errors are the reference's DimensionMismatch cases, raised as RuntimeError."""
+    dev = _dev(device)
+    h = _lib.handle(device)
+    e = _f64(mo_energy, dev)
+    p = _f64(p0_iw, dev)
+    L = _f64(lpq, dev)
+    fr = _f64(freqs, dev)
+    wt = _f64(weights, dev)
+    nmo = e.numel()
+    msg = None
+    if p.dim() != 3 or p.shape[1] != p.shape[2]:
+        msg = f"P₀ not square: {tuple(p.shape)}"
+    elif L.dim() != 3 or L.shape[0] != p.shape[1]:
+        msg = f"Lpq naux {L.shape[0] if L.dim() == 3 else '?'} != P₀ naux {p.shape[1]}"
+    elif L.shape[1] != nmo or L.shape[2] != nmo:
+        msg = f"Lpq MO dimensions ({L.shape[1]}, {L.shape[2]}) != nmo {nmo}"
+    elif wt.numel() != p.shape[0]:
+        msg = f"Weights length {wt.numel()} != nfreq {p.shape[0]}"
+    elif fr.numel() != p.shape[0]:
+        msg = f"Frequencies length {fr.numel()} != nfreq {p.shape[0]}"
+    if msg:
+        raise RuntimeError(f"Correlation self-energy (PySCF CD) error: Dimension mismatch: {msg}")
+    out = torch.empty((nmo, 2), dtype=torch.float64, device=dev)
+    torch.cuda.synchronize(dev)
+    try:
+        h.check(h.lib.qx_sigma_c_pyscf_cd_dev(h.h, nmo, p.shape[1], p.shape[0], float(eta),
+                                              _ptr(e), _ptr(p), _ptr(L), _ptr(fr), _ptr(wt),
+                                              _ptr(out)))
+    except _lib.QxError as err:
+        raise RuntimeError(f"Correlation self-energy (PySCF CD) error: {err}") from err
+    o = out.cpu().numpy()
+    return o[:, 0] + 1j * o[:, 1]
+
+
+def solve_qp_equation(mo_energy, sigma_x, sigma_c, vxc):
+    """quasix_core.solve_qp_equation (python/g0w0.rs:423-450 -> qp/solver.rs:137-221):
+    E = eps + Sigma_x + Sigma_c - V_xc with Z = 1, host code in libqxgw.  Returns
+    {'qp_energies', 'z_factors', 'converged': True}; a length mismatch or a non-finite
+    energy raises RuntimeError("QP solver error: ...")."""
+    arrs = [np.ascontiguousarray(np.asarray(a.cpu() if torch.is_tensor(a) else a,
+                                            dtype=np.float64)).reshape(-1)
+            for a in (mo_energy, sigma_x, sigma_c, vxc)]
+    n = arrs[0].size
+    for name, a in zip(("sigma_x", "sigma_c", "vxc"), arrs[1:]):
+        if a.size != n:
+            raise RuntimeError(f"QP solver error: Invalid input: {name} dimension mismatch: "
+                               f"expected {n}, got {a.size}")
+    qp, z = np.empty(n), np.empty(n)
+    rc = _lib.load().qx_solve_qp_linearized(n, *(a.ctypes.data_as(C.c_void_p) for a in arrs),
+                                            qp.ctypes.data_as(C.c_void_p),
+                                            z.ctypes.data_as(C.c_void_p))
+    if rc != _lib.QX_OK:
+        bad = int(np.flatnonzero(~np.isfinite(qp))[0]) if rc == 3 else -1
+        raise RuntimeError(f"QP solver error: Numerical error: QP energy[{bad}] is not finite"
+                           if rc == 3 else f"QP solver error: {_lib.STATUS_NAMES.get(rc, rc)}")
+    return {"qp_energies": qp, "z_factors": z, "converged": True}
 
 
 def compute_wmn_imag(mo_energy, B_ov, nocc, B_pair, freqs, *, precision="fp64", device=0):

@@ -140,6 +140,7 @@ qx_status qx_qp_solve_ac(int norbs, int nfit, const double* coeff, const double*
     np.use_richardson = cfg->use_richardson;
     np.use_bisection = cfg->use_bisection_fallback;
     np.max_bisection_iterations = cfg->max_bisection_iterations;
+    np.linearized = 0;
     const double ef = 0.5 * (mo_energy[nocc - 1] + mo_energy[nocc]);
     const double de = 1e-6;
     for (int i = 0; i < norbs; ++i) {

@@ -81,18 +81,20 @@ __global__ __launch_bounds__(64) void adj_quad_kernel(
     }
 }
 
-// C_b = op(A_b) op(B_b) for square [n][n] matrices, batched over blockIdx.z; 64x64 tiles,
-// K streamed in chunks of 32 with register prefetch.
+// C_b = op(A_b) op(B_b) for square [n][n] matrices, batched over blockIdx.z with element
+// strides sa / sb / sc between the matrices of a batch (0 broadcasts one operand); 64x64
+// tiles, K streamed in chunks of 32 with register prefetch.
 template <bool TA, bool TB_>
 __global__ __launch_bounds__(256) void batched_gemm_kernel(const double* __restrict__ A,
                                                            const double* __restrict__ Bm,
-                                                           double* __restrict__ C, int n) {
+                                                           double* __restrict__ C, int n,
+                                                           size_t sa, size_t sb, size_t sc) {
     __shared__ double As[TB * (GK + 1)];
     __shared__ double Bs[TB * (GK + 1)];
     const int m0 = blockIdx.y * TB, n0 = blockIdx.x * TB, b = blockIdx.z;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const double* Ab = A + (size_t)b * n * n;
-    const double* Bb = Bm + (size_t)b * n * n;
+    const double* Ab = A + (size_t)b * sa;
+    const double* Bb = Bm + (size_t)b * sb;
     double ra[8], rb[8];
     // LDS layouts: As "[m][k]" (pitch GK+1) or "[k][m]" (pitch TB+1); same for Bs with n.
     auto fetch = [&](int k0) {
@@ -132,7 +134,7 @@ __global__ __launch_bounds__(256) void batched_gemm_kernel(const double* __restr
         }
         __syncthreads();
     }
-    double* Cb = C + (size_t)b * n * n;
+    double* Cb = C + (size_t)b * sc;
     #pragma unroll
     for (int bj = 0; bj < 4; ++bj)
         #pragma unroll
@@ -285,15 +287,26 @@ hipError_t launch_adjoint_quad(const int* targets, const int* pair_off, const in
     return hipGetLastError();
 }
 
+hipError_t launch_strided_gemm(bool ta, bool tb, const double* A, size_t sa, const double* B,
+                               size_t sb, double* C, size_t sc, int n, int batch,
+                               hipStream_t st) {
+    if (n % TB != 0 || n <= 0 || batch <= 0) return hipErrorInvalidValue;
+    dim3 grid(n / TB, n / TB, batch);
+    if (!ta && !tb)
+        hipLaunchKernelGGL((batched_gemm_kernel<false, false>), grid, dim3(256), 0, st, A, B, C, n, sa, sb, sc);
+    else if (ta && !tb)
+        hipLaunchKernelGGL((batched_gemm_kernel<true, false>), grid, dim3(256), 0, st, A, B, C, n, sa, sb, sc);
+    else if (!ta && tb)
+        hipLaunchKernelGGL((batched_gemm_kernel<false, true>), grid, dim3(256), 0, st, A, B, C, n, sa, sb, sc);
+    else
+        hipLaunchKernelGGL((batched_gemm_kernel<true, true>), grid, dim3(256), 0, st, A, B, C, n, sa, sb, sc);
+    return hipGetLastError();
+}
+
 hipError_t launch_batched_gemm(bool ta, bool tb, const double* A, const double* B, double* C,
                                int n, int batch, hipStream_t st) {
-    if (n % TB != 0) return hipErrorInvalidValue;
-    dim3 grid(n / TB, n / TB, batch);
-    if (!ta && !tb) hipLaunchKernelGGL((batched_gemm_kernel<false, false>), grid, dim3(256), 0, st, A, B, C, n);
-    else if (ta && !tb) hipLaunchKernelGGL((batched_gemm_kernel<true, false>), grid, dim3(256), 0, st, A, B, C, n);
-    else if (!ta && tb) hipLaunchKernelGGL((batched_gemm_kernel<false, true>), grid, dim3(256), 0, st, A, B, C, n);
-    else hipLaunchKernelGGL((batched_gemm_kernel<true, true>), grid, dim3(256), 0, st, A, B, C, n);
-    return hipGetLastError();
+    const size_t nn = (size_t)n * n;
+    return launch_strided_gemm(ta, tb, A, nn, B, nn, C, nn, n, batch, st);
 }
 
 hipError_t launch_scaled_gemm(const double* A, int nrow, int ncol, const double* s1,

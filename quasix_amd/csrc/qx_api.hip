@@ -323,8 +323,9 @@ void scaled_roots(int nw, double x0, std::vector<double>& f, std::vector<double>
     }
 }
 
-NewtonParams to_params(const qx_newton_config& c) {
+NewtonParams to_params(const qx_newton_config& c, int qp_solver = QX_SOLVER_NEWTON) {
     NewtonParams p;
+    p.linearized = qp_solver == QX_SOLVER_LINEARIZED;
     p.max_iterations = c.max_iterations;
     p.energy_tolerance = c.energy_tolerance;
     p.initial_damping = c.initial_damping;
@@ -608,6 +609,7 @@ int host_solve(HostCD& cd, const NewtonParams& np, int t, double static_t, bool 
                                s, iters);
     if (st == kNewtonConverged) return QX_QP_NEWTON;
     if (st == kNewtonBisection) return QX_QP_BISECTION;
+    if (st == kNewtonLinearized) return QX_QP_LINEARIZED;
     // Linearised fallback at the Newton start (evgw.rs:766-781).
     double g = guess;
     const double de = 1e-6;
@@ -644,6 +646,8 @@ void validate_desc(qx_handle h, const qx_g0w0_desc* d) {
             "unknown precision %d", d->precision);
     REQUIRE(h, d->ntarget > 0, QX_ERR_INVALID_INPUT, "ntarget must be positive");
     REQUIRE(h, d->npair >= 0, QX_ERR_INVALID_INPUT, "npair must be non-negative");
+    REQUIRE(h, d->qp_solver == QX_SOLVER_NEWTON || d->qp_solver == QX_SOLVER_LINEARIZED,
+            QX_ERR_INVALID_INPUT, "unknown qp_solver %d", d->qp_solver);
     REQUIRE(h, (long long)d->nocc * d->nvir < (1ll << 31), QX_ERR_INVALID_INPUT,
             "nocc*nvir overflows int");
 }
@@ -744,7 +748,7 @@ qx_status run_g0w0(qx_handle h, const qx_g0w0_desc* d, const double* mo_energy,
              static_terms, fr_dev, wt_dev, wnm, cd);
     WnmResult wr = build_wnm(h, d->precision, d->nocc, d->nvir, d->naux, mo_energy, B_ov, nf,
                              fr_dev, npair, B_pair, wnm);
-    const NewtonParams np = to_params(d->newton);
+    const NewtonParams np = to_params(d->newton, d->qp_solver);
     int* st_dev = ws<int>(h, "newton_state", nt);
     int* it_dev = ws<int>(h, "newton_iters", nt);
     {
@@ -765,6 +769,7 @@ qx_status run_g0w0(qx_handle h, const qx_g0w0_desc* d, const double* mo_energy,
     for (int t = 0; t < nt; ++t) {
         if (st[t] == kNewtonConverged) { st[t] = QX_QP_NEWTON; continue; }
         if (st[t] == kNewtonBisection) { st[t] = QX_QP_BISECTION; continue; }
+        if (st[t] == kNewtonLinearized) { st[t] = QX_QP_LINEARIZED; continue; }
         ++residue_targets;
         double E, Z;
         C2 s;
@@ -1066,6 +1071,212 @@ qx_status qx_sigma_cd_dev(qx_handle h, const qx_g0w0_desc* d, const double* mo_e
         h2d(h, sigma, out.data(), out.size());
         return QX_OK;
     });
+}
+
+namespace {
+
+double d2h_scalar(qx_handle h, const double* p) {
+    double v = 0.0;
+    d2h(h, &v, p, 1);
+    return v;
+}
+
+// eps[f] = L L^T with X = L^-1 in place (lower blocks), for every frequency; returns the
+// number of frequencies whose Cholesky failed.  eps is [nf][npad][npad] with an identity
+// padding block.
+int factor_all(qx_handle h, double* eps, int npad, int nf, double** x_out) {
+    const int nblk = npad / NB;
+    double* x = ws<double>(h, "xinv", (size_t)nf * npad * npad);
+    double* linv = ws<double>(h, "linv_diag", (size_t)nf * nblk * NB * NB);
+    int* fail = ws<int>(h, "chol_fail", nf);
+    fill_i32(h, fail, 0, nf);
+    int groups = 1;
+    const StreamFan* fan = potrf_fan(h, groups);
+    HIPCK(h, launch_potrf_batched(eps, x, linv, npad, nf, fail, h->stream, fan, groups));
+    std::vector<int> fh(nf);
+    d2h(h, fh.data(), fail, nf);
+    int nfail = 0;
+    for (int v : fh) nfail += v != 0;
+    *x_out = x;
+    return nfail;
+}
+
+// Gauss-Jordan inverse minus identity of every eps[f] ([nf][npad][npad]) into wc; raises
+// QX_ERR_NUMERICAL naming the first singular frequency.
+void gj_all(qx_handle h, const double* eps, int npad, int nf, double* wc, const char* what) {
+    double* work = ws<double>(h, "gj_work", (size_t)nf * npad * 2 * npad + nf);
+    HIPCK(h, launch_lu_fallback_wc(eps, 1, work, wc, npad, nf, nullptr, h->stream));
+    std::vector<int> sing(nf);
+    d2h(h, sing.data(), reinterpret_cast<int*>(work + (size_t)nf * npad * 2 * npad), nf);
+    for (int f = 0; f < nf; ++f)
+        REQUIRE(h, sing[f] == 0, QX_ERR_NUMERICAL, "%s singular at frequency %d", what, f);
+}
+
+// Principal square root of the SPD metric v [n][n] into vs [npad][npad] (zero padding) by
+// the coupled Newton-Schulz iteration on MFMA GEMMs: Y0 = v / c, Z0 = I,
+// T = (3 I - Z Y) / 2, Y <- Y T, Z <- T Z, vs = sqrt(c) Y with c = |v|_F.  Replaces the
+// eigendecomposition of compute_metric_sqrt (quasix_core/src/dielectric/screening.rs:
+// 580-639) and keeps its acceptance test |vs vs^T - v|_max <= 1e-10.
+void metric_sqrt(qx_handle h, const double* v, int n, int npad, double* vs) {
+    hipStream_t st = h->stream;
+    const size_t nn = (size_t)npad * npad;
+    double* A = ws<double>(h, "ns_a", nn);
+    double* Y = ws<double>(h, "ns_y", nn);
+    double* Z = ws<double>(h, "ns_z", nn);
+    double* Y2 = ws<double>(h, "ns_y2", nn);
+    double* Z2 = ws<double>(h, "ns_z2", nn);
+    double* T = ws<double>(h, "ns_t", nn);
+    double* red = ws<double>(h, "red", 1025);
+    HIPCK(h, launch_pad_square(v, 0, n, npad, 1, false, A, st));
+    HIPCK(h, launch_reduce(A, nullptr, nn, 0, red, st));
+    const double c = std::sqrt(d2h_scalar(h, red));
+    REQUIRE(h, c > 0.0 && std::isfinite(c), QX_ERR_NUMERICAL,
+            "metric square root: |v|_F = %g is not positive and finite", c);
+    HIPCK(h, launch_pad_square(v, 0, n, npad, 1, false, Y, st));
+    HIPCK(h, launch_axpby(0.0, nullptr, 1.0 / c, Y, nn, st));
+    HIPCK(h, launch_pad_square(nullptr, 0, n, npad, 1, false, Z, st));
+    HIPCK(h, launch_add_identity(Z, npad, n, 1, 1.0, st));
+    double prev = INFINITY;
+    for (int it = 0; it < 100; ++it) {
+        HIPCK(h, launch_strided_gemm(false, false, Z, 0, Y, 0, T, 0, npad, 1, st));
+        HIPCK(h, launch_axpby(0.0, nullptr, -0.5, T, nn, st));
+        HIPCK(h, launch_add_identity(T, npad, n, 1, 1.5, st));
+        HIPCK(h, launch_strided_gemm(false, false, Y, 0, T, 0, Y2, 0, npad, 1, st));
+        HIPCK(h, launch_strided_gemm(false, false, T, 0, Z, 0, Z2, 0, npad, 1, st));
+        HIPCK(h, launch_reduce(Y2, Y, nn, 1, red, st));
+        const double diff = d2h_scalar(h, red);
+        std::swap(Y, Y2);
+        std::swap(Z, Z2);
+        if (!(diff > 1e-15) || (diff < 1e-10 && diff >= prev)) break;   // converged or at the rounding floor
+        prev = diff;
+    }
+    // symmetrise, scale, and check the reconstruction
+    HIPCK(h, launch_sym_compact(Y, npad, n, 1, T, st));
+    HIPCK(h, launch_pad_square(T, 0, n, npad, 1, false, vs, st));
+    HIPCK(h, launch_axpby(0.0, nullptr, std::sqrt(c), vs, nn, st));
+    HIPCK(h, launch_strided_gemm(false, true, vs, 0, vs, 0, T, 0, npad, 1, st));
+    HIPCK(h, launch_reduce(T, A, nn, 1, red, st));
+    const double err = d2h_scalar(h, red);
+    REQUIRE(h, err <= 1e-10, QX_ERR_NUMERICAL,
+            "metric square root reconstruction error: %.3e", err);
+}
+
+}  // namespace
+
+qx_status qx_sigma_c_pyscf_cd_dev(qx_handle h, int nmo, int naux, int nfreq, double eta,
+                                  const double* mo_energy, const double* p0_iw,
+                                  const double* lpq, const double* freqs,
+                                  const double* weights, double* sigma) {
+    return guarded(h, [&] {
+        REQUIRE(h, nmo > 0 && naux > 0 && nfreq > 0, QX_ERR_INVALID_INPUT,
+                "dimensions must be positive (nmo=%d naux=%d nfreq=%d)", nmo, naux, nfreq);
+        REQUIRE(h, mo_energy && p0_iw && lpq && freqs && weights && sigma, QX_ERR_INVALID_INPUT,
+                "NULL pointer");
+        REQUIRE(h, (long long)nmo * nmo * round_up(naux, NB) < (1ll << 40), QX_ERR_INVALID_INPUT,
+                "nmo^2 * naux too large");
+        hipStream_t st = h->stream;
+        const int npad = round_up(naux, NB), nblk = npad / NB, nf = nfreq;
+        const int npair = nmo * nmo;
+        // eps = I - P0 (identity padding), Cholesky with X = L^-1
+        double* eps = ws<double>(h, "eps_mat", (size_t)nf * npad * npad);
+        hipLaunchKernelGGL(eps_from_pi_kernel, grid1d((size_t)nf * npad * npad), dim3(256), 0, st,
+                           p0_iw, naux, npad, nf, eps);
+        HIPCK(h, hipGetLastError());
+        double* x = nullptr;
+        const int nfail = factor_all(h, eps, npad, nf, &x);
+        // pair rows of every (n, m): b^T Wc b' = u^T Wc u - v^T Wc v
+        double* u = ws<double>(h, "cd0_u", (size_t)npair * npad);
+        double* v = ws<double>(h, "cd0_v", (size_t)npair * npad);
+        int* anyv = ws<int>(h, "cd0_anyv", 1);
+        HIPCK(h, launch_lpq_pair_rows(lpq, naux, nmo, npad, u, v, anyv, st));
+        int anyv_h = 0;
+        d2h(h, &anyv_h, anyv, 1);
+        double* wu = ws<double>(h, "cd0_wu", (size_t)nf * npair);
+        double* wv = anyv_h ? ws<double>(h, "cd0_wv", (size_t)nf * npair) : nullptr;
+        double* wpart = ws<double>(h, "wpart", (size_t)nf * nblk * npair);
+        if (nfail == 0) {
+            HIPCK(h, launch_wnm(QX_PREC_FP64, u, x, npair, npad, nf, wpart, wu, nullptr, st));
+            if (wv) HIPCK(h, launch_wnm(QX_PREC_FP64, v, x, npair, npad, nf, wpart, wv, nullptr, st));
+        } else {
+            // the reference inverts eps with LU and reports a singular one as an error
+            hipLaunchKernelGGL(eps_from_pi_kernel, grid1d((size_t)nf * npad * npad), dim3(256), 0,
+                               st, p0_iw, naux, npad, nf, eps);
+            HIPCK(h, hipGetLastError());
+            double* wc = ws<double>(h, "cd0_wc", (size_t)nf * npad * npad);
+            gj_all(h, eps, npad, nf, wc, "Epsilon");
+            HIPCK(h, launch_wnm_from_wc(u, wc, npair, npad, nf, wpart, wu, st));
+            if (wv) HIPCK(h, launch_wnm_from_wc(v, wc, npair, npad, nf, wpart, wv, st));
+        }
+        HIPCK(h, launch_sigma_static_cd(wu, wv, nmo, nf, mo_energy, freqs, weights, eta, sigma, st));
+        HIPCK(h, hipStreamSynchronize(st));
+        return QX_OK;
+    });
+}
+
+qx_status qx_screening_w_dev(qx_handle h, int naux, int nfreq, const double* p0,
+                             const double* metric, const double* v_sqrt, double* w_out) {
+    return guarded(h, [&] {
+        REQUIRE(h, naux > 0 && nfreq > 0, QX_ERR_INVALID_INPUT,
+                "dimensions must be positive (naux=%d nfreq=%d)", naux, nfreq);
+        REQUIRE(h, p0 && w_out, QX_ERR_INVALID_INPUT, "NULL pointer");
+        hipStream_t st = h->stream;
+        const int npad = round_up(naux, NB), nf = nfreq;
+        const size_t nn = (size_t)npad * npad;
+        double* vs = nullptr;
+        if (v_sqrt || metric) {
+            vs = ws<double>(h, "scr_vs", nn);
+            if (v_sqrt) HIPCK(h, launch_pad_square(v_sqrt, 0, naux, npad, 1, false, vs, st));
+            else metric_sqrt(h, metric, naux, npad, vs);
+        }
+        // M_f = vs P0_f vs^T, eps_f = I - (M_f + M_f^T) / 2 (screening.rs:334-356)
+        double* P = ws<double>(h, "scr_p", nf * nn);
+        HIPCK(h, launch_pad_square(p0, (size_t)naux * naux, naux, npad, nf, false, P, st));
+        double* T = vs ? ws<double>(h, "scr_t", nf * nn) : nullptr;
+        if (vs) {
+            HIPCK(h, launch_strided_gemm(false, false, vs, 0, P, nn, T, nn, npad, nf, st));
+            HIPCK(h, launch_strided_gemm(false, true, T, nn, vs, 0, P, nn, npad, nf, st));
+        }
+        double* eps = ws<double>(h, "eps_mat", nf * nn);
+        HIPCK(h, launch_eps_sym(P, npad, nf, eps, st));
+        // eps^-1: Cholesky (X^T X), LU on failure, an error if singular (screening.rs:358-383)
+        double* x = nullptr;
+        const int nfail = factor_all(h, eps, npad, nf, &x);
+        double* einv = ws<double>(h, "scr_einv", nf * nn);
+        if (nfail == 0) {
+            HIPCK(h, launch_zero_upper_blocks(x, npad, nf, st));
+            HIPCK(h, launch_strided_gemm(true, false, x, nn, x, nn, einv, nn, npad, nf, st));
+        } else {
+            HIPCK(h, launch_eps_sym(P, npad, nf, eps, st));
+            gj_all(h, eps, npad, nf, einv, "Matrix inversion failed: epsilon");
+            HIPCK(h, launch_add_identity(einv, npad, npad, nf, 1.0, st));
+        }
+        // W_f = vs eps_f^-1 vs^T, symmetrised (screening.rs:385-411)
+        const double* wsrc = einv;
+        if (vs) {
+            HIPCK(h, launch_strided_gemm(false, false, vs, 0, einv, nn, T, nn, npad, nf, st));
+            HIPCK(h, launch_strided_gemm(false, true, T, nn, vs, 0, P, nn, npad, nf, st));
+            wsrc = P;
+        }
+        HIPCK(h, launch_sym_compact(wsrc, npad, naux, nf, w_out, st));
+        double* red = ws<double>(h, "red", 1025);
+        HIPCK(h, launch_reduce(w_out, nullptr, (size_t)nf * naux * naux, 2, red, st));
+        const double mx = d2h_scalar(h, red);
+        REQUIRE(h, std::isfinite(mx), QX_ERR_NUMERICAL, "W contains NaN or Inf values");
+        return QX_OK;
+    });
+}
+
+qx_status qx_solve_qp_linearized(int nmo, const double* mo_energy, const double* sigma_x,
+                                 const double* sigma_c, const double* vxc, double* qp_energy,
+                                 double* z_factor) {
+    if (nmo <= 0 || !mo_energy || !sigma_x || !sigma_c || !vxc || !qp_energy || !z_factor)
+        return QX_ERR_INVALID_INPUT;
+    for (int i = 0; i < nmo; ++i) {
+        qp_energy[i] = mo_energy[i] + sigma_x[i] + sigma_c[i] - vxc[i];
+        z_factor[i] = 1.0;
+        if (!std::isfinite(qp_energy[i])) return QX_ERR_NUMERICAL;
+    }
+    return QX_OK;
 }
 
 qx_status qx_df_mo_transform_dev(qx_handle h, int nao, int nocc, int nvir, int naux,
@@ -1497,7 +1708,7 @@ BatchPlan* build_batch_plan(qx_handle h, const qx_g0w0_desc* d, int ninst,
     plan->nf = nf;
     plan->precision = d->precision;
     plan->eta = d->eta;
-    plan->np = to_params(d->newton);
+    plan->np = to_params(d->newton, d->qp_solver);
     // instances, sorted by padded size then by N_ov (largest first)
     std::vector<BatchInst>& v = plan->inst;
     v.resize(ninst);
@@ -1840,11 +2051,15 @@ qx_status qx_g0w0_batch_ov_dev(qx_handle h, const qx_g0w0_desc* d, int ninst,
             bool bad = false;
             for (int f = 0; f < P.nf; ++f) bad |= fail[(size_t)q * P.nf + f] != 0;
             for (int t = 0; t < in.nt; ++t)
-                bad |= st[in.out_off + t] != kNewtonConverged && st[in.out_off + t] != kNewtonBisection;
+                bad |= st[in.out_off + t] != kNewtonConverged &&
+                       st[in.out_off + t] != kNewtonBisection &&
+                       st[in.out_off + t] != kNewtonLinearized;
             if (bad) redo.push_back(q);
         }
         if (state) {
-            for (int& v : st) v = v == kNewtonBisection ? QX_QP_BISECTION : QX_QP_NEWTON;
+            for (int& v : st)
+                v = v == kNewtonBisection ? QX_QP_BISECTION
+                    : v == kNewtonLinearized ? QX_QP_LINEARIZED : QX_QP_NEWTON;
             h2d(h, state, st.data(), st.size());
         }
         qx_stats acc{};

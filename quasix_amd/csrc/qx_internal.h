@@ -183,6 +183,7 @@ struct NewtonParams {
     double initial_damping, min_damping, max_damping, max_step, derivative_delta;
     double z_min, z_max;
     int use_richardson, use_bisection, max_bisection_iterations;
+    int linearized;              // 1: linearised QP step instead of the Newton solve
 };
 
 struct SigmaProblem {            // everything the CD quadrature needs (device pointers)
@@ -202,7 +203,8 @@ struct SigmaProblem {            // everything the CD quadrature needs (device p
 
 // Status codes written per target by the Newton kernel.
 enum NewtonState : int {
-    kNewtonConverged = 0, kNewtonBisection = 1, kNewtonFailed = 2, kNewtonNeedsResidue = 3
+    kNewtonConverged = 0, kNewtonBisection = 1, kNewtonFailed = 2, kNewtonNeedsResidue = 3,
+    kNewtonLinearized = 4
 };
 
 // Device Newton loop (one wavefront per target), imaginary-axis part only; targets whose
@@ -258,6 +260,44 @@ hipError_t launch_adjoint_quad(const int* targets, const int* pair_off, const in
 // C_b = op(A_b) op(B_b), square n x n (n % 64 == 0), batched.
 hipError_t launch_batched_gemm(bool ta, bool tb, const double* A, const double* B, double* C,
                                int n, int batch, hipStream_t st);
+// The same with element strides between the matrices of a batch (0 broadcasts one operand).
+hipError_t launch_strided_gemm(bool ta, bool tb, const double* A, size_t sa, const double* B,
+                               size_t sb, double* C, size_t sc, int n, int batch,
+                               hipStream_t st);
+
+// ---- Stage entry points on the reference's intermediates (stage_kernels.hip) ------------
+// Pair rows of a full DF tensor lpq [naux][nmo][nmo] for every (n, m), j = n * nmo + m:
+// u_j = (lpq[:, n, m] + lpq[:, m, n]) / 2 and v_j = (lpq[:, n, m] - lpq[:, m, n]) / 2, both
+// [nmo * nmo][npad] zero padded; *any_v (device, cleared by the launcher) becomes 1 when
+// some v entry is non-zero.  b_n^T W b_m = u^T W u - v^T W v for symmetric W.
+hipError_t launch_lpq_pair_rows(const double* lpq, int naux, int nmo, int npad, double* u,
+                                double* v, int* any_v, hipStream_t st);
+// Static CD self-energy at E = 0 for every orbital n (sigma: [nmo] complex, re/im):
+// -1/pi sum_f w_f sum_m W_nm(f) emo / (emo^2 + w_f^2), emo = -eps_m - i eta s,
+// s = +1 if eps_n > eps_m else -1; W_nm(f) = wu[f][n nmo + m] - wv[f][n nmo + m] (wv
+// nullable).
+hipError_t launch_sigma_static_cd(const double* wu, const double* wv, int nmo, int nf,
+                                  const double* eps, const double* freqs, const double* wts,
+                                  double eta, double* sigma, hipStream_t st);
+// dst[b][npad][npad] = src[b] ([n][n], batch stride src_stride elements, 0 broadcasts),
+// zero padded (all zero when src is NULL); diag_pad puts 1 on the padding diagonal.
+hipError_t launch_pad_square(const double* src, size_t src_stride, int n, int npad, int batch,
+                             bool diag_pad, double* dst, hipStream_t st);
+// eps[b] = I - (M[b] + M[b]^T) / 2 over [npad][npad].
+hipError_t launch_eps_sym(const double* M, int npad, int batch, double* eps, hipStream_t st);
+// W[b] ([n][n] compact) = (T[b] + T[b]^T) / 2 from [npad][npad].
+hipError_t launch_sym_compact(const double* T, int npad, int n, int batch, double* W,
+                              hipStream_t st);
+// a[b] += alpha I over [npad][npad] (the first n diagonal entries).
+hipError_t launch_add_identity(double* a, int npad, int n, int batch, double alpha,
+                               hipStream_t st);
+// out[0] = sum a^2 (mode 0), max |a - b| (mode 1) or max |a| (mode 2) over count entries;
+// out holds 1 + 1024 doubles (the result, then the partials of the first pass).
+hipError_t launch_reduce(const double* a, const double* b, size_t count, int mode, double* out,
+                         hipStream_t st);
+// y = alpha x + beta y elementwise over count entries; x == nullptr means y = beta y.
+hipError_t launch_axpby(double alpha, const double* x, double beta, double* y, size_t count,
+                        hipStream_t st);
 // out[r][P] += alpha sum_k sum_Q s1_k(r) A[r][Q] M_k[Q][P]; part[pt][r] = row dots with s2.
 hipError_t launch_scaled_gemm(const double* A, int nrow, int ncol, const double* s1,
                               const double* s2, int sld, const double* M, int npad, int nf,

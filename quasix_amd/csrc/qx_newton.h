@@ -10,7 +10,8 @@
 //   the step (:201), Z = clamp(1/(1 - dSigma/dE), z_min, z_max) (:306-315);
 //   bisection fallback with brackets +-2 Ha x 2^k (:317-391).
 // and the iteration-0 linearised initial guess of quasix_core/src/qp/evgw.rs:730-741
-// (forward difference, clamped Z).
+// (forward difference, clamped Z).  With np.linearized set it returns the linearised QP
+// step itself (the driver's default solver, evgw.rs:786-820).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <cmath>
@@ -30,6 +31,20 @@ __host__ __device__ inline int newton_core(Eval& ev, const NewtonParams& np, dou
     const double de = np.derivative_delta;
     double E;
     C2 s[5];
+    if (np.linearized) {
+        // Linearised QP step at the start point (the guess, else eps_n) with a forward-
+        // difference Z clamped to [z_min, z_max] (quasix_core/src/qp/evgw.rs:786-820).
+        const double om = (have_guess ? guess : eps_n) - ef;
+        double Es[2] = {om + ef, (om + de) + ef};
+        if (ev(Es, 2, s)) return kNewtonNeedsResidue;
+        double z = 1.0 / (1.0 - (s[1].re - s[0].re) / de);
+        z = fmin(fmax(z, np.z_min), np.z_max);
+        E_out = eps_n + z * (stat + s[0].re);
+        Z_out = z;
+        s_out = s[0];
+        iters_out = 0;
+        return kNewtonLinearized;
+    }
     if (have_guess) {
         E = guess;
     } else {

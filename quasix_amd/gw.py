@@ -117,14 +117,29 @@ def symmetrized_ov(iaP, miP, nocc, nvir, naux, device=0):
     return (0.5 * (ia + mi[nocc:].permute(1, 0, 2))).reshape(nocc * nvir, naux).contiguous()
 
 
+def parse_qp_solver(qp_solver) -> str:
+    """'linearized' or 'newton' from the spellings quasix/src/evgw.rs:215-226 accepts
+    (case-insensitive 'linearized' / 'linear', 'newton' / 'newton-raphson' / 'solved');
+    anything else raises ValueError as there."""
+    s = str(qp_solver).lower()
+    if s in ("linearized", "linear"):
+        return "linearized"
+    if s in ("newton", "newton-raphson", "solved"):
+        return "newton"
+    raise ValueError(f"Invalid qp_solver '{qp_solver}'. Valid options: 'linearized' (default) "
+                     "or 'newton'")
+
+
 def g0w0(mo_energy, B_ov, nocc, targets=None, *, nfreq=48, x0=0.5, eta=0.01, precision="fp64",
          static=None, pairs: Pairs | None = None, mo_energy_ref=None, guess=None, device=0,
-         newton=None, stats=False):
+         newton=None, solver="newton", stats=False):
     """One G0W0 solve on the GPU.
 
     mo_energy [nmo]; B_ov [nocc*nvir, naux] (device tensor or array); targets: orbital indices
     (default HOMO-5..LUMO+5); static: Sigma_x - V_xc per target (default 0); pairs: explicit
-    target-partner rows (default: the ov-only pairs gathered on the device).
+    target-partner rows (default: the ov-only pairs gathered on the device); solver "newton"
+    (NewtonQPSolver from the guess, else from the linearised value) or "linearized" (the
+    linearised step at the guess, else at the mean-field energy).
     Returns dict(qp_energies, z_factors, sigma_c (complex), state, targets[, stats]).
     """
     dev = _dev(device)
@@ -141,6 +156,7 @@ def g0w0(mo_energy, B_ov, nocc, targets=None, *, nfreq=48, x0=0.5, eta=0.01, pre
     nt = len(tg)
     d = make_desc(nocc, nvir, naux, nfreq, x0, eta, precision, nt, 0 if pairs is None
                   else pairs.rows.shape[0], **(newton or {}))
+    d.qp_solver = _lib.SOLVER[parse_qp_solver(solver)]
     st = None if static is None else _f64(static, dev)
     er = None if mo_energy_ref is None else _f64(mo_energy_ref, dev)
     gs = None if guess is None else _f64(guess, dev)
@@ -168,47 +184,56 @@ def g0w0(mo_energy, B_ov, nocc, targets=None, *, nfreq=48, x0=0.5, eta=0.01, pre
 
 def evgw_core(mo_energy, B_ov, nocc, targets=None, *, n_iter=8, damping=0.5, conv_tol=1e-4,
               conv_tol_z=1e-3, diis=False, diis_space=6, nfreq=64, x0=0.5, eta=0.01,
-              precision="fp64", static=None, pairs=None, device=0, newton=None, progress=None):
+              precision="fp64", static=None, pairs=None, device=0, newton=None,
+              solver="newton", frontier_convergence=True, frontier_window=1.0, progress=None):
     """evGW on the CD self-energy: Pi, poles and Green's function from the previous QP
     energies, QP equation anchored at the mean-field energies (quasix_core/src/qp/evgw.rs:
-    603-1027: iteration 0 direct, iteration 1 damped, later iterations DIIS-or-damped through
-    ``convergence.evgw_mix``; Newton restarted from the previous QP energy).  Orbitals outside
-    ``targets`` keep their mean-field energies.  B stays resident on the device across
-    iterations.  ``progress(iteration, max_dE)`` is called after each iteration if given."""
-    from .convergence import ConvergenceMonitor, evgw_mix
+    603-1027 with the AC self-energy replaced by the CD one: iteration 0 direct, iteration 1
+    damped, later iterations Pulay-or-damped through ``convergence.evgw_mix``; the QP step
+    (``solver`` "newton" or "linearized") starts from the previous QP energy).  Orbitals
+    outside ``targets`` keep their mean-field energies.  Stop rule (evgw.rs:929-1027): after
+    every iteration, the first included, the largest |dE| over the frontier window of the
+    mean-field energies (evgw.rs:1121-1160) is compared with ``conv_tol``.  B stays resident
+    on the device across iterations.  ``progress(iteration, max_dE)`` is called after each
+    iteration if given."""
+    from .convergence import EvGWHistory, evgw_mix, frontier_window as fwin, window_change
     dev = _dev(device)
     e0 = _f64(mo_energy, dev)
     B = _f64(B_ov, dev)
     nmo = e0.numel()
     tg = frontier_targets(nocc, nmo) if targets is None else np.asarray(targets, dtype=np.int32)
     tgt = torch.as_tensor(tg, dtype=torch.long, device=dev)
-    cur = e0.clone()
+    e0_h = e0.cpu().numpy()
+    win = fwin(e0_h, nocc, frontier_convergence, frontier_window)
+    cur = e0_h.copy()
     zc = np.ones(len(tg))
-    mon = ConvergenceMonitor(conv_tol, conv_tol_z, diis_space, use_diis=diis)
+    mon = EvGWHistory(conv_tol, conv_tol_z, diis_space, use_diis=diis)
     history = []
     last = None
     converged = False
     for it in range(n_iter):
-        guess = None if it == 0 else cur[tgt]
+        guess = None if it == 0 else cur[tg]
         last = g0w0(cur, B, nocc, tg, nfreq=nfreq, x0=x0, eta=eta, precision=precision,
                     static=static, pairs=pairs, mo_energy_ref=e0, guess=guess, device=device,
-                    newton=newton)
-        e_old = cur[tgt].cpu().numpy()
-        e_new, z_new = np.asarray(last["qp_energies"]), np.asarray(last["z_factors"])
-        et, zt, active = evgw_mix(it, mon, e_old, zc, e_new, z_new, damping, use_diis=diis)
-        new = cur.clone()
-        new[tgt] = torch.as_tensor(et, device=dev)
-        dmax = float(np.max(np.abs(et - e_old)))
-        history.append(dict(cycle=it, qp_energies=new.cpu().numpy(), energy_change=dmax,
-                            z_factors=zt, diis_active=active))
+                    newton=newton, solver=solver)
+        e_old = cur[tg]
+        et, zt, active = evgw_mix(it, mon, e_old, zc, np.asarray(last["qp_energies"]),
+                                  np.asarray(last["z_factors"]), damping, use_diis=diis)
+        new = cur.copy()
+        new[tg] = et
+        dmax, drms = window_change(new, cur, win)
+        zf = np.ones(nmo)
+        zf[tg] = zt
+        history.append(dict(cycle=it, qp_energies=new.copy(), energy_change=dmax,
+                            rms_change=drms, z_factors=zf, diis_active=active))
         cur, zc = new, zt
         if progress is not None:
             progress(it, dmax)
-        if it > 0 and dmax < conv_tol:
+        if dmax < conv_tol:
             converged = True
             break
-    return dict(qp_energies=cur.cpu().numpy(), z_factors=zc, converged=converged,
-                n_cycles=len(history), history=history, last=last, targets=tg)
+    return dict(qp_energies=cur, z_factors=zc, converged=converged, n_cycles=len(history),
+                history=history, last=last, targets=tg, window=win)
 
 
 def ov_pairs(B_ov, nocc, nvir, targets, device=0) -> Pairs:
@@ -259,14 +284,18 @@ def sigma_ac(mo_energy, B_ov, nocc, pairs: Pairs, targets, *, nw=100, x0=0.5, iw
 def evgw_ac_core(mo_energy, B_ov, nocc, targets=None, *, pairs: Pairs | None = None, n_iter=8,
                  damping=0.5, conv_tol=1e-4, conv_tol_z=1e-3, diis=False, diis_space=6, nw=100,
                  x0=0.5, iw_cutoff=5.0, solver="linearized", sigma_x=None, vxc=None,
-                 precision="fp64", device=0, newton=None):
+                 precision="fp64", device=0, newton=None, frontier_convergence=True,
+                 frontier_window=1.0):
     """evGW on the AC self-energy, the reference's live driver (quasix_core/src/qp/evgw.rs:
     420-1049): Sigma_c(i nu) from the previous QP energies on the GPU, Thiele-Pade fit and
-    the QP step on the host (libqxgw), the damping/DIIS step of ``convergence.evgw_mix``.
+    the QP step on the host (libqxgw), the damping/Pulay step of ``convergence.evgw_mix``.
     ``targets`` (default all orbitals, as the reference) are updated; ef of the QP step is the
-    mean-field one, as there."""
+    mean-field one, as there.  Stop rule (evgw.rs:929-1027): after every iteration, the first
+    included, the largest |dE| over the frontier window of the mean-field energies
+    (evgw.rs:1121-1160) against ``conv_tol``; the history keeps that value, the RMS over the
+    window and each iteration's Z."""
     from . import core
-    from .convergence import ConvergenceMonitor, evgw_mix
+    from .convergence import EvGWHistory, evgw_mix, frontier_window as fwin, window_change
     dev = _dev(device)
     e0 = np.ascontiguousarray(np.asarray(mo_energy.cpu() if torch.is_tensor(mo_energy)
                                          else mo_energy, dtype=np.float64))
@@ -277,9 +306,10 @@ def evgw_ac_core(mo_energy, B_ov, nocc, targets=None, *, pairs: Pairs | None = N
         pairs = ov_pairs(B, nocc, nmo - nocc, tg, device)
     sx = np.zeros(nmo) if sigma_x is None else np.asarray(sigma_x, dtype=np.float64)
     vx = np.zeros(nmo) if vxc is None else np.asarray(vxc, dtype=np.float64)
+    win = fwin(e0, nocc, frontier_convergence, frontier_window)
     cur = e0.copy()
     zc = np.ones(len(tg))
-    mon = ConvergenceMonitor(conv_tol, conv_tol_z, diis_space, use_diis=diis)
+    mon = EvGWHistory(conv_tol, conv_tol_z, diis_space, use_diis=diis)
     history, last, converged = [], None, False
     for it in range(n_iter):
         sig, om = sigma_ac(cur, B, nocc, pairs, tg, nw=nw, x0=x0, iw_cutoff=iw_cutoff,
@@ -290,17 +320,30 @@ def evgw_ac_core(mo_energy, B_ov, nocc, targets=None, *, pairs: Pairs | None = N
         e_old = cur[tg]
         et, zt, active = evgw_mix(it, mon, e_old, zc, last["qp_energies"], last["z_factors"],
                                   damping, use_diis=diis)
-        cur = cur.copy()
-        cur[tg] = et
-        dmax = float(np.max(np.abs(et - e_old)))
-        history.append(dict(cycle=it, qp_energies=cur.copy(), energy_change=dmax, z_factors=zt,
-                            diis_active=active))
-        zc = zt
-        if it > 0 and dmax < conv_tol:
+        new = cur.copy()
+        new[tg] = et
+        dmax, drms = window_change(new, cur, win)
+        zf = np.ones(nmo)
+        zf[tg] = zt
+        history.append(dict(cycle=it, qp_energies=new.copy(), energy_change=dmax,
+                            rms_change=drms, z_factors=zf, diis_active=active))
+        cur, zc = new, zt
+        if dmax < conv_tol:
             converged = True
             break
     return dict(qp_energies=cur, z_factors=zc, converged=converged, n_cycles=len(history),
-                history=history, last=last, targets=tg)
+                history=history, last=last, targets=tg, window=win)
+
+
+def symmetrized_mi(iaP, miP, nocc, nvir, naux, device=0):
+    """[nmo, nocc, naux] block of the symmetrised tensor, X_sym[m, i] = (X[m, i] + X[i, m]) / 2
+    with X[i, j] = miP[i, j] and X[i, nocc + a] = iaP[i, a]: the df_mi the reference slices
+    from the assembled tensor for the exchange term (quasix/src/evgw.rs:161-205, :305)."""
+    dev = _dev(device)
+    ia = _f64(iaP, dev).reshape(nocc, nvir, naux)
+    mi = _f64(miP, dev).reshape(nocc + nvir, nocc, naux)
+    xt = torch.cat([mi[:nocc].permute(1, 0, 2), ia.permute(1, 0, 2)], dim=0)   # X[i, m] at [m, i]
+    return (0.5 * (mi + xt)).contiguous()
 
 
 def evgw(mo_energy, mo_occ, iaP, ijP, abP, chol_v, vxc_dft, max_cycle=12, conv_tol=1e-4,
@@ -308,101 +351,121 @@ def evgw(mo_energy, mo_occ, iaP, ijP, abP, chol_v, vxc_dft, max_cycle=12, conv_t
          nfreq=48, eta=0.01, verbose=1, energy_window=2.0, frontier_convergence=True,
          frontier_window=1.0, n_occ_active=None, n_vir_active=None, qp_solver="linearized",
          newton_max_iterations=50, newton_energy_tolerance=1e-8, *, precision="fp64",
-         targets=None, sigma_x=None, sigma_method="cd", device=0):
+         targets=None, sigma_x=None, sigma_method="ac", device=0):
     """quasix.quasix.evgw on the MI355X path (quasix/src/evgw.rs:22-453).
 
-    Same positional arguments and keyword defaults.  Sigma_x is computed from ijP (the
-    [nmo, nocc, naux] block) on the GPU as the reference does (quasix_core/src/qp/evgw.rs:
-    501-502); ``sigma_x`` (per orbital) overrides it.  ``sigma_method="cd"`` (default) uses
-    the contour-deformation self-energy of this library; ``"ac"`` uses the reference's own
-    method (Sigma(i nu) + Thiele-Pade, ``qp_solver`` linearized or newton as there, nfreq
-    imaginary points, iw_cutoff 5) and reproduces its iterates.  Only the target orbitals are
-    updated (default: all orbitals for "ac", as the reference; for "cd" the
-    n_occ_active/n_vir_active window, else HOMO-5..LUMO+5).
+    Same positional arguments, keyword defaults, errors and result keys.  As the reference:
+      * the DF tensor is assembled from iaP / ijP (the [nmo, nocc] miP block) / abP and
+        symmetrised (evgw.rs:161-205); Pi uses its ov block, the self-energy its rows;
+      * Sigma_x is the exchange diagonal of the symmetrised [nmo, nocc] block, computed on
+        the GPU (quasix_core/src/qp/evgw.rs:501-502); ``sigma_x`` (per orbital) overrides it;
+      * every orbital is updated (``orbs`` = all, quasix_core/src/qp/evgw.rs:572); the
+        energy_window / n_occ_active / n_vir_active window is only reported there and so has
+        no effect here; ``targets`` restricts the updated orbitals explicitly;
+      * the self-energy is the AC one (Sigma(i nu) on nfreq points, Thiele-Pade, iw_cutoff 5)
+        and ``qp_solver`` picks the linearised or the Newton QP step;
+      * convergence: max |dE| over the frontier window (frontier_convergence,
+        frontier_window) below conv_tol after any iteration, the first included.
+    ``sigma_method="cd"`` switches to this library's contour-deformation self-energy (no
+    reference counterpart on the live path), with the same driver and ``qp_solver``.
+    Result keys as evgw.rs:339-452: ip, ea and gap in Hartree; iteration_history entries
+    carry cycle (1-based), qp_energies, z_factors (the final Z in every entry, as the
+    reference binding reports), z_factors_cycle (that iteration's Z), energy_change and
+    rms_change (max and RMS over the window), damping_used (the damping argument) and
+    converged.  ``eta`` is used by the CD self-energy only, as it is unused by the
+    reference's AC driver; damping_dynamic and verbose are accepted and ignored as there.
     """
-    if qp_solver not in ("linearized", "newton"):
-        raise ValueError(f"qp_solver must be 'linearized' or 'newton', got {qp_solver!r}")
+    solver = parse_qp_solver(qp_solver)
     e = np.ascontiguousarray(mo_energy, dtype=np.float64)
     occ = np.asarray(mo_occ)
     nocc = int(np.count_nonzero(occ > 0))
     nmo = e.size
     nvir = nmo - nocc
     naux = int(np.asarray(chol_v).shape[0])
-    try:
-        ia = np.ascontiguousarray(iaP, dtype=np.float64).reshape(nocc * nvir, naux)
-        mi = np.ascontiguousarray(ijP, dtype=np.float64).reshape(nmo * nocc, naux)
-        ab = np.ascontiguousarray(abP, dtype=np.float64).reshape(nvir * nvir, naux)
-    except ValueError as exc:
-        raise ValueError(f"Failed to reshape DF tensors: {exc}") from exc
-    if targets is None and sigma_method == "ac":
-        targets = np.arange(nmo)                 # the reference's AC driver updates all orbitals
-    elif targets is None:
-        no = 5 if n_occ_active is None else int(n_occ_active) - 1
-        nv = 5 if n_vir_active is None else int(n_vir_active) - 1
-        targets = frontier_targets(nocc, nmo, no, nv)
-    tg = np.asarray(targets, dtype=np.int32)
+    if not 0 < nocc < nmo:
+        raise ValueError(f"need occupied and virtual orbitals (nocc={nocc}, nmo={nmo})")
+
+    def as2d(x, rows, name):
+        a = np.ascontiguousarray(x.cpu() if torch.is_tensor(x) else x, dtype=np.float64)
+        if a.size != rows * naux:
+            raise ValueError(f"Failed to reshape {name} tensor: {a.size} elements, expected "
+                             f"{rows} x {naux}")
+        return a.reshape(rows, naux)
+
+    ia = as2d(iaP, nocc * nvir, "iaP")
+    mi = as2d(ijP, nmo * nocc, "miP")
+    ab = as2d(abP, nvir * nvir, "abP")
+    tg = np.arange(nmo, dtype=np.int32) if targets is None else np.asarray(targets, np.int32)
     vxc = np.asarray(vxc_dft, dtype=np.float64)
     if sigma_x is None:
         from .core import compute_exchange_diag
-        sx = compute_exchange_diag(mi, nmo, nocc, device=device)
+        sx = compute_exchange_diag(symmetrized_mi(ia, mi, nocc, nvir, naux, device), nmo, nocc,
+                                   device=device)
     else:
         sx = np.asarray(sigma_x, dtype=np.float64)
-    static = (sx - vxc)[tg]
     B = symmetrized_ov(ia, mi, nocc, nvir, naux, device)
     pairs = pairs_from_blocks(ia, mi, ab, nocc, nvir, naux, tg, device)
     newton = dict(max_iterations=int(newton_max_iterations),
                   energy_tolerance=float(newton_energy_tolerance))
+    common = dict(n_iter=int(max_cycle), damping=damping, conv_tol=conv_tol,
+                  conv_tol_z=conv_tol_z, diis=bool(diis), diis_space=int(diis_space),
+                  precision=precision, device=device, newton=newton, solver=solver,
+                  frontier_convergence=bool(frontier_convergence),
+                  frontier_window=float(frontier_window))
     if sigma_method == "ac":
-        r = evgw_ac_core(e, B, nocc, tg, pairs=pairs, n_iter=int(max_cycle), damping=damping,
-                         conv_tol=conv_tol, conv_tol_z=conv_tol_z, diis=bool(diis),
-                         diis_space=int(diis_space), nw=int(nfreq), solver=qp_solver,
-                         sigma_x=sx, vxc=vxc, precision=precision, device=device,
-                         newton=newton)
+        r = evgw_ac_core(e, B, nocc, tg, pairs=pairs, nw=int(nfreq), sigma_x=sx, vxc=vxc,
+                         **common)
     elif sigma_method == "cd":
-        r = evgw_core(e, B, nocc, tg, n_iter=int(max_cycle), damping=damping,
-                      conv_tol=conv_tol, conv_tol_z=conv_tol_z, diis=bool(diis),
-                      diis_space=int(diis_space), nfreq=nfreq, eta=eta, precision=precision,
-                      static=static, pairs=pairs, device=device, newton=newton)
+        r = evgw_core(e, B, nocc, tg, nfreq=int(nfreq), eta=eta, static=(sx - vxc)[tg],
+                      pairs=pairs, **common)
     else:
-        raise ValueError(f"sigma_method must be 'cd' or 'ac', got {sigma_method!r}")
+        raise ValueError(f"sigma_method must be 'ac' or 'cd', got {sigma_method!r}")
     qp = r["qp_energies"]
     z = np.ones(nmo)
     z[tg] = r["z_factors"]
     sig = np.zeros(nmo, dtype=np.complex128)
     sig[tg] = r["last"]["sigma_c"]
-    hist = [dict(cycle=h["cycle"], qp_energies=h["qp_energies"], z_factors=z.copy(),
-                 energy_change=h["energy_change"], rms_change=h["energy_change"],
-                 damping_used=damping if h["cycle"] > 0 else 1.0,
-                 converged=(i == len(r["history"]) - 1) and r["converged"])
+    n = len(r["history"])
+    hist = [dict(cycle=i + 1, qp_energies=h["qp_energies"], z_factors=z.copy(),
+                 z_factors_cycle=h["z_factors"], energy_change=h["energy_change"],
+                 rms_change=h["rms_change"], damping_used=float(damping),
+                 converged=(i == n - 1) and r["converged"])
             for i, h in enumerate(r["history"])]
     homo, lumo = qp[nocc - 1], qp[nocc]
     return dict(qp_energies=qp, z_factors=z, converged=r["converged"], n_cycles=r["n_cycles"],
-                final_error=r["history"][-1]["energy_change"], sigma_x=sx, sigma_c_re=sig.real,
-                sigma_c_im=sig.imag, iteration_history=hist, ip=-homo * HARTREE_TO_EV,
-                ea=-lumo * HARTREE_TO_EV, gap=(lumo - homo) * HARTREE_TO_EV)
+                final_error=r["history"][-1]["energy_change"] if n else 0.0, sigma_x=sx,
+                sigma_c_re=sig.real, sigma_c_im=sig.imag, iteration_history=hist, ip=-homo,
+                ea=-lumo, gap=lumo - homo)
 
 
 _RUN_EVGW_REQUIRED = ("mo_energy", "mo_occ", "iaP", "ijP", "chol_v", "vxc_dft")
 
 
 def run_evgw(config: dict):
-    """Dict front-end with the keys of quasix.quasix.run_evgw (quasix/src/evgw.rs:456-677);
-    a missing required key raises KeyError, eta is fixed at 0.01 as there."""
+    """Dict front-end with the keys of quasix.quasix.run_evgw (quasix/src/evgw.rs:456-677):
+    a missing required key raises KeyError; iaP is the 3-D [nocc, nvir, naux] block; abP
+    defaults to zeros; eta (0.01), conv_tol_z (1e-3) and damping_dynamic (off) are fixed;
+    diis defaults to on, frontier_window to 0.1 Ha, n_occ_active / n_vir_active to 3; the
+    AC self-energy and the driver of ``evgw`` as there (freq_int and mnP are unused)."""
     for k in _RUN_EVGW_REQUIRED:
         if k not in config:
             raise KeyError(k)
-    e = np.asarray(config["mo_energy"], dtype=np.float64)
-    occ = np.asarray(config["mo_occ"])
-    nocc = int(np.count_nonzero(occ > 0))
-    nvir = e.size - nocc
-    naux = int(np.asarray(config["chol_v"]).shape[0])
+    ia = np.asarray(config["iaP"], dtype=np.float64)
+    if ia.ndim != 3:
+        raise ValueError(f"iaP must be [nocc, nvir, naux], got shape {ia.shape}")
+    nocc, nvir, naux = ia.shape
     ab = config.get("abP")
     if ab is None:
         ab = np.zeros((nvir * nvir, naux))
-    return evgw(e, occ, config["iaP"], config["ijP"], ab, config["chol_v"], config["vxc_dft"],
-                max_cycle=config.get("max_cycle", 12), conv_tol=config.get("conv_tol", 1e-4),
-                damping=config.get("damping", 0.5), nfreq=config.get("nfreq", 48), eta=0.01,
+    return evgw(np.asarray(config["mo_energy"], dtype=np.float64), np.asarray(config["mo_occ"]),
+                ia.reshape(nocc * nvir, naux), config["ijP"], ab, config["chol_v"],
+                config["vxc_dft"], max_cycle=config.get("max_cycle", 12),
+                conv_tol=config.get("conv_tol", 1e-4), conv_tol_z=1e-3,
+                damping=config.get("damping", 0.5), damping_dynamic=False,
                 diis=config.get("diis", True), diis_space=config.get("diis_space", 6),
+                nfreq=config.get("nfreq", 48), eta=0.01, verbose=config.get("verbose", 1),
+                energy_window=config.get("energy_window", 2.0),
+                frontier_convergence=config.get("frontier_convergence", True),
                 frontier_window=config.get("frontier_window", 0.1),
                 n_occ_active=config.get("n_occ_active", 3),
                 n_vir_active=config.get("n_vir_active", 3),
@@ -411,9 +474,25 @@ def run_evgw(config: dict):
                 newton_energy_tolerance=config.get("newton_energy_tolerance", 1e-8))
 
 
+def _bind_current_stream(obj):
+    """Orders the library's launches after the work torch has queued on its current stream:
+    the handle is bound to that stream when it differs from the last one bound.  torch's
+    default stream (handle 0, the legacy null stream) leaves the library on its own
+    blocking stream, which the null stream already orders against."""
+    s = torch.cuda.current_stream(obj.dev).cuda_stream
+    h = obj.h
+    if getattr(h, "bound_stream", 0) != s:
+        h.check(h.lib.qx_set_stream(h.h, C.c_void_p(s) if s else None))
+        h.bound_stream = s
+
+
 class G0W0Solver:
     """Repeated G0W0 solves on device-resident inputs (no host round trips besides the
-    library's own per-solve status check).  Outputs stay in ``self.out`` (device)."""
+    library's own per-solve status check).  Outputs stay in ``self.out`` (device).
+
+    Ordering: ``run()`` launches on torch's current stream of the device (the handle is
+    rebound when that stream changes), so input updates queued there before ``run()`` are
+    seen and work queued there after it sees the outputs."""
 
     def __init__(self, mo_energy, B_ov, nocc, targets=None, *, nfreq=48, x0=0.5, eta=0.01,
                  precision="bf16x3", device=0, newton=None):
@@ -435,6 +514,7 @@ class G0W0Solver:
         torch.cuda.synchronize(self.dev)
 
     def run(self):
+        _bind_current_stream(self)
         o = self.out
         self.h.check(self.h.lib.qx_g0w0_cd_ov_dev(
             self.h.h, C.byref(self.desc), _ptr(self.e), None, _ptr(self.B), _ptr(self.t_dev),
@@ -461,7 +541,8 @@ class G0W0BatchSolver:
     instances of different sizes.  ``instances``: sequence of (mo_energy, B_ov, nocc,
     targets) with ov-only B_ov [nocc*nvir, naux]; targets None = HOMO-5..LUMO+5.  The first
     ``run()`` executes and tunes the grouped launches; later runs replay them as a HIP graph.
-    Results: ``results()`` gives one dict per instance."""
+    Results: ``results()`` gives one dict per instance.  Ordering as ``G0W0Solver.run``:
+    launches go to torch's current stream."""
 
     def __init__(self, instances, *, nfreq=32, x0=0.5, eta=0.01, precision="bf16x3", device=0,
                  newton=None):
@@ -499,6 +580,7 @@ class G0W0BatchSolver:
         torch.cuda.synchronize(self.dev)
 
     def run(self):
+        _bind_current_stream(self)
         o = self.out
         self.h.check(self.h.lib.qx_g0w0_batch_ov_dev(
             self.h.h, C.byref(self.desc), len(self.arr), self.arr, _ptr(o[0]), _ptr(o[1]),

@@ -44,7 +44,7 @@ def test_library_exports_every_declared_symbol(lib):
     assert set(L.EXPORTED) == exported
 
 
-def test_struct_layouts_match_header(lib):
+def test_struct_layouts_match_header(lib, tmp_path):
     """Compile a tiny C program against the header and compare sizeof/offsetof."""
     import quasix_amd._lib as L
     src = r"""
@@ -52,19 +52,17 @@ def test_struct_layouts_match_header(lib):
 #include <stddef.h>
 #include "qxgw.h"
 int main(void) {
-  printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(qx_newton_config),
+  printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(qx_newton_config),
          sizeof(qx_g0w0_desc), sizeof(qx_stats), offsetof(qx_g0w0_desc, newton),
          offsetof(qx_g0w0_desc, npair), offsetof(qx_stats, seconds),
          offsetof(qx_stats, pi_variant), offsetof(qx_stats, resolved_instances),
          sizeof(qx_batch_instance), offsetof(qx_batch_instance, targets),
-         offsetof(qx_batch_instance, static_terms));
+         offsetof(qx_batch_instance, static_terms), offsetof(qx_g0w0_desc, qp_solver));
   return 0;
 }
 """
-    tmp = os.path.join(ROOT, "_work")
-    os.makedirs(tmp, exist_ok=True)
-    c = os.path.join(tmp, "layout.c")
-    exe = os.path.join(tmp, "layout")
+    c = str(tmp_path / "layout.c")
+    exe = str(tmp_path / "layout")
     open(c, "w").write(src)
     subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), c, "-o", exe], check=True)
     vals = list(map(int, subprocess.run([exe], capture_output=True, text=True,
@@ -73,7 +71,7 @@ int main(void) {
                     L.G0W0Desc.newton.offset, L.G0W0Desc.npair.offset, L.Stats.seconds.offset,
                     L.Stats.pi_variant.offset, L.Stats.resolved_instances.offset,
                     C.sizeof(L.BatchInstance), L.BatchInstance.targets.offset,
-                    L.BatchInstance.static_terms.offset]
+                    L.BatchInstance.static_terms.offset, L.G0W0Desc.qp_solver.offset]
 
 
 @pytest.mark.parametrize("n", [1, 2, 7, 32, 48, 64, 100])
