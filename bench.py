@@ -47,6 +47,21 @@ def measured_traffic(kernel):
     return k["bytes_per_launch"] if k else None
 
 
+COUNTERS = os.path.join(ROOT, "profiles", "mfma_counters.json")
+
+
+def measured_mfma(kernel):
+    """MFMA-pipe busy fraction and issued / algorithmic MFMA flops of ``kernel`` from the
+    rocprofv3 counter passes recorded in profiles/mfma_counters.json ({} if absent)."""
+    try:
+        rec = json.load(open(COUNTERS)).get("kernels", {}).get(kernel)
+    except (OSError, ValueError):
+        return {}
+    if not rec:
+        return {}
+    return {k: rec[k] for k in ("mfma_busy", "mfma_issued_over_algorithmic") if k in rec}
+
+
 def pi_flops(nov, naux, nf):
     """Algorithmic flops of the Pi launch: the symmetric rank-N_ov update of every
     frequency, N_ov * naux * (naux + 1) per frequency (2 flops per multiply-add over the
@@ -60,34 +75,53 @@ def solve_flops(nov, naux, nf, k):
     return nf * (nov * naux * (naux + 1) + naux ** 3 + 2 * naux * naux * k + 2 * naux * k)
 
 
+def host_description():
+    """CPU model, logical CPUs, the BLAS threads numpy will use and numpy's BLAS/LAPACK build."""
+    model = platform.processor() or platform.machine()
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    model = ln.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    threads, blas = None, {}
+    try:
+        import threadpoolctl
+        for p in threadpoolctl.threadpool_info():
+            if p.get("user_api") == "blas":
+                threads = p.get("num_threads")
+                blas = {"internal_api": p.get("internal_api"), "version": p.get("version"),
+                        "architecture": p.get("architecture")}
+                break
+    except Exception:
+        pass
+    try:
+        cfg = np.show_config(mode="dicts")["Build Dependencies"]
+        blas["numpy_blas"] = cfg["blas"].get("name") + " " + str(cfg["blas"].get("version"))
+        blas["numpy_lapack"] = cfg["lapack"].get("name") + " " + str(cfg["lapack"].get("version"))
+    except Exception:
+        pass
+    return {"cpu_model": model, "logical_cpus": os.cpu_count(), "blas_threads": threads,
+            "blas": blas}
+
+
 def cpu_baseline(e, B, nocc, targets):
-    """The numpy/scipy FP64 oracle (restatement of the reference CPU path) on a bounded
-    sample: 4 of the 48 frequencies of the Pi -> Wc -> Wc_nm stage (scaled x12) plus the full
-    Newton stage driven by those values."""
+    """The numpy/scipy FP64 oracle (restatement of the reference CPU path: per-frequency
+    scaled DGEMM, Cholesky inverse, per-target contraction, CD self-energy and Newton) run
+    once in full on the same workload: all 48 frequencies, every target."""
     from oracle import gw_oracle as O
     fr, wt = O.get_scaled_legendre_roots(NF, 0.5)
-    mn = O.assemble_mnP(B, *O.ov_only_blocks(B, nocc, NVIR), nocc, NVIR, NAUX)
-    pairs = O.pair_list(mn, targets)
-    sample = [0, 16, 32, 47]
+    host = host_description()
     t0 = time.perf_counter()
-    w4 = O.wmn_imag(e, B, nocc, pairs[2], fr[sample])
-    t_freq = (time.perf_counter() - t0) / len(sample)
-    wnm = np.repeat(w4, NF // len(sample), axis=0)[:NF]
-    ctx = O.CDContext(e, e, nocc, B, np.asarray(targets), pairs[0], pairs[1], pairs[2], fr, wt,
-                      wnm, 0.01)
-    t1 = time.perf_counter()
-    for t, n in enumerate(targets):
-        f = lambda om, t=t: O.sigma_cd_eval(ctx, t, om + ctx.ef)
-        g = O.linearized_guess(f, e[n], 0.0, ctx.ef)
-        O.NewtonQPSolver(e[n], 0.0, 0.0, ctx.ef).solve(f, g)
-    t_newton = time.perf_counter() - t1
-    t_solve = t_freq * NF + t_newton
-    threads = int(os.environ.get("OPENBLAS_NUM_THREADS", os.cpu_count() or 1))
-    return {"value": 1.0 / t_solve, "unit": "solves/s", "cores": threads, "kind": "port",
-            "sample": f"numpy/scipy FP64 oracle; {len(sample)} of {NF} frequencies of "
-                      f"Pi+Cholesky-inverse+Wc_nm timed ({t_freq * 1e3:.0f} ms/frequency, "
-                      f"scaled to {NF}) + full Newton stage ({t_newton:.2f} s); "
-                      f"{platform.processor() or platform.machine()}"}
+    O.g0w0_cd(e, B, nocc, np.asarray(targets), fr, wt)
+    t_solve = time.perf_counter() - t0
+    return {"value": 1.0 / t_solve, "unit": "solves/s",
+            "cores": host["blas_threads"] or host["logical_cpus"], "kind": "port",
+            "sample": f"one full solve ({NF} frequencies, {len(targets)} targets) of the "
+                      f"numpy/scipy FP64 oracle in {t_solve:.2f} s",
+            "host": host}
 
 
 def run_batch(args):
@@ -306,7 +340,11 @@ def main():
         "roofline": {"kernel": kernel, "bound": "mfma", "achieved": achieved / 1e12,
                      "peak": peak / 1e12, "unit": "TFLOP/s", "frac": achieved / peak,
                      "traffic": measured_traffic(kernel), "flops_per_launch": f_pi,
-                     "launch_ms": t_pi * 1e3},
+                     "launch_ms": t_pi * 1e3,
+                     "traffic_source": "rocprofv3 FETCH_SIZE/WRITE_SIZE passes of this "
+                                       "kernel recorded in profiles/pi_hbm_traffic.json "
+                                       "(not measured in this run)",
+                     **measured_mfma(kernel)},
     }
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(e, B, NOCC, targets)

@@ -1,0 +1,131 @@
+"""GPU: the benchmark configurations at their full sizes.
+
+* nocc 40 / nvir 400 / naux 1800, 48 frequencies, HOMO-5..LUMO+5 (the headline workload,
+  seed 1 as in bench.py): the FP64 GPU path against the CPU oracle to 1e-6 Ha on every QP
+  energy; the FP32 and BF16x3 polarizability paths within 1 meV of FP64.
+* 256 instances, nocc ~ U{8..60}, nvir = 9 nocc, naux = 4 (nocc + nvir), 32 frequencies,
+  HOMO and LUMO (the batched workload): mixed within 1 meV of FP64 on every instance; eight
+  sampled instances against the single-instance FP64 path (1e-10 Ha); one host-generated
+  instance inside the batch against the oracle (1e-6 Ha).
+* nocc 100 / nvir 1200 / naux 5000, 64 frequencies, 8 evGW iterations (the evGW workload):
+  the mixed path within 1 meV of FP64 on every updated energy and on the gap, and the
+  fixed-point property of the FP64 iterates: one more undamped step F(E8) (polarizability
+  and Green's function from E8, QP equation anchored at the mean-field energies) moves by at
+  most 2 max|E8 - E7| (contraction of the damped iteration).
+Each test prints the numbers it checks."""
+import numpy as np
+import pytest
+
+from oracle import gw_oracle as O
+
+pytestmark = pytest.mark.gpu
+MEV = 1.0 / 27211.386245988
+
+
+@pytest.fixture
+def release():
+    yield
+    import quasix_amd._lib as L
+    h = L.handle(0)
+    h.check(h.lib.qx_release_workspace(h.h))
+    import torch
+    torch.cuda.empty_cache()
+
+
+def test_headline_fp64_vs_oracle_and_mixed_precisions(gpu, release):
+    import time
+    from quasix_amd import gw
+    nocc, nvir, naux, nf = 40, 400, 1800, 48
+    e, B = O.make_inputs(nocc, nvir, naux, seed=1)
+    tg = gw.frontier_targets(nocc, nocc + nvir)
+    r64 = gw.g0w0(e, B, nocc, tg, nfreq=nf, precision="fp64")
+    fr, wt = O.get_scaled_legendre_roots(nf, 0.5)
+    t0 = time.perf_counter()
+    ref = O.g0w0_cd(e, B, nocc, tg, fr, wt)
+    t_ref = time.perf_counter() - t0
+    err = float(np.max(np.abs(r64["qp_energies"] - ref["E"])))
+    print(f"c1 FP64 GPU vs oracle ({t_ref:.1f} s on the host): max |dE_QP| {err:.3e} Ha, "
+          f"max |dZ| {np.max(np.abs(r64['z_factors'] - ref['Z'])):.3e}")
+    assert err < 1e-6
+    assert np.all(r64["state"] == 0)
+    for prec in ("fp32", "bf16x3"):
+        r = gw.g0w0(e, B, nocc, tg, nfreq=nf, precision=prec)
+        d = np.abs(r["qp_energies"] - r64["qp_energies"]) / MEV
+        print(f"c1 {prec} vs FP64: max |dE_QP| {d.max():.3e} meV "
+              f"(HOMO {d[5]:.3e}, LUMO {d[6]:.3e})")
+        assert d.max() < 1.0
+
+
+def test_batch_256_instances(gpu, release):
+    import torch
+    from quasix_amd import gw, synthetic
+    nf = 32
+    shapes = synthetic.batch_shapes(256, seed=2)
+    dev = torch.device("cuda", 0)
+    small = int(np.argmin([o for o, _, _ in shapes]))
+    inst = []
+    for q, (o, v, a) in enumerate(shapes):
+        if q == small:                          # host-generated, checked against the oracle
+            e, B = synthetic.make_inputs(o, v, a, seed=77)
+            e, B = torch.as_tensor(e, device=dev), torch.as_tensor(B, device=dev)
+        else:
+            e, B = synthetic.make_inputs_device(o, v, a, seed=1000 + q, device=dev)
+        inst.append((e, B, o, np.array([o - 1, o], np.int32)))
+    torch.cuda.synchronize()
+    res = {}
+    for prec in ("bf16x3", "fp64"):
+        s = gw.G0W0BatchSolver(inst, nfreq=nf, precision=prec)
+        s.run()
+        res[prec] = s.results()
+        print(f"batch {prec}: waves {s.stats.waves}, re-solved {s.stats.resolved_instances}")
+        del s
+    dmax = max(float(np.max(np.abs(a["qp_energies"] - b["qp_energies"])))
+               for a, b in zip(res["bf16x3"], res["fp64"]))
+    print(f"batch mixed vs FP64 over {len(inst)} instances: max |dE_QP| {dmax / MEV:.3e} meV")
+    assert dmax < MEV
+    rng = np.random.default_rng(5)
+    sample = sorted(set(rng.choice(len(inst), 8, replace=False).tolist()) | {small})
+    worst = 0.0
+    for q in sample:
+        e, B, o, tg = inst[q]
+        single = gw.g0w0(e, B, o, tg, nfreq=nf, precision="fp64")
+        worst = max(worst, float(np.max(np.abs(single["qp_energies"]
+                                               - res["fp64"][q]["qp_energies"]))))
+    print(f"batch FP64 vs single-instance FP64 on {len(sample)} instances: {worst:.3e} Ha")
+    assert worst < 1e-10
+    o, v, a = shapes[small]
+    e, B = synthetic.make_inputs(o, v, a, seed=77)
+    fr, wt = O.get_scaled_legendre_roots(nf, 0.5)
+    ref = O.g0w0_cd(e, B, o, np.array([o - 1, o]), fr, wt)
+    err = float(np.max(np.abs(res["fp64"][small]["qp_energies"] - ref["E"])))
+    print(f"batch instance {small} ({o}/{v}/{a}) vs oracle: {err:.3e} Ha")
+    assert err < 1e-6
+
+
+def test_evgw_c4_mixed_vs_fp64_and_fixed_point(gpu, release):
+    import torch
+    from quasix_amd import gw, synthetic
+    nocc, nvir, naux, nf, nit = 100, 1200, 5000, 64, 8
+    dev = torch.device("cuda", 0)
+    e, B = synthetic.make_inputs_device(nocc, nvir, naux, seed=4, device=dev)
+    torch.cuda.synchronize()
+    tg = gw.frontier_targets(nocc, nocc + nvir)
+    out = {}
+    for prec in ("bf16x3", "fp64"):
+        out[prec] = gw.evgw_core(e, B, nocc, tg, n_iter=nit, damping=0.5, conv_tol=0.0,
+                                 nfreq=nf, precision=prec)
+        ch = ", ".join(f"{x['energy_change']:.2e}" for x in out[prec]["history"])
+        print(f"c4 {prec}: window changes {ch}")
+    qm, q64 = out["bf16x3"]["qp_energies"], out["fp64"]["qp_energies"]
+    d = float(np.max(np.abs(qm[tg] - q64[tg])))
+    gap = abs((qm[nocc] - qm[nocc - 1]) - (q64[nocc] - q64[nocc - 1]))
+    print(f"c4 mixed vs FP64 after {nit} iterations: max |dE| {d / MEV:.3e} meV, gap error "
+          f"{gap / MEV:.3e} meV")
+    assert d < MEV and gap < MEV
+    h = out["fp64"]["history"]
+    e8, e7 = h[-1]["qp_energies"], h[-2]["qp_energies"]
+    step = float(np.max(np.abs(e8 - e7)))
+    f8 = gw.g0w0(e8, B, nocc, tg, nfreq=nf, precision="fp64", mo_energy_ref=e, guess=e8[tg])
+    res = float(np.max(np.abs(f8["qp_energies"] - e8[tg])))
+    print(f"c4 FP64 fixed point: |F(E8) - E8| {res:.3e} Ha, |E8 - E7| {step:.3e} Ha")
+    assert res <= 2.0 * step + 1e-9
