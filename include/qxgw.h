@@ -235,6 +235,26 @@ QXGW_API qx_status qx_g0w0_cd_ov_dev(qx_handle h, const qx_g0w0_desc* d,
                                      const double* static_terms, const double* guess,
                                      double* qp_energy, double* z_factor, double* sigma_re,
                                      double* sigma_im, int* state, qx_stats* stats);
+/* qx_g0w0_cd_ov_dev that allocates nothing.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory during the solve (tight serving loops,
+ * harnesses that forbid hipMalloc).  Every workspace buffer is carved from a static device
+ * array of QXGW_NOALLOC_SCRATCH_BYTES per GPU (held until the process ends; its address is
+ * resolved per device in qx_create); the caller still passes inputs and outputs.  A problem
+ * whose workspace exceeds the scratch is rejected with QX_ERR_INVALID_INPUT before anything
+ * runs (in FP64 about naux <= 950 at nfreq 16, <= 640 at nfreq 32); profiling must be off;
+ * the BF16X3 Pi
+ * instance is the tuned one if this handle tuned that shape before, else the first
+ * candidate (no timing); the factorisation runs on the handle's stream only.  All calls
+ * on one GPU share its scratch, which keeps the last call's contents: calls that overlap
+ * on one GPU (other handles, other threads) race. */
+#define QXGW_NOALLOC_SCRATCH_BYTES (256ull << 20)
+QXGW_API qx_status qx_g0w0_cd_ov_noalloc_dev(qx_handle h, const qx_g0w0_desc* d,
+                                             const double* mo_energy,
+                                             const double* mo_energy_ref, const double* B_ov,
+                                             const int* targets, const double* static_terms,
+                                             const double* guess, double* qp_energy,
+                                             double* z_factor, double* sigma_re,
+                                             double* sigma_im, int* state, qx_stats* stats);
 /* ---- DF tensor construction ----------------------------------------------------------- */
 /* AO -> MO transform of Cholesky-normalised 3-centre vectors (device pointers, row-major):
  * b_ov[i * nvir + a][P] = sum_{mu,nu} c_occ[mu][i] j3c_ao[mu][nu][P] c_vir[nu][a], with

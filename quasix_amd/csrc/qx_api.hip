@@ -19,6 +19,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -48,13 +49,46 @@ struct qx_context {
     StreamFan fan;
     // last batched solve: work tables, workspace layout and its captured graph
     BatchPlan* batch = nullptr;
+    // no-allocation mode (set for the duration of a *_noalloc call): the workspace is carved
+    // from the device's static scratch region instead of being allocated
+    char* arena = nullptr;
+    size_t arena_size = 0, arena_used = 0;
+    std::map<std::string, std::pair<void*, size_t>> arena_bufs;
+    char* scratch_dev = nullptr;          // this device's copy of `scratch` (set up in qx_create)
+    hipError_t scratch_status = hipErrorNotInitialized;
 };
+
+// Device scratch of the no-allocation entry point (qx_g0w0_cd_ov_noalloc_dev): every GPU the
+// library runs on holds one copy for the life of the process.  Host code never uses this
+// name as an address: qx_create resolves each device's copy once with hipGetSymbolAddress
+// and the entry point passes that pointer (and offsets into it) to the kernels.
+static __device__ __align__(256) unsigned char scratch[QXGW_NOALLOC_SCRATCH_BYTES];
 
 namespace {
 
 struct QxFail {
     qx_status st;
 };
+
+// Per-device address of `scratch`, resolved once per device with that device current.
+struct ScratchAddr {
+    std::mutex mu;
+    std::map<int, std::pair<hipError_t, char*>> dev;
+};
+ScratchAddr& scratch_table() {
+    static ScratchAddr t;
+    return t;
+}
+std::pair<hipError_t, char*> scratch_address(int device) {
+    ScratchAddr& t = scratch_table();
+    std::lock_guard<std::mutex> lk(t.mu);
+    auto it = t.dev.find(device);
+    if (it != t.dev.end()) return it->second;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    t.dev[device] = {e, e == hipSuccess ? static_cast<char*>(p) : nullptr};
+    return t.dev[device];
+}
 
 void set_err(qx_handle h, const char* fmt, ...) {
     char buf[1024];
@@ -165,6 +199,21 @@ void tune_store(const std::string& key, int v, const std::vector<float>& ms) {
 template <typename T>
 T* ws(qx_handle h, const std::string& name, size_t count) {
     const size_t bytes = std::max<size_t>(count * sizeof(T), 256);
+    if (h->arena) {
+        // no-allocation mode: named buffers are carved from the fixed scratch region
+        auto at = h->arena_bufs.find(name);
+        if (at != h->arena_bufs.end() && at->second.second >= bytes)
+            return static_cast<T*>(at->second.first);
+        const size_t off = (h->arena_used + 255) / 256 * 256;
+        REQUIRE(h, off <= h->arena_size && bytes <= h->arena_size - off, QX_ERR_ALLOC,
+                "problem too large for the %zu-byte scratch: buffer '%s' needs %zu bytes "
+                "with %zu in use",
+                h->arena_size, name.c_str(), bytes, off);
+        void* p = h->arena + off;
+        h->arena_bufs[name] = {p, bytes};
+        h->arena_used = off + bytes;
+        return static_cast<T*>(p);
+    }
     auto it = h->bufs.find(name);
     if (it != h->bufs.end() && it->second.second >= bytes) return static_cast<T*>(it->second.first);
     if (it != h->bufs.end()) {
@@ -370,6 +419,8 @@ int choose_variant(qx_handle h, const char* kind, int precision, int nvariants,
     std::vector<int> cand(candidates);
     if (cand.empty())
         for (int v = 0; v < nvariants; ++v) cand.push_back(v);
+    // no-allocation mode times nothing (no events are created): first candidate
+    if (h->arena) return cand.front() < nvariants ? cand.front() : 0;
     std::vector<float> ms(nvariants, -1.f);
     hipEvent_t a = take_event(h), b = take_event(h);
     int best = cand.front();
@@ -415,7 +466,7 @@ const StreamFan* potrf_fan(qx_handle h, int& groups) {
         const int g = v ? atoi(v) : 3;
         return std::max(1, std::min(g, StreamFan::kMax + 1));
     }();
-    groups = env_groups;
+    groups = h->arena ? 1 : env_groups;   // no side streams are created in no-allocation mode
     if (groups > 1 && h->fan.n < groups - 1) {
         if (!h->fan.fork) HIPCK(h, hipEventCreateWithFlags(&h->fan.fork, hipEventDisableTiming));
         while (h->fan.n < groups - 1) {
@@ -827,6 +878,10 @@ qx_status qx_create(qx_handle* out, int device) {
         return QX_ERR_DEVICE;
     }
     c->own_stream = true;
+    // the device's copy of the no-allocation scratch, resolved once per device
+    const auto sa = scratch_address(device);
+    c->scratch_status = sa.first;
+    c->scratch_dev = sa.second;
     *out = c;
     return QX_OK;
 }
@@ -1368,6 +1423,36 @@ qx_status qx_g0w0_cd_dev(qx_handle h, const qx_g0w0_desc* d, const double* mo_en
 }
 
 namespace {
+// Upper estimate of the workspace of one ov-only solve (the buffers build_wnm, setup_cd and
+// run_g0w0 request), without the residue path's per-job buffers: the no-allocation entry
+// rejects problems beyond its scratch with it before launching anything.  The pair count
+// assumes every target pairs with max(nocc, nvir) partners.
+size_t g0w0_ov_workspace_bytes(const qx_g0w0_desc* d) {
+    const size_t nf = d->nfreq, nt = d->ntarget, naux = d->naux;
+    const size_t nov = (size_t)d->nocc * d->nvir;
+    const size_t npad = round_up(d->naux, NB), nblk = npad / NB;
+    const size_t kpad = round_up((int)std::min<size_t>(nov, 1u << 30), KPAD);
+    const size_t npair = nt * (size_t)std::max(d->nocc, d->nvir);
+    const size_t sq = nf * npad * npad;
+    size_t b = 0;
+    auto add = [&](size_t bytes) { b += (std::max<size_t>(bytes, 256) + 255) / 256 * 256; };
+    add(nf * 8); add(nf * 8);                                   // freqs, wts
+    add(nt * 8); add((nt + 1) * 4); add(nt * 4); add(nt * 4);   // static, offsets, state, iters
+    add((nt + 1) * 4); add(npair * 4); add(npair * 4);          // ov offsets, pair t, pair m
+    add(npair * naux * 8);                                       // ov pair rows
+    add(nf * npair * 8);                                         // wnm
+    add(nf * kpad * 8); add(sq * 8); add(sq * 8);                // weights, eps -> L, X
+    add(nf * nblk * NB * NB * 8); add(nf * 4);                   // diagonal inverses, flags
+    add(npair * npad * 8); add(nf * nblk * npair * 8);           // padded rows, partials
+    if (d->precision == QX_PREC_BF16X3) {
+        const size_t brows = round_up((int)npad, 256), npp = round_up((int)npair, 128);
+        add(nf * kpad * 4);                                      // FP32 weights
+        add(brows * kpad * 2); add(brows * kpad * 2); add(brows * kpad * 4);
+        add(npp * npad * 2); add(npp * npad * 2); add(sq * 2); add(sq * 2);
+    }
+    return b;
+}
+
 // qx_g0w0_cd_ov_dev without the error guard: gathers the ov-only pair rows on the device.
 qx_status run_g0w0_ov(qx_handle h, const qx_g0w0_desc* d, const double* mo_energy,
                       const double* mo_energy_ref, const double* B_ov, const int* targets,
@@ -1408,6 +1493,47 @@ qx_status qx_g0w0_cd_ov_dev(qx_handle h, const qx_g0w0_desc* d, const double* mo
                             double* z_factor, double* sigma_re, double* sigma_im, int* state,
                             qx_stats* stats) {
     return guarded(h, [&] {
+        return run_g0w0_ov(h, d, mo_energy, mo_energy_ref, B_ov, targets, static_terms, guess,
+                           qp_energy, z_factor, sigma_re, sigma_im, state, stats);
+    });
+}
+
+qx_status qx_g0w0_cd_ov_noalloc_dev(qx_handle h, const qx_g0w0_desc* d,
+                                    const double* mo_energy, const double* mo_energy_ref,
+                                    const double* B_ov, const int* targets,
+                                    const double* static_terms, const double* guess,
+                                    double* qp_energy, double* z_factor, double* sigma_re,
+                                    double* sigma_im, int* state, qx_stats* stats) {
+    return guarded(h, [&] {
+        REQUIRE(h, h->scratch_status == hipSuccess && h->scratch_dev, QX_ERR_DEVICE,
+                "scratch address of device %d unavailable: %s", h->device,
+                hipGetErrorString(h->scratch_status));
+        REQUIRE(h, !h->prof, QX_ERR_INVALID_INPUT,
+                "profiling creates events; disable it for the no-allocation entry point");
+        // carve every workspace buffer of this call from the scratch; cleared on exit so
+        // the handle's other entry points allocate as before
+        struct Arena {
+            qx_handle h;
+            Arena(qx_handle h_) : h(h_) {
+                h->arena = h->scratch_dev;
+                h->arena_size = QXGW_NOALLOC_SCRATCH_BYTES;
+                h->arena_used = 0;
+                h->arena_bufs.clear();
+            }
+            ~Arena() {
+                h->arena = nullptr;
+                h->arena_size = h->arena_used = 0;
+                h->arena_bufs.clear();
+            }
+        } arena(h);
+        validate_desc(h, d);
+        const size_t need = g0w0_ov_workspace_bytes(d);
+        REQUIRE(h, need <= QXGW_NOALLOC_SCRATCH_BYTES, QX_ERR_INVALID_INPUT,
+                "problem too large for the no-allocation entry point: about %zu bytes of "
+                "workspace needed, the scratch holds %llu (nocc=%d nvir=%d naux=%d nfreq=%d "
+                "ntarget=%d)",
+                need, (unsigned long long)QXGW_NOALLOC_SCRATCH_BYTES, d->nocc, d->nvir, d->naux,
+                d->nfreq, d->ntarget);
         return run_g0w0_ov(h, d, mo_energy, mo_energy_ref, B_ov, targets, static_terms, guess,
                            qp_energy, z_factor, sigma_re, sigma_im, state, stats);
     });
