@@ -256,6 +256,23 @@ QXGW_API qx_status qx_g0w0_cd_ov_noalloc_dev(qx_handle h, const qx_g0w0_desc* d,
                                              double* z_factor, double* sigma_re,
                                              double* sigma_im, int* state, qx_stats* stats);
 /* ---- DF tensor construction ----------------------------------------------------------- */
+/* Assembly of the symmetrised DF tensor from the caller's blocks, on the device: with
+ * X[:, :nocc] = miP [nmo][nocc][naux], X[:nocc, nocc:] = iaP [nocc][nvir][naux] and
+ * X[nocc:, nocc:] = abP [nvir][nvir][naux], the symmetrised rows (X[n][m] + X[m][n]) / 2.
+ * Outputs (each nullable): B_ov_sym [nocc*nvir][naux] (the polarizability's operand),
+ * mi_sym [nmo][nocc][naux] (the exchange block), and for the ntarget targets the
+ * target-partner rows that are not identically zero, in (target, partner) order:
+ * B_pair [npair][naux], pair_target (target slot), pair_m (partner), *npair <= pair_capacity
+ * (ntarget * nmo always suffices; a smaller capacity that cannot hold them is
+ * QX_ERR_DIMENSION_MISMATCH, as is a target outside the orbitals).  abP may be NULL when
+ * ntarget is 0.  The [nmo][nmo][naux] tensor itself is never formed.  Replaces the block
+ * assembly of quasix/src/evgw.rs:161-205 and the df_mi slice of :305. */
+QXGW_API qx_status qx_assemble_blocks_dev(qx_handle h, int nocc, int nvir, int naux,
+                                          const double* iaP, const double* miP,
+                                          const double* abP, int ntarget, const int* targets,
+                                          double* B_ov_sym, double* mi_sym, int pair_capacity,
+                                          double* B_pair, int* pair_target, int* pair_m,
+                                          int* npair);
 /* AO -> MO transform of Cholesky-normalised 3-centre vectors (device pointers, row-major):
  * b_ov[i * nvir + a][P] = sum_{mu,nu} c_occ[mu][i] j3c_ao[mu][nu][P] c_vir[nu][a], with
  * j3c_ao [nao][nao][naux], c_occ [nao][nocc], c_vir [nao][nvir]; FP64 MFMA GEMMs through a

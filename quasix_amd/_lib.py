@@ -31,6 +31,7 @@ EXPORTED = [
     "qx_sigma_ac_dev", "qx_thiele", "qx_pade_thiele", "qx_ac_pade_thiele_diag", "qx_qp_solve_ac",
     "qx_exchange_diag_dev", "qx_df_mo_transform_dev", "qx_screening_w_dev",
     "qx_sigma_c_pyscf_cd_dev", "qx_solve_qp_linearized", "qx_g0w0_cd_ov_noalloc_dev",
+    "qx_assemble_blocks_dev",
 ]
 
 
@@ -120,6 +121,9 @@ def load():
         lib.qx_g0w0_cd_ov_dev.argtypes = [vp, C.POINTER(G0W0Desc), dp, dp, dp, ip, dp, dp, dp, dp,
                                           dp, dp, ip, C.POINTER(Stats)]
         lib.qx_g0w0_cd_ov_noalloc_dev.argtypes = lib.qx_g0w0_cd_ov_dev.argtypes
+        lib.qx_assemble_blocks_dev.argtypes = [vp, C.c_int, C.c_int, C.c_int, dp, dp, dp,
+                                               C.c_int, ip, dp, dp, C.c_int, dp, ip, ip,
+                                               C.POINTER(C.c_int)]
         lib.qx_g0w0_adjoint_ov_dev.argtypes = [vp, C.POINTER(G0W0Desc), dp, dp, ip, dp, dp, dp, dp]
         lib.qx_g0w0_batch_ov_dev.argtypes = [vp, C.POINTER(G0W0Desc), C.c_int,
                                              C.POINTER(BatchInstance), dp, dp, dp, dp, ip,

@@ -236,6 +236,139 @@ __global__ __launch_bounds__(256, 2) void pi_syrk_kernel(
                                   As, Bs);
 }
 
+// ---- FP64, two frequencies per tile ---------------------------------------------------------
+// 512 threads own one 128x128 lower tile for the frequencies f0 and f0 + 1: the N_ov rows of
+// B are staged once, unscaled, and serve both; the weight s_f(r) multiplies the row-side
+// MFMA operand in registers (Pi_IJ = sum_r (s_f(r) B_rI) B_rJ).  Eight waves of 64 x 32
+// (4 x 2 blocks of 16x16x4 MFMAs, 2 x 32 accumulators); per k step a wave reads 6 operand
+// values from LDS for 16 MFMAs, and the global operand traffic per flop is half that of the
+// one-frequency kernel.
+constexpr int P2_KC = 16;
+
+template <bool Guarded>
+__device__ __forceinline__ void pi_f64_pair_body(const double* __restrict__ B,
+                                                 const double* __restrict__ s, int nov,
+                                                 int naux, int npad, int sld, int nf,
+                                                 double* __restrict__ out, PiOut mode, int I0,
+                                                 int J0, int f0, double (*As)[TILE + 2],
+                                                 double (*Bs)[TILE + 2],
+                                                 double (*Ss)[P2_KC]) {
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int wr = wave >> 2, wc = wave & 3;            // 64-row x 32-column wave tile
+    const bool two = f0 + 1 < nf;                       // wave-uniform
+    constexpr int PER = P2_KC * TILE / 512;
+    const int c = tid % TILE, kr0 = tid / TILE;         // column, first row of the thread
+    const bool ca = !Guarded || I0 + c < naux, cb = !Guarded || J0 + c < naux;
+    double ra[PER], rb[PER], rs = 0.0;
+    const int sf_i = tid / P2_KC, sk = tid % P2_KC;     // weight staging: threads 0 .. 31
+    auto fetch = [&](int r0) {
+        #pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            const int r = r0 + kr0 + 4 * q;
+            double a = 0.0, b = 0.0;
+            if (r < nov) {
+                const double* row = B + (size_t)r * naux;
+                if (ca) a = row[I0 + c];
+                if (cb) b = row[J0 + c];
+            }
+            ra[q] = a;
+            rb[q] = b;
+        }
+        if (tid < 2 * P2_KC) {
+            const int f = f0 + sf_i, r = r0 + sk;
+            rs = (f < nf && r < nov) ? s[(size_t)f * sld + r] : 0.0;
+        }
+    };
+    d4 acc[2][4][2];
+    #pragma unroll
+    for (int f = 0; f < 2; ++f)
+        #pragma unroll
+        for (int bi = 0; bi < 4; ++bi)
+            #pragma unroll
+            for (int bj = 0; bj < 2; ++bj) acc[f][bi][bj] = d4{0, 0, 0, 0};
+    auto live = [&](int bi, int bj) {
+        const int r = I0 + wr * 64 + bi * 16, cc = J0 + wc * 32 + bj * 16;
+        return !Guarded || (r < naux && cc < naux && (I0 != J0 || r >= cc));
+    };
+    fetch(0);
+    for (int r0 = 0; r0 < nov; r0 += P2_KC) {
+        #pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            As[kr0 + 4 * q][c] = ra[q];
+            Bs[kr0 + 4 * q][c] = rb[q];
+        }
+        if (tid < 2 * P2_KC) Ss[sf_i][sk] = rs;
+        __syncthreads();
+        if (r0 + P2_KC < nov) fetch(r0 + P2_KC);        // in flight during the MFMA phase
+        #pragma unroll
+        for (int kk = 0; kk < P2_KC; kk += 4) {
+            const int k = kk + (lane >> 4);
+            double av[4], bv[2];
+            #pragma unroll
+            for (int q = 0; q < 4; ++q) av[q] = As[k][wr * 64 + q * 16 + (lane & 15)];
+            #pragma unroll
+            for (int q = 0; q < 2; ++q) bv[q] = Bs[k][wc * 32 + q * 16 + (lane & 15)];
+            const double s0 = Ss[0][k], s1 = Ss[1][k];
+            #pragma unroll
+            for (int bi = 0; bi < 4; ++bi) {
+                const double a0 = av[bi] * s0, a1 = av[bi] * s1;
+                #pragma unroll
+                for (int bj = 0; bj < 2; ++bj)
+                    if (live(bi, bj)) {
+                        acc[0][bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, bv[bj],
+                                                                             acc[0][bi][bj], 0, 0, 0);
+                        if (two)
+                            acc[1][bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(
+                                a1, bv[bj], acc[1][bi][bj], 0, 0, 0);
+                    }
+            }
+        }
+        __syncthreads();
+    }
+    #pragma unroll
+    for (int f = 0; f < 2; ++f) {
+        if (f == 1 && !two) break;
+        #pragma unroll
+        for (int bi = 0; bi < 4; ++bi)
+            #pragma unroll
+            for (int bj = 0; bj < 2; ++bj)
+                #pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int row = I0 + wr * 64 + bi * 16 + mfma_row_f64(lane, v);
+                    const int col = J0 + wc * 32 + bj * 16 + (lane & 15);
+                    const double p = acc[f][bi][bj][v];
+                    const size_t fo = (size_t)(f0 + f) * npad;
+                    if (mode == PiOut::EpsLower) {
+                        if (row < npad && col < npad)
+                            out[(fo + row) * npad + col] = (row == col ? 1.0 : 0.0) - p;
+                    } else if (row < naux && col < naux && (I0 != J0 || row >= col)) {
+                        out[(fo + row) * npad + col] = p;
+                        out[(fo + col) * npad + row] = p;
+                    }
+                }
+    }
+}
+
+__global__ __launch_bounds__(512, 1) void pi_f64_pair_kernel(
+    const double* __restrict__ B, const double* __restrict__ s, int nov, int naux, int npad,
+    int nf, int ntiles, int sld, double* __restrict__ out, PiOut mode) {
+    __shared__ double As[P2_KC][TILE + 2];
+    __shared__ double Bs[P2_KC][TILE + 2];
+    __shared__ double Ss[2][P2_KC];
+    const int nfp = (nf + 1) / 2;
+    const int work = xcd_remap(blockIdx.x, ntiles * nfp);
+    const int tile = work / nfp, f0 = 2 * (work % nfp);
+    int I, J;
+    tri_index(tile, I, J);
+    if (tile_full(I * TILE, J * TILE, naux))
+        pi_f64_pair_body<false>(B, s, nov, naux, npad, sld, nf, out, mode, I * TILE, J * TILE,
+                                f0, As, Bs, Ss);
+    else
+        pi_f64_pair_body<true>(B, s, nov, naux, npad, sld, nf, out, mode, I * TILE, J * TILE,
+                               f0, As, Bs, Ss);
+}
+
 // ---- BF16x3 kernel ----------------------------------------------------------------------
 // Operands are staged K-contiguous: LDS rows are auxiliary indices, 32 N_ov rows per
 // chunk, so each lane fetches its 8 consecutive k values with one 16-byte read.
@@ -1353,8 +1486,12 @@ hipError_t launch_pi_syrk(int precision, const double* B, const PiOperands& ops,
     const int sld = round_up(nov, KPAD);
     dim3 grid(ntiles * nf), block(256);
     if (precision == 0) {
-        hipLaunchKernelGGL((pi_syrk_kernel<double, 16>), grid, block, 0, st, B, s, nov, naux,
-                           npad, nf, ntiles, sld, out, mode);
+        if (variant == 1)
+            hipLaunchKernelGGL(pi_f64_pair_kernel, dim3(ntiles * ((nf + 1) / 2)), dim3(512), 0, st,
+                               B, s, nov, naux, npad, nf, ntiles, sld, out, mode);
+        else
+            hipLaunchKernelGGL((pi_syrk_kernel<double, 16>), grid, block, 0, st, B, s, nov, naux,
+                               npad, nf, ntiles, sld, out, mode);
     } else if (precision == 1) {
         hipLaunchKernelGGL((pi_syrk_kernel<float, 32>), grid, block, 0, st, B, s, nov, naux,
                            npad, nf, ntiles, sld, out, mode);
@@ -1420,7 +1557,7 @@ hipError_t launch_pi_syrk(int precision, const double* B, const PiOperands& ops,
     return hipGetLastError();
 }
 
-int pi_variant_count(int precision) { return precision == 2 ? 11 : 1; }
+int pi_variant_count(int precision) { return precision == 2 ? 11 : precision == 0 ? 2 : 1; }
 
 hipError_t launch_pi_group(int precision, const PiInst* inst_dev, const int* off_dev, int ninst,
                            int nf, int total, int grid, hipStream_t st, int variant) {

@@ -403,7 +403,7 @@ struct WnmResult {
 template <class Launch>
 int choose_variant(qx_handle h, const char* kind, int precision, int nvariants,
                    std::initializer_list<int> shape, Launch launch,
-                   std::initializer_list<int> candidates = {}) {
+                   const std::vector<int>& candidates = {}) {
     if (nvariants <= 1) return 0;
     if (const char* f = std::getenv("QXGW_PI_VARIANT")) {
         const int v = std::atoi(f);
@@ -416,16 +416,18 @@ int choose_variant(qx_handle h, const char* kind, int precision, int nvariants,
     tune_load(h);
     auto it = h->tuned.find(key);
     if (it != h->tuned.end() && it->second < nvariants) return it->second;
-    std::vector<int> cand(candidates);
+    // the listed candidates that exist for this precision, else every variant
+    std::vector<int> cand;
+    for (int v : candidates)
+        if (v >= 0 && v < nvariants) cand.push_back(v);
     if (cand.empty())
         for (int v = 0; v < nvariants; ++v) cand.push_back(v);
     // no-allocation mode times nothing (no events are created): first candidate
-    if (h->arena) return cand.front() < nvariants ? cand.front() : 0;
+    if (h->arena) return cand.front();
     std::vector<float> ms(nvariants, -1.f);
     hipEvent_t a = take_event(h), b = take_event(h);
     int best = cand.front();
     for (int v : cand) {
-        if (v < 0 || v >= nvariants) continue;
         launch(v);                                   // warm-up (code object load, caches)
         HIPCK(h, hipEventRecord(a, h->stream));
         launch(v);
@@ -456,7 +458,12 @@ int tune_nf(int naux_or_npad, int nf) {
 // Autotuner candidates of the BF16x3 Pi kernels: the instances that win or come within a few
 // per cent at the measured shapes (the producer/consumer forms, K64 on 32x32x16 MFMAs, K64 on
 // 16x16x32); the others stay selectable by QXGW_PI_VARIANT.
-#define QX_PI_CANDIDATES {8, 5, 7, 9, 1}
+// FP64 has the one-frequency (0) and the frequency-pair (1) instance, both timed.
+std::vector<int> pi_candidates(int precision) {
+    if (precision == QX_PREC_BF16X3) return {8, 5, 7, 9, 1};
+    if (precision == QX_PREC_FP64) return {1, 0};
+    return {};
+}
 
 // Side streams and the group count for the factorisation step loop (QXGW_POTRF_GROUPS,
 // default 3; 1 runs everything on the handle's stream).
@@ -532,7 +539,7 @@ WnmResult build_wnm(qx_handle h, int precision, int nocc, int nvir, int naux,
                                   {naux, nov, nf}, [&](int v) {
         HIPCK(h, launch_pi_syrk(precision, B_ov, ops, s, nov, naux, npad, tune_nf(npad, nf),
                                 eps_mat, PiOut::EpsLower, st, v));
-    }, QX_PI_CANDIDATES);
+    }, pi_candidates(precision));
     r.pi_variant = pv;
     {
         Phase ph(h, QX_PHASE_PI);
@@ -1040,7 +1047,7 @@ qx_status qx_rho_response_dev(qx_handle h, int precision, int nocc, int nvir, in
                                       {naux, nov, nfreq}, [&](int v) {
             HIPCK(h, launch_pi_syrk(precision, B_ov, ops, s, nov, naux, naux,
                                     tune_nf(naux, nfreq), pi_out, PiOut::PiFull, h->stream, v));
-        }, QX_PI_CANDIDATES);
+        }, pi_candidates(precision));
         HIPCK(h, launch_pi_syrk(precision, B_ov, ops, s, nov, naux, naux, nfreq, pi_out,
                                 PiOut::PiFull, h->stream, pv));
         HIPCK(h, hipStreamSynchronize(h->stream));
@@ -1317,6 +1324,50 @@ qx_status qx_screening_w_dev(qx_handle h, int naux, int nfreq, const double* p0,
         HIPCK(h, launch_reduce(w_out, nullptr, (size_t)nf * naux * naux, 2, red, st));
         const double mx = d2h_scalar(h, red);
         REQUIRE(h, std::isfinite(mx), QX_ERR_NUMERICAL, "W contains NaN or Inf values");
+        return QX_OK;
+    });
+}
+
+qx_status qx_assemble_blocks_dev(qx_handle h, int nocc, int nvir, int naux, const double* iaP,
+                                 const double* miP, const double* abP, int ntarget,
+                                 const int* targets, double* B_ov_sym, double* mi_sym,
+                                 int pair_capacity, double* B_pair, int* pair_target,
+                                 int* pair_m, int* npair) {
+    return guarded(h, [&] {
+        REQUIRE(h, nocc > 0 && nvir > 0 && naux > 0 && ntarget >= 0, QX_ERR_INVALID_INPUT,
+                "Invalid dimensions nocc=%d nvir=%d naux=%d ntarget=%d", nocc, nvir, naux,
+                ntarget);
+        REQUIRE(h, iaP && miP, QX_ERR_INVALID_INPUT, "iaP and miP are required");
+        REQUIRE(h, ntarget == 0 || (targets && abP && B_pair && pair_target && pair_m && npair),
+                QX_ERR_INVALID_INPUT, "pair outputs, targets and abP are required for targets");
+        const int nmo = nocc + nvir;
+        hipStream_t st = h->stream;
+        if (npair) *npair = 0;
+        HIPCK(h, launch_sym_blocks(iaP, miP, abP, nocc, nvir, naux, B_ov_sym, mi_sym, st));
+        if (ntarget > 0) {
+            REQUIRE(h, (long long)ntarget * nmo < (1ll << 31), QX_ERR_INVALID_INPUT,
+                    "ntarget * nmo overflows int");
+            std::vector<int> tg(ntarget);
+            d2h(h, tg.data(), targets, ntarget);
+            for (int t = 0; t < ntarget; ++t)
+                REQUIRE(h, tg[t] >= 0 && tg[t] < nmo, QX_ERR_DIMENSION_MISMATCH,
+                        "target %d is outside the %d orbitals", tg[t], nmo);
+            const size_t rows = (size_t)ntarget * nmo;
+            int* flags = ws<int>(h, "asm_flags", rows);
+            int* pos = ws<int>(h, "asm_pos", rows);
+            int* total = ws<int>(h, "asm_total", 1);
+            HIPCK(h, launch_assemble_pairs(iaP, miP, abP, nocc, nvir, naux, targets, ntarget,
+                                           flags, pos, total, st));
+            int n = 0;
+            d2h(h, &n, total, 1);
+            REQUIRE(h, n <= pair_capacity, QX_ERR_DIMENSION_MISMATCH,
+                    "pair_capacity %d is below the %d non-zero target-partner rows",
+                    pair_capacity, n);
+            HIPCK(h, launch_scatter_pairs(iaP, miP, abP, nocc, nvir, naux, targets, ntarget,
+                                          flags, pos, B_pair, pair_target, pair_m, st));
+            *npair = n;
+        }
+        HIPCK(h, hipStreamSynchronize(st));
         return QX_OK;
     });
 }

@@ -272,6 +272,23 @@ hipError_t launch_strided_gemm(bool ta, bool tb, const double* A, size_t sa, con
 // some v entry is non-zero.  b_n^T W b_m = u^T W u - v^T W v for symmetric W.
 hipError_t launch_lpq_pair_rows(const double* lpq, int naux, int nmo, int npad, double* u,
                                 double* v, int* any_v, hipStream_t st);
+// Target-partner rows of the symmetrised tensor assembled from the iaP [nocc][nvir][naux],
+// miP [nmo][nocc][naux] and abP [nvir][nvir][naux] blocks (quasix/src/evgw.rs:161-205), for
+// the targets [nt], rows that are identically zero dropped.  launch_assemble_pairs writes
+// flags[t * nmo + m] (row non-zero), their exclusive prefix pos[] and the row count total[0]
+// (flags, pos: nt * nmo ints); launch_scatter_pairs then writes row pos[r] of rows_out
+// ([total][naux]) with its target slot and partner, in (target, partner) order.
+hipError_t launch_assemble_pairs(const double* ia, const double* mi, const double* ab, int nocc,
+                                 int nvir, int naux, const int* targets, int nt, int* flags,
+                                 int* pos, int* total, hipStream_t st);
+hipError_t launch_scatter_pairs(const double* ia, const double* mi, const double* ab, int nocc,
+                                int nvir, int naux, const int* targets, int nt, const int* flags,
+                                const int* pos, double* rows_out, int* pair_t, int* pair_m,
+                                hipStream_t st);
+// The ov block [nocc * nvir][naux] and the [nmo][nocc][naux] block of the same symmetrised
+// tensor (either output nullable).
+hipError_t launch_sym_blocks(const double* ia, const double* mi, const double* ab, int nocc,
+                             int nvir, int naux, double* ov_sym, double* mi_sym, hipStream_t st);
 // Static CD self-energy at E = 0 for every orbital n (sigma: [nmo] complex, re/im):
 // -1/pi sum_f w_f sum_m W_nm(f) emo / (emo^2 + w_f^2), emo = -eps_m - i eta s,
 // s = +1 if eps_n > eps_m else -1; W_nm(f) = wu[f][n nmo + m] - wv[f][n nmo + m] (wv

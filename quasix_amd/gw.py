@@ -70,51 +70,51 @@ class Pairs:
     rows: torch.Tensor       # f64 [npair, naux]
 
 
-def pairs_from_blocks(iaP, miP, abP, nocc, nvir, naux, targets, device=0) -> Pairs:
-    """Pair rows of the tensor assembled as quasix/src/evgw.rs:161-205 does.
-
-    X[:, :nocc] = miP, X[:nocc, nocc:] = iaP, X[nocc:, nocc:] = abP, then
-    B_nm = (X[n, m] + X[m, n]) / 2.  Only the rows of the targets are formed.
-    """
+def assemble_blocks(iaP, miP, abP, nocc, nvir, naux, targets=None, *, ov=True, mi=True,
+                    device=0):
+    """The symmetrised DF tensor of quasix/src/evgw.rs:161-205 (X[:, :nocc] = miP,
+    X[:nocc, nocc:] = iaP, X[nocc:, nocc:] = abP, then (X[n, m] + X[m, n]) / 2), assembled on
+    the device by qx_assemble_blocks_dev without forming the [nmo, nmo, naux] tensor.
+    Returns (B_ov_sym [nocc*nvir, naux] or None, mi_sym [nmo, nocc, naux] or None,
+    Pairs of the targets or None): the polarizability's operand, the exchange term's block
+    (evgw.rs:305) and the target-partner rows, identically zero rows dropped, in (target,
+    partner) order."""
     dev = _dev(device)
+    h = _lib.handle(device)
     nmo = nocc + nvir
-    ia = _f64(iaP, dev).reshape(nocc, nvir, naux)
-    mi = _f64(miP, dev).reshape(nmo, nocc, naux)
-    ab = _f64(abP, dev).reshape(nvir, nvir, naux)
+    ia = _f64(iaP, dev).reshape(nocc * nvir, naux)
+    m = _f64(miP, dev).reshape(nmo * nocc, naux)
+    ab = None if abP is None else _f64(abP, dev).reshape(nvir * nvir, naux)
+    ov_out = torch.empty((nocc * nvir, naux), dtype=torch.float64, device=dev) if ov else None
+    mi_out = torch.empty((nmo, nocc, naux), dtype=torch.float64, device=dev) if mi else None
+    tg = None if targets is None else np.asarray(targets, dtype=np.int32)
+    nt = 0 if tg is None else len(tg)
+    cap = nt * nmo
+    rows = torch.empty((max(cap, 1), naux), dtype=torch.float64, device=dev)
+    pt = torch.empty(max(cap, 1), dtype=torch.int32, device=dev)
+    pm = torch.empty(max(cap, 1), dtype=torch.int32, device=dev)
+    t_dev = _i32(tg, dev) if nt else None
+    npair = C.c_int(0)
+    torch.cuda.synchronize(dev)
+    h.check(h.lib.qx_assemble_blocks_dev(h.h, nocc, nvir, naux, _ptr(ia), _ptr(m), _ptr(ab), nt,
+                                         _ptr(t_dev), _ptr(ov_out), _ptr(mi_out), cap,
+                                         _ptr(rows), _ptr(pt), _ptr(pm), C.byref(npair)))
+    n = npair.value
+    pairs = Pairs(pt[:n], pm[:n], rows[:n]) if nt else None
+    return ov_out, mi_out, pairs
 
-    def xrow(n):                       # X[n, :, :]  [nmo, naux]
-        r = torch.zeros((nmo, naux), dtype=torch.float64, device=dev)
-        r[:nocc] = mi[n]
-        if n < nocc:
-            r[nocc:] = ia[n]
-        else:
-            r[nocc:] = ab[n - nocc]
-        return r
 
-    def xcol(n):                       # X[:, n, :]  [nmo, naux]
-        if n < nocc:
-            return mi[:, n, :].clone()
-        c = torch.zeros((nmo, naux), dtype=torch.float64, device=dev)
-        c[:nocc] = ia[:, n - nocc, :]
-        c[nocc:] = ab[:, n - nocc, :]
-        return c
-
-    ts, ms, rows = [], [], []
-    for t, n in enumerate(np.asarray(targets).tolist()):
-        row = 0.5 * (xrow(n) + xcol(n))
-        nz = torch.nonzero(torch.any(row != 0.0, dim=1)).flatten()
-        ts.append(torch.full((nz.numel(),), t, dtype=torch.int32, device=dev))
-        ms.append(nz.to(torch.int32))
-        rows.append(row[nz])
-    return Pairs(torch.cat(ts), torch.cat(ms), torch.cat(rows).contiguous())
+def pairs_from_blocks(iaP, miP, abP, nocc, nvir, naux, targets, device=0) -> Pairs:
+    """Target-partner rows of the symmetrised tensor (quasix/src/evgw.rs:161-205), rows that
+    are identically zero dropped (they contribute exactly zero to Wc_nm)."""
+    return assemble_blocks(iaP, miP, abP, nocc, nvir, naux, targets, ov=False, mi=False,
+                           device=device)[2]
 
 
 def symmetrized_ov(iaP, miP, nocc, nvir, naux, device=0):
     """Pi's ov block of the symmetrised tensor: (iaP[i,a] + miP[nocc+a, i]) / 2."""
-    dev = _dev(device)
-    ia = _f64(iaP, dev).reshape(nocc, nvir, naux)
-    mi = _f64(miP, dev).reshape(nocc + nvir, nocc, naux)
-    return (0.5 * (ia + mi[nocc:].permute(1, 0, 2))).reshape(nocc * nvir, naux).contiguous()
+    return assemble_blocks(iaP, miP, None, nocc, nvir, naux, ov=True, mi=False,
+                           device=device)[0]
 
 
 def parse_qp_solver(qp_solver) -> str:
@@ -335,17 +335,6 @@ def evgw_ac_core(mo_energy, B_ov, nocc, targets=None, *, pairs: Pairs | None = N
                 history=history, last=last, targets=tg, window=win)
 
 
-def symmetrized_mi(iaP, miP, nocc, nvir, naux, device=0):
-    """[nmo, nocc, naux] block of the symmetrised tensor, X_sym[m, i] = (X[m, i] + X[i, m]) / 2
-    with X[i, j] = miP[i, j] and X[i, nocc + a] = iaP[i, a]: the df_mi the reference slices
-    from the assembled tensor for the exchange term (quasix/src/evgw.rs:161-205, :305)."""
-    dev = _dev(device)
-    ia = _f64(iaP, dev).reshape(nocc, nvir, naux)
-    mi = _f64(miP, dev).reshape(nocc + nvir, nocc, naux)
-    xt = torch.cat([mi[:nocc].permute(1, 0, 2), ia.permute(1, 0, 2)], dim=0)   # X[i, m] at [m, i]
-    return (0.5 * (mi + xt)).contiguous()
-
-
 def evgw(mo_energy, mo_occ, iaP, ijP, abP, chol_v, vxc_dft, max_cycle=12, conv_tol=1e-4,
          conv_tol_z=1e-3, damping=0.5, damping_dynamic=False, diis=False, diis_space=6,
          nfreq=48, eta=0.01, verbose=1, energy_window=2.0, frontier_convergence=True,
@@ -397,14 +386,14 @@ def evgw(mo_energy, mo_occ, iaP, ijP, abP, chol_v, vxc_dft, max_cycle=12, conv_t
     ab = as2d(abP, nvir * nvir, "abP")
     tg = np.arange(nmo, dtype=np.int32) if targets is None else np.asarray(targets, np.int32)
     vxc = np.asarray(vxc_dft, dtype=np.float64)
+    B, mi_sym, pairs = assemble_blocks(ia, mi, ab, nocc, nvir, naux, tg, ov=True,
+                                       mi=sigma_x is None, device=device)
     if sigma_x is None:
         from .core import compute_exchange_diag
-        sx = compute_exchange_diag(symmetrized_mi(ia, mi, nocc, nvir, naux, device), nmo, nocc,
-                                   device=device)
+        sx = compute_exchange_diag(mi_sym, nmo, nocc, device=device)
+        del mi_sym
     else:
         sx = np.asarray(sigma_x, dtype=np.float64)
-    B = symmetrized_ov(ia, mi, nocc, nvir, naux, device)
-    pairs = pairs_from_blocks(ia, mi, ab, nocc, nvir, naux, tg, device)
     newton = dict(max_iterations=int(newton_max_iterations),
                   energy_tolerance=float(newton_energy_tolerance))
     common = dict(n_iter=int(max_cycle), damping=damping, conv_tol=conv_tol,

@@ -272,3 +272,24 @@ def test_g0w0_bf16x3_every_variant(gpu, variant, monkeypatch):
     rm = gw.g0w0(e, B, nocc, targets, nfreq=24, precision="bf16x3", stats=True)
     assert rm["stats"]["pi_variant"] == variant
     _close(f"E_QP bf16x3 v{variant}", rm["qp_energies"], r64["qp_energies"], MEV)
+
+
+@pytest.mark.parametrize("variant", ["0", "1"])
+@pytest.mark.parametrize("naux,nf", [(150, 6), (300, 7), (64, 1)])
+def test_pi_fp64_both_instances(gpu, variant, naux, nf, monkeypatch):
+    """FP64 Pi, one-frequency (0) and frequency-pair (1) instance, against the oracle: ragged
+    naux, an odd frequency count (the last pair holds one frequency) and a single frequency,
+    in both output forms (full Pi, and eps = I - Pi inside a G0W0 solve)."""
+    from quasix_amd import core, gw
+    monkeypatch.setenv("QXGW_PI_VARIANT", variant)
+    nocc, nvir = 7, 45
+    e, B = O.make_inputs(nocc, nvir, naux, seed=naux + nf)
+    fr, wt = O.get_scaled_legendre_roots(nf, 0.5)
+    occ = np.array([2.0] * nocc + [0.0] * nvir)
+    p0 = core.compute_polarizability_p0(B.reshape(nocc, nvir, naux), e, occ, fr)
+    for k, w in enumerate(fr):
+        ref = O.get_rho_response(w, e, B, nocc)
+        _close(f"pi fp64 v{variant} k={k}", p0[k], ref, 1e-12, np.max(np.abs(ref)))
+    tg = np.arange(nocc - 2, nocc + 2)
+    r = gw.g0w0(e, B, nocc, tg, nfreq=nf)
+    _close(f"g0w0 fp64 v{variant}", r["qp_energies"], O.g0w0_cd(e, B, nocc, tg, fr, wt)["E"], 1e-6)

@@ -129,9 +129,9 @@ def test_screening_w_errors(gpu):
     v[0, 0] = -1.0                           # indefinite metric: no real square root
     with pytest.raises(RuntimeError, match="W\\(iω\\) computation error"):
         core.compute_screening_w(p0, v, np.ones(nf))
-    p0[1] = np.eye(naux)                     # eps = 0 at one frequency
+    p0[1] = np.eye(naux)                     # with v_sqrt = I exactly: eps = 0 at frequency 1
     with pytest.raises(RuntimeError, match="singular at frequency 1"):
-        core.compute_screening_w(p0, np.eye(naux), np.ones(nf))
+        core.compute_screening_w(p0, np.eye(naux), np.ones(nf), np.eye(naux))
 
 
 def test_screened_wc_identity_fallback(gpu):
