@@ -227,6 +227,43 @@ def run_evgw(args):
     print(json.dumps(line), flush=True)
 
 
+def run_ac(args):
+    """AC self-energy workload (the reference's live method, correlation_ac.rs:241-449):
+    Sigma_c(i nu) of HOMO-5..LUMO+5 at nocc 40 / nvir 400 / naux 1800 on the grid of 48 scaled
+    Gauss-Legendre frequencies (nu = 0 and the nodes below 5 Ha), FP64 and mixed; one step is
+    one qx_sigma_ac_dev call (Pi, factorisation, Wc_nm, quadrature) on resident inputs."""
+    import torch
+    from quasix_amd import gw, synthetic
+    e, B = synthetic.make_inputs(NOCC, NVIR, NAUX, seed=1)
+    dev = torch.device("cuda", 0)
+    e_d, B_d = torch.as_tensor(e, device=dev), torch.as_tensor(B, device=dev)
+    tg = gw.frontier_targets(NOCC, NOCC + NVIR)
+    pairs = gw.ov_pairs(B_d, NOCC, NVIR, tg)
+    out = {}
+    for prec in ("bf16x3", "fp64"):
+        for _ in range(args.warmup):
+            sig, _ = gw.sigma_ac(e_d, B_d, NOCC, pairs, tg, nw=NF, precision=prec)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            sig, _ = gw.sigma_ac(e_d, B_d, NOCC, pairs, tg, nw=NF, precision=prec)
+        torch.cuda.synchronize()
+        out[prec] = ((time.perf_counter() - t0) / args.steps, sig)
+    d = float(np.max(np.abs(out["bf16x3"][1] - out["fp64"][1])))
+    line = {
+        "metric": "AC Sigma_c(i nu) evaluations/s (nocc40/nvir400/naux1800, 48 frequencies)",
+        "value": 1.0 / out["bf16x3"][0], "unit": "evaluations/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": out["bf16x3"][0] * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "bf16x3+f64", "data": "synthetic (seeded U/V QR; inputs resident in HBM)",
+        "config": {"workload": "sigma_ac nocc40/nvir400/naux1800, 48 frequencies, "
+                               "HOMO-5..LUMO+5", "nfreq": NF, "targets": len(tg)},
+        "fp64_ms_per_step": out["fp64"][0] * 1e3,
+        "max_abs_dsigma_meV": d / MEV,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -236,9 +273,10 @@ def main():
     ap.add_argument("--precision", default="bf16x3")
     ap.add_argument("--mixed-only", action="store_true",
                     help="skip the FP64 comparison (short profiled runs)")
-    ap.add_argument("--workload", choices=["single", "batch", "evgw"], default="single",
+    ap.add_argument("--workload", choices=["single", "batch", "evgw", "ac"], default="single",
                     help="batch: the 256-instance grouped workload; evgw: 8 evGW iterations at "
-                         "nocc 100 / nvir 1200 / naux 5000 (each its own JSON line)")
+                         "nocc 100 / nvir 1200 / naux 5000; ac: the AC self-energy on the "
+                         "imaginary axis at the headline size (each its own JSON line)")
     ap.add_argument("--batch-size", type=int, default=256)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB device-to-device 3 times (scales HBM counters)")
@@ -264,6 +302,9 @@ def main():
         return
     if args.workload == "evgw":
         run_evgw(args)
+        return
+    if args.workload == "ac":
+        run_ac(args)
         return
     from quasix_amd import gw, synthetic
 

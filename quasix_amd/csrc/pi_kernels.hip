@@ -243,9 +243,7 @@ __global__ __launch_bounds__(256, 2) void pi_syrk_kernel(
 // (4 x 2 blocks of 16x16x4 MFMAs, 2 x 32 accumulators); per k step a wave reads 6 operand
 // values from LDS for 16 MFMAs, and the global operand traffic per flop is half that of the
 // one-frequency kernel.
-constexpr int P2_KC = 16;
-
-template <bool Guarded>
+template <int P2_KC, bool Guarded>
 __device__ __forceinline__ void pi_f64_pair_body(const double* __restrict__ B,
                                                  const double* __restrict__ s, int nov,
                                                  int naux, int npad, int sld, int nf,
@@ -350,6 +348,7 @@ __device__ __forceinline__ void pi_f64_pair_body(const double* __restrict__ B,
     }
 }
 
+template <int P2_KC>
 __global__ __launch_bounds__(512, 1) void pi_f64_pair_kernel(
     const double* __restrict__ B, const double* __restrict__ s, int nov, int naux, int npad,
     int nf, int ntiles, int sld, double* __restrict__ out, PiOut mode) {
@@ -362,11 +361,11 @@ __global__ __launch_bounds__(512, 1) void pi_f64_pair_kernel(
     int I, J;
     tri_index(tile, I, J);
     if (tile_full(I * TILE, J * TILE, naux))
-        pi_f64_pair_body<false>(B, s, nov, naux, npad, sld, nf, out, mode, I * TILE, J * TILE,
-                                f0, As, Bs, Ss);
+        pi_f64_pair_body<P2_KC, false>(B, s, nov, naux, npad, sld, nf, out, mode, I * TILE,
+                                       J * TILE, f0, As, Bs, Ss);
     else
-        pi_f64_pair_body<true>(B, s, nov, naux, npad, sld, nf, out, mode, I * TILE, J * TILE,
-                               f0, As, Bs, Ss);
+        pi_f64_pair_body<P2_KC, true>(B, s, nov, naux, npad, sld, nf, out, mode, I * TILE,
+                                      J * TILE, f0, As, Bs, Ss);
 }
 
 // ---- BF16x3 kernel ----------------------------------------------------------------------
@@ -1487,8 +1486,11 @@ hipError_t launch_pi_syrk(int precision, const double* B, const PiOperands& ops,
     dim3 grid(ntiles * nf), block(256);
     if (precision == 0) {
         if (variant == 1)
-            hipLaunchKernelGGL(pi_f64_pair_kernel, dim3(ntiles * ((nf + 1) / 2)), dim3(512), 0, st,
-                               B, s, nov, naux, npad, nf, ntiles, sld, out, mode);
+            hipLaunchKernelGGL(pi_f64_pair_kernel<16>, dim3(ntiles * ((nf + 1) / 2)), dim3(512), 0,
+                               st, B, s, nov, naux, npad, nf, ntiles, sld, out, mode);
+        else if (variant == 2)
+            hipLaunchKernelGGL(pi_f64_pair_kernel<32>, dim3(ntiles * ((nf + 1) / 2)), dim3(512), 0,
+                               st, B, s, nov, naux, npad, nf, ntiles, sld, out, mode);
         else
             hipLaunchKernelGGL((pi_syrk_kernel<double, 16>), grid, block, 0, st, B, s, nov, naux,
                                npad, nf, ntiles, sld, out, mode);
@@ -1557,7 +1559,7 @@ hipError_t launch_pi_syrk(int precision, const double* B, const PiOperands& ops,
     return hipGetLastError();
 }
 
-int pi_variant_count(int precision) { return precision == 2 ? 11 : precision == 0 ? 2 : 1; }
+int pi_variant_count(int precision) { return precision == 2 ? 11 : precision == 0 ? 3 : 1; }
 
 hipError_t launch_pi_group(int precision, const PiInst* inst_dev, const int* off_dev, int ninst,
                            int nf, int total, int grid, hipStream_t st, int variant) {

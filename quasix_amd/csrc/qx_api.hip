@@ -458,10 +458,11 @@ int tune_nf(int naux_or_npad, int nf) {
 // Autotuner candidates of the BF16x3 Pi kernels: the instances that win or come within a few
 // per cent at the measured shapes (the producer/consumer forms, K64 on 32x32x16 MFMAs, K64 on
 // 16x16x32); the others stay selectable by QXGW_PI_VARIANT.
-// FP64 has the one-frequency (0) and the frequency-pair (1) instance, both timed.
+// FP64 has the one-frequency instance (0) and the frequency-pair instances with K chunks of
+// 16 (1) and 32 (2), all timed.
 std::vector<int> pi_candidates(int precision) {
     if (precision == QX_PREC_BF16X3) return {8, 5, 7, 9, 1};
-    if (precision == QX_PREC_FP64) return {1, 0};
+    if (precision == QX_PREC_FP64) return {2, 1, 0};
     return {};
 }
 

@@ -274,10 +274,11 @@ def test_g0w0_bf16x3_every_variant(gpu, variant, monkeypatch):
     _close(f"E_QP bf16x3 v{variant}", rm["qp_energies"], r64["qp_energies"], MEV)
 
 
-@pytest.mark.parametrize("variant", ["0", "1"])
+@pytest.mark.parametrize("variant", ["0", "1", "2"])
 @pytest.mark.parametrize("naux,nf", [(150, 6), (300, 7), (64, 1)])
 def test_pi_fp64_both_instances(gpu, variant, naux, nf, monkeypatch):
-    """FP64 Pi, one-frequency (0) and frequency-pair (1) instance, against the oracle: ragged
+    """FP64 Pi, one-frequency (0) and frequency-pair (1: K16, 2: K32) instances, against the
+    oracle: ragged
     naux, an odd frequency count (the last pair holds one frequency) and a single frequency,
     in both output forms (full Pi, and eps = I - Pi inside a G0W0 solve)."""
     from quasix_amd import core, gw
