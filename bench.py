@@ -32,6 +32,8 @@ PI_KERNELS = {0: "pi_syrk_bf16x3_kernel<32>", 1: "pi_syrk_bf16x3_kernel<64>",
               6: "pi_syrk_bf16x3_m32_kernel<32>", 7: "pi_syrk_bf16x3_ws_kernel<64, 4>",
               8: "pi_syrk_bf16x3_ws2_kernel", 9: "pi_syrk_bf16x3_ws_kernel<32, 4>",
               10: "pi_syrk_bf16x3_ws_kernel<64, 8>"}
+PI_KERNELS_F64 = {0: "pi_syrk_kernel<double, 16>", 1: "pi_f64_pair_kernel<16>",
+                  2: "pi_f64_pair_kernel<32>"}
 TRAFFIC = os.path.join(ROOT, "profiles", "pi_hbm_traffic.json")
 
 
@@ -352,7 +354,16 @@ def main():
 
     kernel = (PI_KERNELS[r_mix["pi_variant"]] if args.precision == "bf16x3"
               else "pi_syrk_kernel<float, 32>" if args.precision == "fp32"
-              else "pi_syrk_kernel<double, 16>")
+              else PI_KERNELS_F64[r_mix["pi_variant"]])
+    fp64_pi = None
+    if prof64.get("pi", (0, 0))[1]:
+        ms_pi64, n_pi64 = prof64["pi"]
+        t_pi64 = ms_pi64 / n_pi64 * 1e-3
+        kernel64 = PI_KERNELS_F64[r_64["pi_variant"]]
+        fp64_pi = {"kernel": kernel64, "bound": "mfma", "launch_ms": t_pi64 * 1e3,
+                   "achieved": f_pi / t_pi64 / 1e12, "peak": PEAK["fp64"] / 1e12,
+                   "unit": "TFLOP/s", "frac": f_pi / t_pi64 / PEAK["fp64"],
+                   "traffic": measured_traffic(kernel64), **measured_mfma(kernel64)}
     line = {
         "metric": METRIC,
         "value": 1.0 / dt_mix,
@@ -386,6 +397,7 @@ def main():
                                        "kernel recorded in profiles/pi_hbm_traffic.json "
                                        "(not measured in this run)",
                      **measured_mfma(kernel)},
+        "fp64_pi_roofline": fp64_pi,
     }
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(e, B, NOCC, targets)
