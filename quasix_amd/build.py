@@ -15,7 +15,8 @@ CSRC = PKG / "csrc"
 INCLUDE = PKG.parent / "include"
 LIB = PKG / "libqxgw.so"
 SOURCES = ["pi_kernels.hip", "chol_kernels.hip", "sigma_kernels.hip", "adjoint_kernels.hip",
-           "ac_pade.hip", "df_kernels.hip", "stage_kernels.hip", "qx_api.hip"]
+           "ac_pade.hip", "df_kernels.hip", "stage_kernels.hip", "residue_kernels.hip",
+           "qx_api.hip"]
 ARCH = os.environ.get("QXGW_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 

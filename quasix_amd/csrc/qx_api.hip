@@ -624,24 +624,41 @@ struct HostCD {
         const int nj = (int)om.size();
         if (nj == 0) return;
         residue_jobs += nj;
-        const size_t per = (size_t)naux * (naux + 1) * 2;
-        int chunk = (int)std::max<size_t>(1, std::min<size_t>(64, (size_t)(2ull << 30) / (per * 8)));
+        // all jobs of this evaluation in as few batches as the workspace allows: the complex
+        // eps(z) of a job takes two [npad][npad] planes
+        const int nov = nocc * nvir, kpad = round_up(nov, KPAD), npad = round_up(naux, NB);
+        const size_t per = (size_t)2 * npad * npad * sizeof(double);
+        const int chunk = (int)std::max<size_t>(1, std::min<size_t>(64, (size_t)(1ull << 30) / per));
         for (int j0 = 0; j0 < nj; j0 += chunk) {
             const int cnt = std::min(chunk, nj - j0);
             double* om_dev = ws<double>(h, "res_om", chunk);
             double* rows = ws<double>(h, "res_rows", (size_t)chunk * naux);
-            double* work = ws<double>(h, "res_work", (size_t)chunk * per);
+            double* s = ws<double>(h, "res_weights", (size_t)2 * chunk * kpad);
+            double* P = ws<double>(h, "res_eps", (size_t)2 * chunk * npad * npad);
+            double* rhs = ws<double>(h, "res_rhs", (size_t)2 * chunk * npad);
+            int* fail = ws<int>(h, "res_fail", chunk);
             double* wout = ws<double>(h, "res_out", (size_t)chunk * 2);
             h2d(h, om_dev, om.data() + j0, cnt);
             for (int k = 0; k < cnt; ++k)
                 HIPCK(h, hipMemcpyAsync(rows + (size_t)k * naux, B_pair + (size_t)row[j0 + k] * naux,
                                         naux * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
             Phase ph(h, QX_PHASE_RESIDUE);
-            HIPCK(h, launch_residue_jobs(B_ov, eps_g_dev, nocc, nvir, naux, cnt, om_dev, eta,
-                                         rows, work, wout, h->stream));
+            fill_i32(h, fail, 0, cnt);
+            HIPCK(h, launch_residue_weights(eps_g_dev, nocc, nvir, cnt, om_dev, eta, kpad, s,
+                                            h->stream));
+            // Re Pi and Im Pi of a job are the two frequencies of one frequency-pair workgroup
+            HIPCK(h, launch_pi_syrk(QX_PREC_FP64, B_ov, PiOperands{}, s, nov, naux, npad, 2 * cnt,
+                                    P, PiOut::PiFull, h->stream, 2));
+            HIPCK(h, launch_residue_lu_solve(P, naux, npad, cnt, rows, rhs, fail, wout,
+                                             h->stream));
             std::vector<double> w(2 * cnt);
+            std::vector<int> f(cnt);
             d2h(h, w.data(), wout, 2 * cnt);
+            d2h(h, f.data(), fail, cnt);
             for (int k = 0; k < cnt; ++k) {
+                REQUIRE(h, f[k] == 0, QX_ERR_NUMERICAL,
+                        "eps(w + i eta) is singular at w = %.6f (residue of partner %d)",
+                        om[j0 + k], pair_m[row[j0 + k]]);
                 out[qidx[j0 + k]].re += sgn[j0 + k] * w[2 * k];
                 out[qidx[j0 + k]].im += sgn[j0 + k] * w[2 * k + 1];
             }

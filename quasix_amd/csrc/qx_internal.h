@@ -234,15 +234,21 @@ hipError_t launch_sigma_ac(const SigmaProblem& p, const double* nu, int nws, dou
 hipError_t launch_sigma_imag_eval(const SigmaProblem& p, const double* energies, int nq,
                                   double* out, hipStream_t st);
 
-// Residues: for job q, W = b^T[(I - Pi(z_q))^-1 - I] b, z_q = omega_q + i eta.
-// Builds complex Pi(z_q) (FP64) and solves with a one-workgroup complex LU with partial
-// pivoting.
-hipError_t launch_residue_jobs(const double* B, const double* eps, int nocc, int nvir,
-                               int naux, int njobs, const double* omegas, double eta,
-                               const double* brows,      // [njobs][naux]
-                               double* work,             // [njobs][naux][2*naux+2] complex
-                               double* w_out,            // [njobs][2] re, im
-                               hipStream_t st);
+// ---- Residues (residue_kernels.hip) ------------------------------------------------------
+// For job j, w_j = b_j^T [(I - Pi(z_j))^-1 - I] b_j with z_j = omegas[j] + i eta.
+// launch_residue_weights: s[2j][r] = Re c_r(z_j), s[2j + 1][r] = Im c_r(z_j),
+// c_r(z) = 2 [1/(z + e_r) - 1/(z - e_r)], rows r < nocc * nvir, zero up to ld; the two
+// frequencies (2j, 2j + 1) of the FP64 polarizability with these weights (PiOut::PiFull,
+// leading dimension npad) are Re Pi and Im Pi of job j.
+hipError_t launch_residue_weights(const double* eps, int nocc, int nvir, int njobs,
+                                  const double* omegas, double eta, int ld, double* s,
+                                  hipStream_t st);
+// P: [2 njobs][npad][npad] holding Re Pi, Im Pi of each job inside n (npad % 64 == 0);
+// overwritten by eps = I - Pi and its blocked LU with partial pivoting.  brows [njobs][n],
+// rhs: [njobs][2][npad] workspace, fail [njobs] (zeroed by the caller; 1 on a zero
+// pivot), w_out [njobs][2] (re, im).
+hipError_t launch_residue_lu_solve(double* P, int n, int npad, int njobs, const double* brows,
+                                   double* rhs, int* fail, double* w_out, hipStream_t st);
 
 // ---- DF construction (df_kernels.hip) -----------------------------------------------------
 // out[i * nvir + a][P] = sum_{mu,nu} c_occ[mu][i] j3c[mu][nu][P] c_vir[nu][a];

@@ -741,163 +741,6 @@ __global__ __launch_bounds__(64) void newton_group_kernel(const NewtonInst* __re
     }
 }
 
-// ---- Residue jobs ---------------------------------------------------------------------------
-// eps(z) = I - Pi(z),  Pi(z) = 2 sum_r B_r B_r^T [1/(z + e_r) - 1/(z - e_r)],  z = w + i eta.
-// work per job: [naux][naux+1] complex (augmented with b).
-__global__ __launch_bounds__(256) void residue_build_kernel(
-    const double* __restrict__ B, const double* __restrict__ eps, int nocc, int nvir, int naux,
-    const double* __restrict__ omegas, double eta, const double* __restrict__ brows,
-    double* __restrict__ work) {
-    __shared__ double Bp[16][17];
-    __shared__ double Bq[16][17];
-    __shared__ double dre[16], dim[16];
-    const int job = blockIdx.z;
-    const int p0 = blockIdx.y * 16, q0 = blockIdx.x * 16;
-    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
-    const int nov = nocc * nvir;
-    const double zr = omegas[job], zi = eta;
-    double are = 0.0, aim = 0.0;
-    for (int r0 = 0; r0 < nov; r0 += 16) {
-        const int r = r0 + ty;
-        Bp[ty][tx] = (r < nov && p0 + tx < naux) ? B[(size_t)r * naux + p0 + tx] : 0.0;
-        Bq[ty][tx] = (r < nov && q0 + tx < naux) ? B[(size_t)r * naux + q0 + tx] : 0.0;
-        if (threadIdx.x < 16) {
-            const int rr = r0 + threadIdx.x;
-            double d_re = 0.0, d_im = 0.0;
-            if (rr < nov) {
-                const double e = eps[rr / nvir] - eps[nocc + rr % nvir];
-                // 1/(z+e) - 1/(z-e)
-                const double ar = zr + e, br = zr - e;
-                const double i1 = 1.0 / (ar * ar + zi * zi), i2 = 1.0 / (br * br + zi * zi);
-                d_re = 2.0 * (ar * i1 - br * i2);
-                d_im = 2.0 * (-zi * i1 + zi * i2);
-            }
-            dre[threadIdx.x] = d_re;
-            dim[threadIdx.x] = d_im;
-        }
-        __syncthreads();
-        for (int k = 0; k < 16; ++k) {
-            const double bb = Bp[k][ty] * Bq[k][tx];
-            are += dre[k] * bb;
-            aim += dim[k] * bb;
-        }
-        __syncthreads();
-    }
-    const int pp = p0 + ty, qq = q0 + tx;
-    if (pp < naux && qq < naux) {
-        double* W = work + (size_t)job * naux * (naux + 1) * 2;
-        const size_t o = ((size_t)pp * (naux + 1) + qq) * 2;
-        W[o] = (pp == qq ? 1.0 : 0.0) - are;
-        W[o + 1] = -aim;
-        if (qq == 0) {
-            const size_t ob = ((size_t)pp * (naux + 1) + naux) * 2;
-            W[ob] = brows[(size_t)job * naux + pp];
-            W[ob + 1] = 0.0;
-        }
-    }
-}
-
-// Gaussian elimination with partial pivoting on [eps | b], then back substitution;
-// w_out[job] = b^T x - b^T b (complex).
-__global__ __launch_bounds__(1024) void residue_solve_kernel(int naux,
-                                                             const double* __restrict__ brows,
-                                                             double* __restrict__ work,
-                                                             double* __restrict__ w_out) {
-    const int job = blockIdx.x, tid = threadIdx.x, nt = blockDim.x;
-    const int n = naux, ld = naux + 1;
-    double* W = work + (size_t)job * n * ld * 2;
-    __shared__ double rv[1024];
-    __shared__ int ri[1024];
-    __shared__ double red_re[1024], red_im[1024];
-    for (int k = 0; k < n; ++k) {
-        double best = -1.0;
-        int bi = k;
-        for (int r = k + tid; r < n; r += nt) {
-            const size_t o = ((size_t)r * ld + k) * 2;
-            const double v = W[o] * W[o] + W[o + 1] * W[o + 1];
-            if (v > best) { best = v; bi = r; }
-        }
-        rv[tid] = best;
-        ri[tid] = bi;
-        __syncthreads();
-        for (int s = nt / 2; s > 0; s >>= 1) {
-            if (tid < s && (rv[tid + s] > rv[tid] || (rv[tid + s] == rv[tid] && ri[tid + s] < ri[tid]))) {
-                rv[tid] = rv[tid + s];
-                ri[tid] = ri[tid + s];
-            }
-            __syncthreads();
-        }
-        const int piv = ri[0];
-        __syncthreads();
-        if (piv != k)
-            for (int c = k + tid; c < ld; c += nt) {
-                const size_t a = ((size_t)k * ld + c) * 2, b = ((size_t)piv * ld + c) * 2;
-                const double t0 = W[a], t1 = W[a + 1];
-                W[a] = W[b]; W[a + 1] = W[b + 1];
-                W[b] = t0; W[b + 1] = t1;
-            }
-        __syncthreads();
-        const size_t okk = ((size_t)k * ld + k) * 2;
-        const double pr = W[okk], pi = W[okk + 1];
-        const double pd = pr * pr + pi * pi;
-        const double ir = pd > 0.0 ? pr / pd : 0.0, ii = pd > 0.0 ? -pi / pd : 0.0;
-        // eliminate rows below: W[r][c] -= (W[r][k]/W[k][k]) W[k][c]
-        const int rows = n - k - 1, cols = ld - k - 1;
-        for (int idx = tid; idx < rows * cols; idx += nt) {
-            const int r = k + 1 + idx / cols, c = k + 1 + idx % cols;
-            const size_t ork = ((size_t)r * ld + k) * 2, okc = ((size_t)k * ld + c) * 2;
-            const double lr = W[ork] * ir - W[ork + 1] * ii, li = W[ork] * ii + W[ork + 1] * ir;
-            const size_t orc = ((size_t)r * ld + c) * 2;
-            W[orc] -= lr * W[okc] - li * W[okc + 1];
-            W[orc + 1] -= lr * W[okc + 1] + li * W[okc];
-        }
-        __syncthreads();
-    }
-    // back substitution (serial over rows, parallel dot products)
-    for (int k = n - 1; k >= 0; --k) {
-        double sre = 0.0, sim = 0.0;
-        for (int c = k + 1 + tid; c < n; c += nt) {
-            const size_t okc = ((size_t)k * ld + c) * 2, oc = ((size_t)c * ld + n) * 2;
-            sre += W[okc] * W[oc] - W[okc + 1] * W[oc + 1];
-            sim += W[okc] * W[oc + 1] + W[okc + 1] * W[oc];
-        }
-        red_re[tid] = sre;
-        red_im[tid] = sim;
-        __syncthreads();
-        for (int s = nt / 2; s > 0; s >>= 1) {
-            if (tid < s) { red_re[tid] += red_re[tid + s]; red_im[tid] += red_im[tid + s]; }
-            __syncthreads();
-        }
-        if (tid == 0) {
-            const size_t ob = ((size_t)k * ld + n) * 2, okk = ((size_t)k * ld + k) * 2;
-            const double yr = W[ob] - red_re[0], yi = W[ob + 1] - red_im[0];
-            const double pr = W[okk], pi = W[okk + 1], pd = pr * pr + pi * pi;
-            W[ob] = (yr * pr + yi * pi) / pd;
-            W[ob + 1] = (yi * pr - yr * pi) / pd;
-        }
-        __syncthreads();
-    }
-    double sre = 0.0, sim = 0.0, sbb = 0.0;
-    for (int r = tid; r < n; r += nt) {
-        const double b = brows[(size_t)job * n + r];
-        const size_t o = ((size_t)r * ld + n) * 2;
-        sre += b * W[o];
-        sim += b * W[o + 1];
-        sbb += b * b;
-    }
-    red_re[tid] = sre - sbb;
-    red_im[tid] = sim;
-    __syncthreads();
-    for (int s = nt / 2; s > 0; s >>= 1) {
-        if (tid < s) { red_re[tid] += red_re[tid + s]; red_im[tid] += red_im[tid + s]; }
-        __syncthreads();
-    }
-    if (tid == 0) {
-        w_out[(size_t)job * 2] = red_re[0];
-        w_out[(size_t)job * 2 + 1] = red_im[0];
-    }
-}
-
 }  // namespace
 
 hipError_t launch_wnm_split_rows(const double* bpairs, int npair, int npad,
@@ -1001,18 +844,6 @@ hipError_t launch_sigma_imag_eval(const SigmaProblem& p, const double* energies,
                                   double* out, hipStream_t st) {
     if (p.nt <= 0 || nq <= 0) return hipSuccess;
     hipLaunchKernelGGL(sigma_imag_eval_kernel, dim3(p.nt), dim3(64), 0, st, p, energies, nq, out);
-    return hipGetLastError();
-}
-
-hipError_t launch_residue_jobs(const double* B, const double* eps, int nocc, int nvir, int naux,
-                               int njobs, const double* omegas, double eta, const double* brows,
-                               double* work, double* w_out, hipStream_t st) {
-    if (njobs <= 0) return hipSuccess;
-    dim3 g1(ceil_div(naux, 16), ceil_div(naux, 16), njobs);
-    hipLaunchKernelGGL(residue_build_kernel, g1, dim3(256), 0, st, B, eps, nocc, nvir, naux,
-                       omegas, eta, brows, work);
-    hipLaunchKernelGGL(residue_solve_kernel, dim3(njobs), dim3(1024), 0, st, naux, brows, work,
-                       w_out);
     return hipGetLastError();
 }
 
