@@ -32,8 +32,9 @@ PI_KERNELS = {0: "pi_syrk_bf16x3_kernel<32>", 1: "pi_syrk_bf16x3_kernel<64>",
               6: "pi_syrk_bf16x3_m32_kernel<32>", 7: "pi_syrk_bf16x3_ws_kernel<64, 4>",
               8: "pi_syrk_bf16x3_ws2_kernel", 9: "pi_syrk_bf16x3_ws_kernel<32, 4>",
               10: "pi_syrk_bf16x3_ws_kernel<64, 8>"}
-PI_KERNELS_F64 = {0: "pi_syrk_kernel<double, 16>", 1: "pi_f64_pair_kernel<16>",
-                  2: "pi_f64_pair_kernel<32>"}
+PI_KERNELS_F64 = {0: "pi_syrk_kernel<double, 16>", 1: "pi_f64_pair_kernel<16, false>",
+                  2: "pi_f64_pair_kernel<32, false>", 3: "pi_f64_pair_kernel<32, true>",
+                  4: "pi_f64_pair_kernel<16, true>"}
 TRAFFIC = os.path.join(ROOT, "profiles", "pi_hbm_traffic.json")
 
 

@@ -243,7 +243,9 @@ __global__ __launch_bounds__(256, 2) void pi_syrk_kernel(
 // (4 x 2 blocks of 16x16x4 MFMAs, 2 x 32 accumulators); per k step a wave reads 6 operand
 // values from LDS for 16 MFMAs, and the global operand traffic per flop is half that of the
 // one-frequency kernel.
-template <int P2_KC, bool Guarded>
+// DB: two LDS buffers, so the next chunk is stored while other waves still multiply the
+// current one and one barrier per chunk suffices (K32: 134 KB of LDS, one workgroup per CU).
+template <int P2_KC, bool DB, bool Guarded>
 __device__ __forceinline__ void pi_f64_pair_body(const double* __restrict__ B,
                                                  const double* __restrict__ s, int nov,
                                                  int naux, int npad, int sld, int nf,
@@ -253,7 +255,12 @@ __device__ __forceinline__ void pi_f64_pair_body(const double* __restrict__ B,
                                                  double (*Ss)[P2_KC]) {
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-    const int wr = wave >> 2, wc = wave & 3;            // 64-row x 32-column wave tile
+    // 64-row x 32-column wave tile (wr, wc).  On a diagonal tile the live 16x16 blocks per
+    // role are 7, 3, 0, 0 (wr = 0) and 8, 8, 7, 3 (wr = 1); waves w and w + 4 share a SIMD, so
+    // there the roles are permuted to pair heavy with light ones (8+0, 8+0, 7+3, 7+3 instead
+    // of 7+8, 3+8, 0+7, 0+3).
+    const int role = (Guarded && I0 == J0) ? (0x17326054 >> (4 * wave)) & 15 : wave;
+    const int wr = role >> 2, wc = role & 3;
     const bool two = f0 + 1 < nf;                       // wave-uniform
     constexpr int PER = P2_KC * TILE / 512;
     const int c = tid % TILE, kr0 = tid / TILE;         // column, first row of the thread
@@ -289,41 +296,74 @@ __device__ __forceinline__ void pi_f64_pair_body(const double* __restrict__ B,
         const int r = I0 + wr * 64 + bi * 16, cc = J0 + wc * 32 + bj * 16;
         return !Guarded || (r < naux && cc < naux && (I0 != J0 || r >= cc));
     };
-    fetch(0);
-    for (int r0 = 0; r0 < nov; r0 += P2_KC) {
+    auto stage = [&](int buf) {                         // registers -> LDS buffer buf
         #pragma unroll
         for (int q = 0; q < PER; ++q) {
-            As[kr0 + 4 * q][c] = ra[q];
-            Bs[kr0 + 4 * q][c] = rb[q];
+            As[buf * P2_KC + kr0 + 4 * q][c] = ra[q];
+            Bs[buf * P2_KC + kr0 + 4 * q][c] = rb[q];
         }
-        if (tid < 2 * P2_KC) Ss[sf_i][sk] = rs;
-        __syncthreads();
-        if (r0 + P2_KC < nov) fetch(r0 + P2_KC);        // in flight during the MFMA phase
+        if (tid < 2 * P2_KC) Ss[2 * buf + sf_i][sk] = rs;
+    };
+    auto compute = [&](int buf) {
         #pragma unroll
         for (int kk = 0; kk < P2_KC; kk += 4) {
-            const int k = kk + (lane >> 4);
+            const int k = buf * P2_KC + kk + (lane >> 4);
             double av[4], bv[2];
             #pragma unroll
             for (int q = 0; q < 4; ++q) av[q] = As[k][wr * 64 + q * 16 + (lane & 15)];
             #pragma unroll
             for (int q = 0; q < 2; ++q) bv[q] = Bs[k][wc * 32 + q * 16 + (lane & 15)];
-            const double s0 = Ss[0][k], s1 = Ss[1][k];
+            const double s0 = Ss[2 * buf][k - buf * P2_KC], s1 = Ss[2 * buf + 1][k - buf * P2_KC];
             #pragma unroll
             for (int bi = 0; bi < 4; ++bi) {
                 const double a0 = av[bi] * s0, a1 = av[bi] * s1;
+                // no test of `two` here: a branch per MFMA pair splits the cluster into basic
+                // blocks and the next k step's LDS reads can no longer overlap it; past nf the
+                // weights are zero and the second accumulator is not stored
                 #pragma unroll
                 for (int bj = 0; bj < 2; ++bj)
                     if (live(bi, bj)) {
                         acc[0][bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, bv[bj],
                                                                              acc[0][bi][bj], 0, 0, 0);
-                        if (two)
-                            acc[1][bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(
-                                a1, bv[bj], acc[1][bi][bj], 0, 0, 0);
+                        acc[1][bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, bv[bj],
+                                                                             acc[1][bi][bj], 0, 0, 0);
                     }
             }
         }
+    };
+    fetch(0);
+    if constexpr (DB) {
+        stage(0);
         __syncthreads();
+        int cur = 0;
+        for (int r0 = 0; r0 < nov; r0 += P2_KC) {
+            const bool more = r0 + P2_KC < nov;         // uniform over the workgroup
+            if (more) fetch(r0 + P2_KC);                // in flight during the MFMA phase
+            compute(cur);
+            // buffer cur ^ 1 was last read before the previous barrier
+            if (more) stage(cur ^ 1);
+            __syncthreads();
+            cur ^= 1;
+        }
+    } else {
+        for (int r0 = 0; r0 < nov; r0 += P2_KC) {
+            stage(0);
+            __syncthreads();
+            if (r0 + P2_KC < nov) fetch(r0 + P2_KC);    // in flight during the MFMA phase
+            compute(0);
+            __syncthreads();
+        }
     }
+    auto put = [&](int f, int row, int col, double p) {
+        const size_t fo = (size_t)(f0 + f) * npad;
+        if (mode == PiOut::EpsLower) {
+            if (row < npad && col < npad)
+                out[(fo + row) * npad + col] = (row == col ? 1.0 : 0.0) - p;
+        } else if (row < naux && col < naux && (I0 != J0 || row >= col)) {
+            out[(fo + row) * npad + col] = p;
+            out[(fo + col) * npad + row] = p;
+        }
+    };
     #pragma unroll
     for (int f = 0; f < 2; ++f) {
         if (f == 1 && !two) break;
@@ -332,40 +372,31 @@ __device__ __forceinline__ void pi_f64_pair_body(const double* __restrict__ B,
             #pragma unroll
             for (int bj = 0; bj < 2; ++bj)
                 #pragma unroll
-                for (int v = 0; v < 4; ++v) {
-                    const int row = I0 + wr * 64 + bi * 16 + mfma_row_f64(lane, v);
-                    const int col = J0 + wc * 32 + bj * 16 + (lane & 15);
-                    const double p = acc[f][bi][bj][v];
-                    const size_t fo = (size_t)(f0 + f) * npad;
-                    if (mode == PiOut::EpsLower) {
-                        if (row < npad && col < npad)
-                            out[(fo + row) * npad + col] = (row == col ? 1.0 : 0.0) - p;
-                    } else if (row < naux && col < naux && (I0 != J0 || row >= col)) {
-                        out[(fo + row) * npad + col] = p;
-                        out[(fo + col) * npad + row] = p;
-                    }
-                }
+                for (int v = 0; v < 4; ++v)
+                    put(f, I0 + wr * 64 + bi * 16 + mfma_row_f64(lane, v),
+                        J0 + wc * 32 + bj * 16 + (lane & 15), acc[f][bi][bj][v]);
     }
 }
 
-template <int P2_KC>
+template <int P2_KC, bool DB>
 __global__ __launch_bounds__(512, 1) void pi_f64_pair_kernel(
     const double* __restrict__ B, const double* __restrict__ s, int nov, int naux, int npad,
     int nf, int ntiles, int sld, double* __restrict__ out, PiOut mode) {
-    __shared__ double As[P2_KC][TILE + 2];
-    __shared__ double Bs[P2_KC][TILE + 2];
-    __shared__ double Ss[2][P2_KC];
+    constexpr int NBUF = DB ? 2 : 1;
+    __shared__ double As[NBUF * P2_KC][TILE + 2];
+    __shared__ double Bs[NBUF * P2_KC][TILE + 2];
+    __shared__ double Ss[2 * NBUF][P2_KC];
     const int nfp = (nf + 1) / 2;
     const int work = xcd_remap(blockIdx.x, ntiles * nfp);
     const int tile = work / nfp, f0 = 2 * (work % nfp);
     int I, J;
     tri_index(tile, I, J);
     if (tile_full(I * TILE, J * TILE, naux))
-        pi_f64_pair_body<P2_KC, false>(B, s, nov, naux, npad, sld, nf, out, mode, I * TILE,
-                                       J * TILE, f0, As, Bs, Ss);
+        pi_f64_pair_body<P2_KC, DB, false>(B, s, nov, naux, npad, sld, nf, out, mode, I * TILE,
+                                           J * TILE, f0, As, Bs, Ss);
     else
-        pi_f64_pair_body<P2_KC, true>(B, s, nov, naux, npad, sld, nf, out, mode, I * TILE,
-                                      J * TILE, f0, As, Bs, Ss);
+        pi_f64_pair_body<P2_KC, DB, true>(B, s, nov, naux, npad, sld, nf, out, mode, I * TILE,
+                                          J * TILE, f0, As, Bs, Ss);
 }
 
 // ---- BF16x3 kernel ----------------------------------------------------------------------
@@ -1485,12 +1516,19 @@ hipError_t launch_pi_syrk(int precision, const double* B, const PiOperands& ops,
     const int sld = round_up(nov, KPAD);
     dim3 grid(ntiles * nf), block(256);
     if (precision == 0) {
+        const dim3 gp(ntiles * ((nf + 1) / 2));
         if (variant == 1)
-            hipLaunchKernelGGL(pi_f64_pair_kernel<16>, dim3(ntiles * ((nf + 1) / 2)), dim3(512), 0,
-                               st, B, s, nov, naux, npad, nf, ntiles, sld, out, mode);
+            hipLaunchKernelGGL((pi_f64_pair_kernel<16, false>), gp, dim3(512), 0, st, B, s, nov,
+                               naux, npad, nf, ntiles, sld, out, mode);
         else if (variant == 2)
-            hipLaunchKernelGGL(pi_f64_pair_kernel<32>, dim3(ntiles * ((nf + 1) / 2)), dim3(512), 0,
-                               st, B, s, nov, naux, npad, nf, ntiles, sld, out, mode);
+            hipLaunchKernelGGL((pi_f64_pair_kernel<32, false>), gp, dim3(512), 0, st, B, s, nov,
+                               naux, npad, nf, ntiles, sld, out, mode);
+        else if (variant == 3)
+            hipLaunchKernelGGL((pi_f64_pair_kernel<32, true>), gp, dim3(512), 0, st, B, s, nov,
+                               naux, npad, nf, ntiles, sld, out, mode);
+        else if (variant == 4)
+            hipLaunchKernelGGL((pi_f64_pair_kernel<16, true>), gp, dim3(512), 0, st, B, s, nov,
+                               naux, npad, nf, ntiles, sld, out, mode);
         else
             hipLaunchKernelGGL((pi_syrk_kernel<double, 16>), grid, block, 0, st, B, s, nov, naux,
                                npad, nf, ntiles, sld, out, mode);
@@ -1559,7 +1597,7 @@ hipError_t launch_pi_syrk(int precision, const double* B, const PiOperands& ops,
     return hipGetLastError();
 }
 
-int pi_variant_count(int precision) { return precision == 2 ? 11 : precision == 0 ? 3 : 1; }
+int pi_variant_count(int precision) { return precision == 2 ? 11 : precision == 0 ? 5 : 1; }
 
 hipError_t launch_pi_group(int precision, const PiInst* inst_dev, const int* off_dev, int ninst,
                            int nf, int total, int grid, hipStream_t st, int variant) {

@@ -459,10 +459,10 @@ int tune_nf(int naux_or_npad, int nf) {
 // per cent at the measured shapes (the producer/consumer forms, K64 on 32x32x16 MFMAs, K64 on
 // 16x16x32); the others stay selectable by QXGW_PI_VARIANT.
 // FP64 has the one-frequency instance (0) and the frequency-pair instances with K chunks of
-// 16 (1) and 32 (2), all timed.
+// 16 (1) and 32 (2), and with double-buffered LDS, K32 (3) and K16 (4), all timed.
 std::vector<int> pi_candidates(int precision) {
     if (precision == QX_PREC_BF16X3) return {8, 5, 7, 9, 1};
-    if (precision == QX_PREC_FP64) return {2, 1, 0};
+    if (precision == QX_PREC_FP64) return {2, 3, 1, 4, 0};
     return {};
 }
 

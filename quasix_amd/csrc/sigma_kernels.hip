@@ -358,23 +358,55 @@ __global__ __launch_bounds__(256, 2) void wnm_bf16x3_kernel(
     }
 }
 
+// Work items of the two-frequency kernel: (pair tile, frequency pair, auxiliary row tile).
+// A tile's K range (0 .. P0 + 128) grows with P0, so the row tiles are taken from the last
+// one down: the longest workgroups are dispatched first and the short ones fill the tail.
+//   blocked = 0: grid (pair tiles, frequency pairs, row tiles) in that order.
+//   blocked = 1: the items of one row tile are cut into blocks of bp pair tiles x bf
+//   frequency pairs (bp * bf <= the 64 workgroups one XCD holds at a time); block b runs on
+//   XCD b % 8 (workgroups are dispatched round-robin over the 8 XCDs), as the slots
+//   (b / 8) * bsz .. + bsz of that XCD.  A block's workgroups start together and share K
+//   ranges, so each chunk of its bp pair-row tiles and bf M blocks is fetched into that
+//   XCD's L2 once and read there by the others, instead of every workgroup streaming both
+//   operands from beyond L2.  Slots past the items of a partial block return at once.
+struct WnmMap {
+    int blocked, npt, nfp, nP, bp, bf, nbp, nbf;
+};
+
+__device__ __forceinline__ bool wnm_item(const WnmMap& m, int& pt, int& fp, int& pz) {
+    if (!m.blocked) {
+        pt = blockIdx.x;
+        fp = blockIdx.y;
+        pz = blockIdx.z;
+        return true;
+    }
+    const int bsz = m.bp * m.bf, xcd = blockIdx.x % 8, slot = blockIdx.x / 8;
+    const int blk = (slot / bsz) * 8 + xcd, it = slot % bsz;
+    const int per = m.nbp * m.nbf;
+    if (blk >= m.nP * per) return false;
+    pz = blk / per;
+    const int b = blk % per;
+    pt = (b / m.nbf) * m.bp + it / m.bf;
+    fp = (b % m.nbf) * m.bf + it % m.bf;
+    return pt < m.npt && fp < m.nfp;
+}
+
 // Two frequencies (f0, f1) per workgroup on 32x32x16 MFMAs: the pair-row operand Bp of a
 // chunk is loaded and staged once for both, which cuts the operand traffic per MFMA by a
 // quarter.  f1 == f0 (odd frequency count) computes one twice and writes it once.
-// Grid (pair tiles, frequency pairs, auxiliary row tiles): the row tile P0 is the slowest
-// index, taken from the last one down, because a tile's K range (0 .. P0 + 128) grows with
-// P0: the longest workgroups are dispatched first and the short ones fill the tail.
 __global__ __launch_bounds__(256, 2) void wnm_bf16x3_f2_kernel(
     const __bf16* __restrict__ bh, const __bf16* __restrict__ bl, const __bf16* __restrict__ mh,
     const __bf16* __restrict__ ml, const double* __restrict__ bp, int npair, int npad, int nf,
-    double* __restrict__ wpart) {
+    WnmMap map, double* __restrict__ wpart) {
     typedef float f16v __attribute__((ext_vector_type(16)));
     __shared__ __attribute__((aligned(16))) __bf16 Ah[WT][WKP];
     __shared__ __attribute__((aligned(16))) __bf16 Al[WT][WKP];
     __shared__ __attribute__((aligned(16))) __bf16 Mh[2][WT][WKP];
     __shared__ __attribute__((aligned(16))) __bf16 Ml[2][WT][WKP];
-    const int j0 = blockIdx.x * WT, P0 = (gridDim.z - 1 - blockIdx.z) * WT;
-    const int f0 = 2 * blockIdx.y, f1 = min(f0 + 1, nf - 1);
+    int pt, fp, pz;
+    if (!wnm_item(map, pt, fp, pz)) return;             // uniform over the workgroup
+    const int j0 = pt * WT, P0 = (map.nP - 1 - pz) * WT;
+    const int f0 = 2 * fp, f1 = min(f0 + 1, nf - 1);
     const int nblk = npad / NB;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wr = wave >> 1, wc = wave & 1, r32 = lane & 31;
@@ -767,9 +799,34 @@ hipError_t launch_wnm(int precision, const double* bpairs, const double* x, int 
         const char* fv = std::getenv("QXGW_WNM_F2");
         const bool f2 = !(fv && std::atoi(fv) == 0);
         if (f2) {
-            dim3 grid(npp / WT, (nf + 1) / 2, ceil_div(npad, WT));
-            hipLaunchKernelGGL(wnm_bf16x3_f2_kernel, grid, dim3(256), 0, st, mixed->bh,
-                               mixed->bl, mixed->mh, mixed->ml, bpairs, npair, npad, nf, wpart);
+            // XCD blocks when there are at least 16 (pair tile, frequency pair) items per row
+            // tile; QXGW_WNM_BLOCKED=0 / 1 forces the plain grid order / the blocks
+            const char* bv = std::getenv("QXGW_WNM_BLOCKED");
+            WnmMap m{};
+            m.npt = npp / WT;
+            m.nfp = (nf + 1) / 2;
+            m.nP = ceil_div(npad, WT);
+            m.blocked = bv ? std::atoi(bv) != 0 : m.nfp * m.npt >= 16;
+
+            if (m.blocked) {
+                // up to 8 frequency pairs per block, then as many pair tiles as fit in 64
+                m.bf = std::min(m.nfp, 8);
+                m.bp = std::min(m.npt, std::max(1, 64 / m.bf));
+                // even out the pair-tile blocks: bp = ceil(npt / ceil(npt / bp))
+                m.bp = ceil_div(m.npt, ceil_div(m.npt, m.bp));
+                m.nbp = ceil_div(m.npt, m.bp);
+                m.nbf = ceil_div(m.nfp, m.bf);
+                const long long nblk = (long long)m.nP * m.nbp * m.nbf;
+                const long long slots = ceil_div((int)nblk, 8) * 8LL * m.bp * m.bf;
+                hipLaunchKernelGGL(wnm_bf16x3_f2_kernel, dim3((unsigned)slots), dim3(256), 0, st,
+                                   mixed->bh, mixed->bl, mixed->mh, mixed->ml, bpairs, npair,
+                                   npad, nf, m, wpart);
+            } else {
+                dim3 grid(m.npt, m.nfp, m.nP);
+                hipLaunchKernelGGL(wnm_bf16x3_f2_kernel, grid, dim3(256), 0, st, mixed->bh,
+                                   mixed->bl, mixed->mh, mixed->ml, bpairs, npair, npad, nf, m,
+                                   wpart);
+            }
         } else {
             dim3 grid(npp / WT, ceil_div(npad, WT), nf);
             hipLaunchKernelGGL(wnm_bf16x3_kernel, grid, dim3(256), 0, st, mixed->bh, mixed->bl,

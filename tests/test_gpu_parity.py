@@ -187,6 +187,27 @@ def test_wnm_bf16x3_both_kernels(gpu, f2, nf, monkeypatch):
     _close(f"wnm bf16x3 f2={f2} nf={nf}", got, ref, 2e-5, np.max(np.abs(ref)))
 
 
+@pytest.mark.parametrize("nf,blocked", [(7, "1"), (20, "1"), (20, "0"), (3, "1")])
+def test_wnm_bf16x3_xcd_blocks(gpu, nf, blocked, monkeypatch):
+    """The two-frequency mixed Wc_nm kernel with its work items cut into per-XCD blocks of
+    pair tiles x frequency pairs (QXGW_WNM_BLOCKED=1) and in the plain grid order (0):
+    5 pair tiles, 3 auxiliary row tiles (naux 300), and 4, 10 or 2 frequency pairs, so
+    blocks are partial in the frequency direction, the block count is not a multiple of 8,
+    and empty slots return at once."""
+    from quasix_amd import core
+    monkeypatch.setenv("QXGW_WNM_BLOCKED", blocked)
+    nocc, nvir, naux = 7, 100, 300
+    e, B = O.make_inputs(nocc, nvir, naux, seed=19)
+    targets = O.frontier_targets(nocc, nocc + nvir, 4, 4)
+    mn = O.assemble_mnP(B, *O.ov_only_blocks(B, nocc, nvir), nocc, nvir, naux)
+    pt, pm, Bp = O.pair_list(mn, targets)
+    assert 512 < Bp.shape[0] <= 640
+    fr, _ = O.get_scaled_legendre_roots(nf, 0.5)
+    ref = O.wmn_imag(e, B, nocc, Bp, fr)
+    got = core.compute_wmn_imag(e, B, nocc, Bp, fr, precision="bf16x3")
+    _close(f"wnm bf16x3 blocked={blocked} nf={nf}", got, ref, 2e-5, np.max(np.abs(ref)))
+
+
 @pytest.mark.parametrize("prec,tol", [("fp32", 2e-5), ("bf16x3", 2e-5)])
 def test_wnm_mixed_precision(gpu, prec, tol):
     """Wc_nm through the mixed kernels against the FP64 oracle (ragged naux = 150)."""
@@ -274,16 +295,17 @@ def test_g0w0_bf16x3_every_variant(gpu, variant, monkeypatch):
     _close(f"E_QP bf16x3 v{variant}", rm["qp_energies"], r64["qp_energies"], MEV)
 
 
-@pytest.mark.parametrize("variant", ["0", "1", "2"])
-@pytest.mark.parametrize("naux,nf", [(150, 6), (300, 7), (64, 1)])
-def test_pi_fp64_both_instances(gpu, variant, naux, nf, monkeypatch):
-    """FP64 Pi, one-frequency (0) and frequency-pair (1: K16, 2: K32) instances, against the
-    oracle: ragged
-    naux, an odd frequency count (the last pair holds one frequency) and a single frequency,
-    in both output forms (full Pi, and eps = I - Pi inside a G0W0 solve)."""
+@pytest.mark.parametrize("variant", ["0", "1", "2", "3", "4"])
+@pytest.mark.parametrize("naux,nf,nocc,nvir", [(150, 6, 7, 45), (300, 7, 7, 45), (64, 1, 7, 45),
+                                               (200, 5, 3, 9)])
+def test_pi_fp64_both_instances(gpu, variant, naux, nf, nocc, nvir, monkeypatch):
+    """FP64 Pi, one-frequency (0) and frequency-pair instances (1: K16, 2: K32, 3: K32 and
+    4: K16 with double-buffered LDS), against the oracle:
+    ragged naux, N_ov not a multiple of the chunk (315) and below one chunk (27), an odd
+    frequency count (the last pair holds one frequency) and a single frequency, in both
+    output forms (full Pi, and eps = I - Pi inside a G0W0 solve)."""
     from quasix_amd import core, gw
     monkeypatch.setenv("QXGW_PI_VARIANT", variant)
-    nocc, nvir = 7, 45
     e, B = O.make_inputs(nocc, nvir, naux, seed=naux + nf)
     fr, wt = O.get_scaled_legendre_roots(nf, 0.5)
     occ = np.array([2.0] * nocc + [0.0] * nvir)
