@@ -21,6 +21,7 @@
 #include <memory>
 #include <mutex>
 #include <string>
+#include <tuple>
 #include <vector>
 
 using namespace qx;
@@ -56,6 +57,12 @@ struct qx_context {
     std::map<std::string, std::pair<void*, size_t>> arena_bufs;
     char* scratch_dev = nullptr;          // this device's copy of `scratch` (set up in qx_create)
     hipError_t scratch_status = hipErrorNotInitialized;
+    // pinned host ring for the small per-solve transfers (allocated on first use, never in
+    // no-allocation mode): pin_off is the next free byte, reset only after the stream has
+    // been synchronised; pin_hold > 0 while a batch of downloads still holds ring space
+    char* pin = nullptr;
+    size_t pin_cap = 0, pin_off = 0;
+    int pin_hold = 0;
 };
 
 // Device scratch of the no-allocation entry point (qx_g0w0_cd_ov_noalloc_dev): every GPU the
@@ -232,18 +239,85 @@ T* ws(qx_handle h, const std::string& name, size_t count) {
     return static_cast<T*>(p);
 }
 
+// Small transfers go through the pinned ring: a copy from pageable memory costs a host round
+// trip of tens of microseconds each, during which the GPU idles between solves.  An upload is
+// copied into the ring and queued without waiting; a download waits once.  The ring is
+// reused from its start after a stream synchronisation, when every queued upload has been
+// read.  nullptr: no ring space (no-allocation mode, too large, or a pending batch that
+// holds it); the caller then copies directly.
+constexpr size_t kPinBytes = 1 << 20;
+
+char* pin_take(qx_handle h, size_t bytes) {
+    if (h->arena) return nullptr;
+    bytes = (bytes + 63) / 64 * 64;
+    if (!h->pin) {
+        void* p = nullptr;
+        if (hipHostMalloc(&p, kPinBytes) != hipSuccess) return nullptr;
+        h->pin = static_cast<char*>(p);
+        h->pin_cap = kPinBytes;
+        h->pin_off = 0;
+    }
+    if (bytes > h->pin_cap) return nullptr;
+    if (h->pin_off + bytes > h->pin_cap) {
+        if (h->pin_hold > 0) return nullptr;
+        HIPCK(h, hipStreamSynchronize(h->stream));
+        h->pin_off = 0;
+    }
+    char* p = h->pin + h->pin_off;
+    h->pin_off += bytes;
+    return p;
+}
+
 template <typename T>
 void d2h(qx_handle h, T* dst, const T* src, size_t n) {
     if (n == 0) return;
-    HIPCK(h, hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyDeviceToHost, h->stream));
+    const size_t bytes = n * sizeof(T);
+    char* p = pin_take(h, bytes);
+    HIPCK(h, hipMemcpyAsync(p ? static_cast<void*>(p) : static_cast<void*>(dst), src, bytes,
+                            hipMemcpyDeviceToHost, h->stream));
     HIPCK(h, hipStreamSynchronize(h->stream));
+    if (p) std::memcpy(dst, p, bytes);
+    if (h->pin_hold == 0) h->pin_off = 0;
 }
 template <typename T>
 void h2d(qx_handle h, T* dst, const T* src, size_t n) {
     if (n == 0) return;
-    HIPCK(h, hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyHostToDevice, h->stream));
+    const size_t bytes = n * sizeof(T);
+    if (char* p = pin_take(h, bytes)) {
+        std::memcpy(p, src, bytes);
+        HIPCK(h, hipMemcpyAsync(dst, p, bytes, hipMemcpyHostToDevice, h->stream));
+        return;
+    }
+    HIPCK(h, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, h->stream));
     HIPCK(h, hipStreamSynchronize(h->stream));
 }
+
+// Several downloads behind one synchronisation: add() queues each into the ring, run()
+// waits once and copies them out (a download that finds no ring space runs on its own).
+struct D2HBatch {
+    qx_handle h;
+    std::vector<std::tuple<void*, const char*, size_t>> items;
+    explicit D2HBatch(qx_handle h_) : h(h_) { ++h->pin_hold; }
+    ~D2HBatch() { --h->pin_hold; }
+    template <typename T>
+    void add(T* dst, const T* src, size_t n) {
+        if (n == 0) return;
+        const size_t bytes = n * sizeof(T);
+        char* p = pin_take(h, bytes);
+        if (!p) {
+            d2h(h, dst, src, n);
+            return;
+        }
+        HIPCK(h, hipMemcpyAsync(p, src, bytes, hipMemcpyDeviceToHost, h->stream));
+        items.emplace_back(dst, p, bytes);
+    }
+    void run() {
+        if (!items.empty()) HIPCK(h, hipStreamSynchronize(h->stream));
+        for (auto& it : items) std::memcpy(std::get<0>(it), std::get<1>(it), std::get<2>(it));
+        items.clear();
+        if (h->pin_hold == 1) h->pin_off = 0;
+    }
+};
 
 // ---- small utility kernels --------------------------------------------------------------
 __global__ void fill_i32_kernel(int* p, int v, size_t n) {
@@ -749,14 +823,18 @@ void setup_cd(qx_handle h, const qx_g0w0_desc* d, const double* mo_energy,
     cd.eta = d->eta;
     cd.eps_g.resize(nmo);
     cd.eps_ref.resize(nmo);
-    d2h(h, cd.eps_g.data(), mo_energy, nmo);
-    d2h(h, cd.eps_ref.data(), mo_energy_ref ? mo_energy_ref : mo_energy, nmo);
     cd.targets.resize(nt);
-    d2h(h, cd.targets.data(), targets, nt);
     std::vector<int> pt(npair);
     cd.pair_m.resize(npair);
-    d2h(h, pt.data(), pair_target, npair);
-    d2h(h, cd.pair_m.data(), pair_m, npair);
+    {
+        D2HBatch bt(h);
+        bt.add(cd.eps_g.data(), mo_energy, nmo);
+        bt.add(cd.eps_ref.data(), mo_energy_ref ? mo_energy_ref : mo_energy, nmo);
+        bt.add(cd.targets.data(), targets, nt);
+        bt.add(pt.data(), pair_target, npair);
+        bt.add(cd.pair_m.data(), pair_m, npair);
+        bt.run();
+    }
     for (int t = 0; t < nt; ++t)
         REQUIRE(h, cd.targets[t] >= 0 && cd.targets[t] < nmo, QX_ERR_INVALID_INPUT,
                 "target %d out of range [0,%d)", cd.targets[t], nmo);
@@ -833,15 +911,19 @@ qx_status run_g0w0(qx_handle h, const qx_g0w0_desc* d, const double* mo_energy,
                                   st_dev, it_dev, h->stream));
     }
     std::vector<int> st(nt);
-    d2h(h, st.data(), st_dev, nt);
     int residue_targets = 0, newton_fail = 0;
     std::vector<double> guess_h;
-    if (guess) {
-        guess_h.resize(nt);
-        d2h(h, guess_h.data(), guess, nt);
-    }
     std::vector<double> stat_h(nt);
-    d2h(h, stat_h.data(), static_terms, nt);
+    {
+        D2HBatch bt(h);
+        bt.add(st.data(), st_dev, nt);
+        if (guess) {
+            guess_h.resize(nt);
+            bt.add(guess_h.data(), guess, nt);
+        }
+        bt.add(stat_h.data(), static_terms, nt);
+        bt.run();
+    }
     for (int t = 0; t < nt; ++t) {
         if (st[t] == kNewtonConverged) { st[t] = QX_QP_NEWTON; continue; }
         if (st[t] == kNewtonBisection) { st[t] = QX_QP_BISECTION; continue; }
@@ -928,6 +1010,7 @@ void qx_destroy(qx_handle h) {
         (void)hipEventDestroy(h->fan.join[g]);
     }
     if (h->fan.fork) (void)hipEventDestroy(h->fan.fork);
+    if (h->pin) (void)hipHostFree(h->pin);
     if (h->own_stream) (void)hipStreamDestroy(h->stream);
     delete h;
 }
@@ -960,6 +1043,7 @@ const char* qx_last_error(qx_handle h) { return h ? h->err.c_str() : "null handl
 qx_status qx_set_stream(qx_handle h, void* s) {
     return guarded(h, [&] {
         HIPCK(h, hipStreamSynchronize(h->stream));
+        h->pin_off = 0;                   // every queued upload has been read
         if (h->own_stream) HIPCK(h, hipStreamDestroy(h->stream));
         if (s) {
             h->stream = static_cast<hipStream_t>(s);
