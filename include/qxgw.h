@@ -343,9 +343,11 @@ typedef struct {
 /* qx_g0w0_cd_ov_dev for many instances of different sizes in grouped launches: one
  * persistent Pi launch over every instance's tiles and frequencies, one launch per
  * factorisation kernel and step over every instance that still has that block column, one
- * Wc_nm launch pair and one Newton launch.  d supplies nfreq, x0, eta, precision (FP64 or
- * BF16X3; the Wc_nm contraction is FP64) and newton; its sizes are ignored.  Outputs are
- * device arrays of sum(ntarget) entries, instance after instance in the given order.  The
+ * grouped Wc_nm contraction (BF16X3: a split launch and the two-frequency BF16 kernel, as
+ * the single-instance path; FP64: the FP64 kernel) with its reduction, and one Newton
+ * launch.  d supplies nfreq, x0, eta, precision (FP64 or BF16X3) and newton; its sizes are
+ * ignored.  Outputs are device arrays of sum(ntarget) entries, instance after instance in
+ * the given order.  The
  * first call with a batch (sizes, pointers, targets, options, outputs) runs the launches
  * directly and tunes them; later calls with the same batch replay them as a captured HIP
  * graph, so they follow the current contents of mo_energy, B_ov and static_terms.

@@ -1873,6 +1873,8 @@ struct BatchInst {
            *bp = nullptr, *wpart = nullptr, *wnm = nullptr;
     float *s32 = nullptr, *f32 = nullptr;
     uint16_t *hi = nullptr, *lo = nullptr;
+    __bf16 *bh = nullptr, *bl = nullptr;  // mixed: BF16 hi/lo of the padded pair rows
+    int npp = 0;                          // pair rows rounded up to the 128-row tile
     int *pt = nullptr, *pm = nullptr, *fail = nullptr, *targets_dev = nullptr,
         *pair_off_dev = nullptr;
 };
@@ -1939,6 +1941,9 @@ void carve_instance(Bump& b, BatchInst& in, int nf, bool mixed) {
     in.pm = b.take<int>(in.npair);
     in.wpart = b.take<double>((size_t)nf * in.nblk * in.npair);
     in.wnm = b.take<double>((size_t)nf * in.npair);
+    in.npp = round_up(std::max(in.npair, 1), 128);
+    in.bh = mixed ? b.take<__bf16>((size_t)in.npp * in.npad) : nullptr;
+    in.bl = mixed ? b.take<__bf16>((size_t)in.npp * in.npad) : nullptr;
 }
 
 size_t batch_budget(qx_handle h) {
@@ -2129,7 +2134,19 @@ BatchPlan* build_batch_plan(qx_handle h, const qx_g0w0_desc* d, int ninst,
             rp[i + 1] = rp[i] + (in.nblk - 1) * nf;
             w.nblk[i] = in.nblk;
             wn[i] = WnmInst{in.bp, in.x, in.wpart, in.wnm, in.npair, in.npad};
-            wno[i + 1] = wno[i] + ceil_div(std::max(in.npair, 1), NB) * in.nblk * nf;
+            if (mixed) {
+                // the BF16 split of M = X - I goes into the eps buffer, free once X is formed
+                // (4 of its 8 bytes per element)
+                const size_t mat = (size_t)in.npad * in.npad;
+                wn[i].bh = in.bh;
+                wn[i].bl = in.bl;
+                wn[i].mh = reinterpret_cast<__bf16*>(in.eps);
+                wn[i].ml = wn[i].mh + (size_t)nf * mat;
+                wn[i].npp = in.npp;
+                wno[i + 1] = wno[i] + in.npp / 128 * ((nf + 1) / 2) * ceil_div(in.npad, 128);
+            } else {
+                wno[i + 1] = wno[i] + ceil_div(std::max(in.npair, 1), NB) * in.nblk * nf;
+            }
             red[i + 1] = red[i] + nf * in.npair;
             SigmaProblem& p = nw[i].p;
             p.nt = in.nt;
@@ -2237,7 +2254,7 @@ void enqueue_batch(qx_handle h, BatchPlan& P, bool direct) {
         });
         phase(QX_PHASE_WNM, [&] {
             HIPCK(h, launch_wnm_group(w.wn, w.wn_off, w.red_off, n, nf, w.wn_total, w.red_total,
-                                      st));
+                                      st, mixed));
         });
         phase(QX_PHASE_NEWTON, [&] {
             HIPCK(h, launch_newton_group(w.nw, w.t_off, n, w.t_total, P.np, P.e_out, P.z_out,

@@ -75,7 +75,9 @@ hipError_t launch_pi_prepare_mixed(const double* B, int nov, int naux, int nrows
 // K chunks of 32 (3) and 64 (4), 128x128 tiles on 32x32x16 MFMAs with K chunks of 64 (5)
 // and 32 (6), and the producer/consumer form with 512 threads: one frequency and K64 chunks
 // (7), two frequencies and K32 chunks (8), one frequency and K32 chunks with two workgroups
-// per CU (9); FP64/FP32 have one instance.
+// per CU (9), eight producer waves with K64 (10).  FP64 has one frequency per 128x128 tile
+// (0) and two frequencies per tile with K chunks of 16 (1) and 32 (2), and the same with
+// double-buffered LDS, K32 (3) and K16 (4); FP32 has one instance.
 // pi_variant_count() tells how many exist per precision.
 hipError_t launch_pi_syrk(int precision, const double* B, const PiOperands& ops,
                           const double* s, int nov, int naux, int npad, int nf,
@@ -165,17 +167,25 @@ hipError_t launch_wnm_split_rows(const double* bpairs, int npair, int npad,
 hipError_t launch_wnm_from_wc(const double* bpairs, const double* wc, int npair, int npad,
                               int nf, double* wpart, double* wnm, hipStream_t st);
 
-// Grouped Wc_nm (FP64, batched instances).  off: prefix of ceil(npair_q/64) * nblk_q * nf
-// (tiles), red_off: prefix of nf * npair_q (reduction items).
+// Grouped Wc_nm (batched instances).  off: prefix of the work items per instance (see
+// launch_wnm_group), red_off: prefix of nf * npair_q (reduction items).
 struct WnmInst {
     const double* bp;   // [npair][npad] pair rows, zero padded
     const double* x;    // [nf][npad][npad] X = L^-1
     double* wpart;      // [nf][nblk][npair]
     double* wnm;        // [nf][npair]
     int npair, npad;
+    // mixed only: hi/lo BF16 copies of the pair rows ([npp][npad], npp = npair rounded up to
+    // 128) and of M = X - I ([nf][npad][npad])
+    __bf16 *bh = nullptr, *bl = nullptr, *mh = nullptr, *ml = nullptr;
+    int npp = 0;
 };
+// mixed: split the pair rows and M per instance, then the two-frequency BF16x3 kernel over
+// total_tiles = sum of npp / 128 * ceil(nf / 2) * ceil(npad / 128) items; otherwise FP64
+// over sum of ceil(npair / 64) * nblk * nf tiles.  Then the per-pair reduction.
 hipError_t launch_wnm_group(const WnmInst* inst_dev, const int* off_dev, const int* red_off_dev,
-                            int ninst, int nf, int total_tiles, int total_red, hipStream_t st);
+                            int ninst, int nf, int total_tiles, int total_red, hipStream_t st,
+                            bool mixed = false);
 
 struct NewtonParams {
     int max_iterations;

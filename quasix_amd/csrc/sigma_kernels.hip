@@ -393,19 +393,25 @@ __device__ __forceinline__ bool wnm_item(const WnmMap& m, int& pt, int& fp, int&
 
 // Two frequencies (f0, f1) per workgroup on 32x32x16 MFMAs: the pair-row operand Bp of a
 // chunk is loaded and staged once for both, which cuts the operand traffic per MFMA by a
-// quarter.  f1 == f0 (odd frequency count) computes one twice and writes it once.
-__global__ __launch_bounds__(256, 2) void wnm_bf16x3_f2_kernel(
-    const __bf16* __restrict__ bh, const __bf16* __restrict__ bl, const __bf16* __restrict__ mh,
-    const __bf16* __restrict__ ml, const double* __restrict__ bp, int npair, int npad, int nf,
-    WnmMap map, double* __restrict__ wpart) {
+// quarter.  f1 == f0 (odd frequency count) computes one twice and writes it once.  One item:
+// pair tile pt, frequency pair fp, row tile P0.
+struct WnmLds {
+    __bf16 Ah[WT][WKP], Al[WT][WKP], Mh[2][WT][WKP], Ml[2][WT][WKP];
+};
+
+__device__ __forceinline__ void wnm_f2_item(const __bf16* __restrict__ bh,
+                                            const __bf16* __restrict__ bl,
+                                            const __bf16* __restrict__ mh,
+                                            const __bf16* __restrict__ ml,
+                                            const double* __restrict__ bp, int npair, int npad,
+                                            int nf, int pt, int fp, int P0,
+                                            double* __restrict__ wpart, WnmLds& L) {
     typedef float f16v __attribute__((ext_vector_type(16)));
-    __shared__ __attribute__((aligned(16))) __bf16 Ah[WT][WKP];
-    __shared__ __attribute__((aligned(16))) __bf16 Al[WT][WKP];
-    __shared__ __attribute__((aligned(16))) __bf16 Mh[2][WT][WKP];
-    __shared__ __attribute__((aligned(16))) __bf16 Ml[2][WT][WKP];
-    int pt, fp, pz;
-    if (!wnm_item(map, pt, fp, pz)) return;             // uniform over the workgroup
-    const int j0 = pt * WT, P0 = (map.nP - 1 - pz) * WT;
+    auto& Ah = L.Ah;
+    auto& Al = L.Al;
+    auto& Mh = L.Mh;
+    auto& Ml = L.Ml;
+    const int j0 = pt * WT;
     const int f0 = 2 * fp, f1 = min(f0 + 1, nf - 1);
     const int nblk = npad / NB;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -519,6 +525,66 @@ __global__ __launch_bounds__(256, 2) void wnm_bf16x3_f2_kernel(
                     if (j < npair) wpart[((size_t)f * nblk + pb) * npair + j] = rs[v];
                 }
         }
+    }
+}
+
+__global__ __launch_bounds__(256, 2) void wnm_bf16x3_f2_kernel(
+    const __bf16* __restrict__ bh, const __bf16* __restrict__ bl, const __bf16* __restrict__ mh,
+    const __bf16* __restrict__ ml, const double* __restrict__ bp, int npair, int npad, int nf,
+    WnmMap map, double* __restrict__ wpart) {
+    __shared__ __attribute__((aligned(16))) WnmLds L;
+    int pt, fp, pz;
+    if (!wnm_item(map, pt, fp, pz)) return;             // uniform over the workgroup
+    wnm_f2_item(bh, bl, mh, ml, bp, npair, npad, nf, pt, fp, (map.nP - 1 - pz) * WT, wpart, L);
+}
+
+// Grouped form for a batch: instance q owns npt * nfp * nP consecutive workgroups (pair tile
+// fastest, row tiles from the last one down), off is their prefix.
+__global__ __launch_bounds__(256, 2) void wnm_bf16x3_f2_group_kernel(
+    const WnmInst* __restrict__ inst, const int* __restrict__ off, int ninst, int nf) {
+    __shared__ __attribute__((aligned(16))) WnmLds L;
+    const int w = blockIdx.x;
+    const int q = find_inst(off, ninst, w);
+    const WnmInst in = inst[q];
+    const int npt = in.npp / WT, nfp = (nf + 1) / 2, nP = (in.npad + WT - 1) / WT;
+    const int local = w - off[q];
+    const int pt = local % npt, fp = (local / npt) % nfp, pz = local / (npt * nfp);
+    wnm_f2_item(in.bh, in.bl, in.mh, in.ml, in.bp, in.npair, in.npad, nf, pt, fp,
+                (nP - 1 - pz) * WT, in.wpart, L);
+}
+
+// Grouped hi/lo split for a batch, blockIdx.y = instance: the pair rows (zero padded to npp
+// rows) and M_f = X_f - I over the lower 64-blocks (zero above them), as split_rows_kernel
+// and split_m_kernel do for one instance.
+__global__ void split_group_kernel(const WnmInst* __restrict__ inst, int nf) {
+    const WnmInst in = inst[blockIdx.y];
+    const size_t rtot = (size_t)in.npp * in.npad / 4;
+    for (size_t i4 = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i4 < rtot;
+         i4 += (size_t)gridDim.x * blockDim.x) {
+        const size_t i = i4 * 4;
+        const int r = i / in.npad;
+        const dv4 v = r < in.npair ? *reinterpret_cast<const dv4*>(in.bp + i)
+                                   : dv4{0.0, 0.0, 0.0, 0.0};
+        bf16x4 h, l;
+        split4(v, h, l);
+        *reinterpret_cast<bf16x4*>(in.bh + i) = h;
+        *reinterpret_cast<bf16x4*>(in.bl + i) = l;
+    }
+    const size_t mat = (size_t)in.npad * in.npad, mtot = (size_t)nf * mat / 4;
+    for (size_t i4 = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i4 < mtot;
+         i4 += (size_t)gridDim.x * blockDim.x) {
+        const size_t i = i4 * 4;
+        const size_t rc = i % mat;
+        const int P = rc / in.npad, Q = rc % in.npad;
+        dv4 v = dv4{0.0, 0.0, 0.0, 0.0};
+        if (Q / NB <= P / NB) {
+            v = *reinterpret_cast<const dv4*>(in.x + i);
+            if (P >= Q && P < Q + 4) v[P - Q] -= 1.0;
+        }
+        bf16x4 h, l;
+        split4(v, h, l);
+        *reinterpret_cast<bf16x4*>(in.mh + i) = h;
+        *reinterpret_cast<bf16x4*>(in.ml + i) = l;
     }
 }
 
@@ -842,11 +908,18 @@ hipError_t launch_wnm(int precision, const double* bpairs, const double* x, int 
 }
 
 hipError_t launch_wnm_group(const WnmInst* inst_dev, const int* off_dev, const int* red_off_dev,
-                            int ninst, int nf, int total_tiles, int total_red, hipStream_t st) {
+                            int ninst, int nf, int total_tiles, int total_red, hipStream_t st,
+                            bool mixed) {
     if (ninst <= 0) return hipSuccess;
-    if (total_tiles > 0)
+    if (mixed) {
+        hipLaunchKernelGGL(split_group_kernel, dim3(256, ninst), dim3(256), 0, st, inst_dev, nf);
+        if (total_tiles > 0)
+            hipLaunchKernelGGL(wnm_bf16x3_f2_group_kernel, dim3(total_tiles), dim3(256), 0, st,
+                               inst_dev, off_dev, ninst, nf);
+    } else if (total_tiles > 0) {
         hipLaunchKernelGGL(wnm_linv_group_kernel, dim3(total_tiles), dim3(256), 0, st, inst_dev,
                            off_dev, ninst, nf);
+    }
     if (total_red > 0)
         hipLaunchKernelGGL(wnm_reduce_group_kernel, dim3(std::min(ceil_div(total_red, 256), 4096)),
                            dim3(256), 0, st, inst_dev, red_off_dev, ninst, total_red);
