@@ -54,6 +54,12 @@ __device__ __forceinline__ void tri_index_cols(int t, int T, int& I, int& J) {
     I = j + (t - start);
 }
 
+// The same for a kernel that knows only the tile count T (T + 1) / 2.
+__device__ __forceinline__ void tri_index_cols_n(int t, int ntiles, int& I, int& J) {
+    const int T = (int)((sqrtf(8.0f * (float)ntiles + 1.0f) - 1.0f) * 0.5f + 0.5f);
+    tri_index_cols(t, T, I, J);
+}
+
 __device__ __forceinline__ void tri_index(int t, int& I, int& J) {
     // t enumerates lower-triangle tiles (I >= J) row by row.
     int i = (int)((sqrtf(8.0f * (float)t + 1.0f) - 1.0f) * 0.5f);
@@ -245,7 +251,8 @@ __global__ __launch_bounds__(256, 2) void pi_syrk_kernel(
 // one-frequency kernel.
 // DB: two LDS buffers, so the next chunk is stored while other waves still multiply the
 // current one and one barrier per chunk suffices (K32: 134 KB of LDS, one workgroup per CU).
-template <int P2_KC, bool DB, bool Guarded>
+// NFR = 1: the first frequency only (tail items): the second one's MFMAs and store are skipped.
+template <int P2_KC, bool DB, bool Guarded, int NFR = 2>
 __device__ __forceinline__ void pi_f64_pair_body(const double* __restrict__ B,
                                                  const double* __restrict__ s, int nov,
                                                  int naux, int npad, int sld, int nf,
@@ -325,8 +332,9 @@ __device__ __forceinline__ void pi_f64_pair_body(const double* __restrict__ B,
                     if (live(bi, bj)) {
                         acc[0][bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, bv[bj],
                                                                              acc[0][bi][bj], 0, 0, 0);
-                        acc[1][bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, bv[bj],
-                                                                             acc[1][bi][bj], 0, 0, 0);
+                        if (NFR == 2)
+                            acc[1][bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(
+                                a1, bv[bj], acc[1][bi][bj], 0, 0, 0);
                     }
             }
         }
@@ -366,7 +374,7 @@ __device__ __forceinline__ void pi_f64_pair_body(const double* __restrict__ B,
     };
     #pragma unroll
     for (int f = 0; f < 2; ++f) {
-        if (f == 1 && !two) break;
+        if (f == 1 && (NFR == 1 || !two)) break;
         #pragma unroll
         for (int bi = 0; bi < 4; ++bi)
             #pragma unroll
@@ -378,25 +386,48 @@ __device__ __forceinline__ void pi_f64_pair_body(const double* __restrict__ B,
     }
 }
 
+// Items are (tile, frequency pair), tile-major, tiles column by column (so that no XCD's
+// contiguous range holds only the cheap edge tiles of the last tile row).  The first nfull
+// items run as pairs; the rest (the part of the last round of workgroups that would leave
+// CUs idle) run as two single-frequency workgroups each, dispatched last.  Grid:
+// nfull + 2 (ntiles * nfp - nfull).
 template <int P2_KC, bool DB>
 __global__ __launch_bounds__(512, 1) void pi_f64_pair_kernel(
     const double* __restrict__ B, const double* __restrict__ s, int nov, int naux, int npad,
-    int nf, int ntiles, int sld, double* __restrict__ out, PiOut mode) {
+    int nf, int ntiles, int nfull, int sld, double* __restrict__ out, PiOut mode) {
     constexpr int NBUF = DB ? 2 : 1;
     __shared__ double As[NBUF * P2_KC][TILE + 2];
     __shared__ double Bs[NBUF * P2_KC][TILE + 2];
     __shared__ double Ss[2 * NBUF][P2_KC];
     const int nfp = (nf + 1) / 2;
-    const int work = xcd_remap(blockIdx.x, ntiles * nfp);
-    const int tile = work / nfp, f0 = 2 * (work % nfp);
+    const int bid = blockIdx.x;
+    const bool single = bid >= nfull;
+    int work, f0;
+    if (!single) {
+        work = xcd_remap(bid, nfull);
+        f0 = 2 * (work % nfp);
+    } else {
+        work = nfull + (bid - nfull) / 2;
+        f0 = 2 * (work % nfp) + (bid - nfull) % 2;
+        if (f0 >= nf) return;                     // odd nf: the pair's second slot is empty
+    }
     int I, J;
-    tri_index(tile, I, J);
-    if (tile_full(I * TILE, J * TILE, naux))
+    tri_index_cols_n(work / nfp, ntiles, I, J);
+    const bool full = tile_full(I * TILE, J * TILE, naux);
+    if (single) {
+        if (full)
+            pi_f64_pair_body<P2_KC, DB, false, 1>(B, s, nov, naux, npad, sld, nf, out, mode,
+                                                  I * TILE, J * TILE, f0, As, Bs, Ss);
+        else
+            pi_f64_pair_body<P2_KC, DB, true, 1>(B, s, nov, naux, npad, sld, nf, out, mode,
+                                                 I * TILE, J * TILE, f0, As, Bs, Ss);
+    } else if (full) {
         pi_f64_pair_body<P2_KC, DB, false>(B, s, nov, naux, npad, sld, nf, out, mode, I * TILE,
                                            J * TILE, f0, As, Bs, Ss);
-    else
+    } else {
         pi_f64_pair_body<P2_KC, DB, true>(B, s, nov, naux, npad, sld, nf, out, mode, I * TILE,
                                           J * TILE, f0, As, Bs, Ss);
+    }
 }
 
 // ---- BF16x3 kernel ----------------------------------------------------------------------
@@ -1516,19 +1547,26 @@ hipError_t launch_pi_syrk(int precision, const double* B, const PiOperands& ops,
     const int sld = round_up(nov, KPAD);
     dim3 grid(ntiles * nf), block(256);
     if (precision == 0) {
-        const dim3 gp(ntiles * ((nf + 1) / 2));
+        // one 512-thread workgroup per CU: the items past the last full round of CUs run as
+        // single-frequency halves when those fit in one round (QXGW_PI_TAIL=0: never)
+        const int nitems = ntiles * ((nf + 1) / 2), cus = device_cu_count();
+        const int ntail = cus > 0 ? nitems % cus : 0;
+        const char* tv = std::getenv("QXGW_PI_TAIL");
+        const bool tail = !(tv && std::atoi(tv) == 0) && ntail > 0 && 2 * ntail <= cus;
+        const int nfull = tail ? nitems - ntail : nitems;
+        const dim3 gp(nfull + 2 * (nitems - nfull));
         if (variant == 1)
             hipLaunchKernelGGL((pi_f64_pair_kernel<16, false>), gp, dim3(512), 0, st, B, s, nov,
-                               naux, npad, nf, ntiles, sld, out, mode);
+                               naux, npad, nf, ntiles, nfull, sld, out, mode);
         else if (variant == 2)
             hipLaunchKernelGGL((pi_f64_pair_kernel<32, false>), gp, dim3(512), 0, st, B, s, nov,
-                               naux, npad, nf, ntiles, sld, out, mode);
+                               naux, npad, nf, ntiles, nfull, sld, out, mode);
         else if (variant == 3)
             hipLaunchKernelGGL((pi_f64_pair_kernel<32, true>), gp, dim3(512), 0, st, B, s, nov,
-                               naux, npad, nf, ntiles, sld, out, mode);
+                               naux, npad, nf, ntiles, nfull, sld, out, mode);
         else if (variant == 4)
             hipLaunchKernelGGL((pi_f64_pair_kernel<16, true>), gp, dim3(512), 0, st, B, s, nov,
-                               naux, npad, nf, ntiles, sld, out, mode);
+                               naux, npad, nf, ntiles, nfull, sld, out, mode);
         else
             hipLaunchKernelGGL((pi_syrk_kernel<double, 16>), grid, block, 0, st, B, s, nov, naux,
                                npad, nf, ntiles, sld, out, mode);
