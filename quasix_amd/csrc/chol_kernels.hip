@@ -279,6 +279,38 @@ __global__ __launch_bounds__(256) void potrf_diag_kernel(double* __restrict__ a,
     diag_body(a, x, linv_diag, npad, j, fail, blockIdx.x, P, Lp, idg);
 }
 
+// Workgroup w -> (tile bx, frequency f) for a launch of nb tiles per frequency and nfg
+// frequencies, with the tiles of frequency f on XCD f % 8.  Workgroups go to the 8 XCDs
+// round-robin (w % 8), so XCD x runs w = x, x + 8, ...  The items are listed by home XCD
+// (frequencies h, h + 8, ... for home h; with diag, the diagonal tiles bx = 0 of a home
+// block first, so they are dispatched first on their XCD), and XCD x takes the items from
+// x (W / 8) + min(x, W % 8) on, W = nb nfg: a bijection for any nb, nfg.  Every tile of a
+// frequency then reads the operand it shares with the others (the rows of block column
+// j, j + 1 for the update, Linv_jj for the panel) through one L2 instead of eight.
+__device__ __forceinline__ void xcd_item(int nb, int nfg, bool diag, int& bx, int& f) {
+    const int w = blockIdx.x + gridDim.x * blockIdx.y;
+    const int W = nb * nfg, x = w & 7;
+    const int item = x * (W >> 3) + min(x, W & 7) + (w >> 3);
+    int hs = 0;
+    for (int h = 0; h < 8; ++h) {
+        const int c = h < nfg ? (nfg - h + 7) >> 3 : 0;
+        if (item < hs + c * nb) {
+            int r = item - hs;
+            if (diag && r < c) {
+                f = h + 8 * r;
+                bx = 0;
+            } else {
+                if (diag) r -= c;
+                const int per = diag ? nb - 1 : nb;
+                f = h + 8 * (r / per);
+                bx = (diag ? 1 : 0) + r % per;
+            }
+            return;
+        }
+        hs += c * nb;
+    }
+}
+
 // After the diagonal factor of step j:
 //   blockIdx.x <  j : finish block (j, bx) of X = L^-1:  X_j,bx = -Linv_jj S_j,bx
 //   blockIdx.x >= j : panel block i = j+1+(bx-j):         L_ij  =  A_ij Linv_jj^T
@@ -311,10 +343,12 @@ __device__ __forceinline__ void rowpanel_body(double* __restrict__ a, double* __
 __global__ __launch_bounds__(256) void potrf_rowpanel_kernel(double* __restrict__ a,
                                                              double* __restrict__ x,
                                                              const double* __restrict__ linv_diag,
-                                                             int npad, int j) {
+                                                             int npad, int j, int xmap) {
     __shared__ double As[NB][P];
     __shared__ double Ls[NB][P];
-    rowpanel_body(a, x, linv_diag, npad, j, blockIdx.x, blockIdx.y, As, Ls);
+    int bx = blockIdx.x, f = blockIdx.y;
+    if (xmap) xcd_item(gridDim.x, gridDim.y, false, bx, f);
+    rowpanel_body(a, x, linv_diag, npad, j, bx, f, As, Ls);
 }
 
 // Left-looking update of step j, one 64x64 output tile per workgroup, K streamed through
@@ -471,10 +505,11 @@ __global__ __launch_bounds__(256, 3) void potrf_colupdate_kernel(double* __restr
                                                               double* __restrict__ x, int npad,
                                                               int j, int kmin,
                                                               double* __restrict__ linv_diag,
-                                                              int* __restrict__ fail) {
+                                                              int* __restrict__ fail, int xmap) {
     __shared__ __attribute__((aligned(16))) double S[2 * NB * GK > DIAG_LDS ? 2 * NB * GK : DIAG_LDS];
     int bx = blockIdx.x, f = blockIdx.y;
-    if (linv_diag) diag_first(npad / NB, bx, f);
+    if (xmap) xcd_item(gridDim.x, gridDim.y, true, bx, f);
+    else if (linv_diag) diag_first(npad / NB, bx, f);
     colupdate_tile(a, x, npad, j, bx, f, kmin, S);
     if (linv_diag && bx == 0) fused_diag(a, x, linv_diag, npad, j, fail, f, S);
 }
@@ -485,10 +520,11 @@ __global__ __launch_bounds__(256, 2) void potrf_colupdate2_kernel(double* __rest
                                                                double* __restrict__ x, int npad,
                                                                int j,
                                                                double* __restrict__ linv_diag,
-                                                               int* __restrict__ fail) {
+                                                               int* __restrict__ fail, int xmap) {
     __shared__ __attribute__((aligned(16))) double S[3 * NB * GK > DIAG_LDS ? 3 * NB * GK : DIAG_LDS];
     int bx = blockIdx.x, f = blockIdx.y;
-    if (linv_diag) diag_first(npad / NB, bx, f);
+    if (xmap) xcd_item(gridDim.x, gridDim.y, true, bx, f);
+    else if (linv_diag) diag_first(npad / NB, bx, f);
     if (bx < npad / NB - j) colupdate_body<true, true>(a, x, npad, j, bx, f, 0, S);
     else colupdate_body<false, true>(a, x, npad, j, bx, f, 0, S);
     if (linv_diag && bx == 0) fused_diag(a, x, linv_diag, npad, j, fail, f, S);
@@ -497,6 +533,13 @@ __global__ __launch_bounds__(256, 2) void potrf_colupdate2_kernel(double* __rest
 // QXGW_POTRF_FUSE=0: separate diagonal launches.
 bool potrf_fuse_diag() {
     const char* v = std::getenv("QXGW_POTRF_FUSE");
+    return !(v && std::atoi(v) == 0);
+}
+
+// QXGW_POTRF_XCD=0: workgroups in launch order (tile-major per frequency) instead of the
+// per-XCD frequency mapping of xcd_item.
+bool potrf_xcd_map() {
+    const char* v = std::getenv("QXGW_POTRF_XCD");
     return !(v && std::atoi(v) == 0);
 }
 
@@ -677,6 +720,7 @@ hipError_t launch_potrf_batched(double* a, double* x, double* linv_diag, int npa
     // then the diagonal and panel kernels of j + 1.  A last odd step runs alone.
     // With fusion the diagonal factor of every step but the first runs inside its update.
     const bool pairs = potrf_pair_steps(), fuse = potrf_fuse_diag();
+    const int xmap = potrf_xcd_map() ? 1 : 0;
     for (int j = 0; j < nblk;) {
         const bool pair = pairs && j + 1 < nblk;
         for (int g = 0; g < ng; ++g) {
@@ -689,17 +733,17 @@ hipError_t launch_potrf_batched(double* a, double* x, double* linv_diag, int npa
                 double* dfuse = fuse && js > 0 ? dg : nullptr;
                 if (pair && s == 0 && js > 0)
                     hipLaunchKernelGGL(potrf_colupdate2_kernel, dim3(nblk, nfg), dim3(256), 0,
-                                       sg[g], ag, xg, npad, js, dfuse, fail + f0);
+                                       sg[g], ag, xg, npad, js, dfuse, fail + f0, xmap);
                 else if (js > 0)
                     hipLaunchKernelGGL(potrf_colupdate_kernel, dim3(nblk, nfg), dim3(256), 0,
                                        sg[g], ag, xg, npad, js, pair ? j * NB : 0, dfuse,
-                                       fail + f0);
+                                       fail + f0, xmap);
                 if (!dfuse)
                     hipLaunchKernelGGL(potrf_diag_kernel, dim3(nfg), dim3(256), 0, sg[g], ag,
                                        xg, dg, npad, js, fail + f0);
                 if (nblk > 1)
                     hipLaunchKernelGGL(potrf_rowpanel_kernel, dim3(nblk - 1, nfg), dim3(256), 0,
-                                       sg[g], ag, xg, dg, npad, js);
+                                       sg[g], ag, xg, dg, npad, js, xmap);
             }
         }
         j += pair ? 2 : 1;

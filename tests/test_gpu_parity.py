@@ -90,6 +90,33 @@ def test_screened_wc_multi_block(gpu, pair, fuse, n, monkeypatch):
         _close(f"wc n={n} pair={pair} fuse={fuse} k={k}", wc[k], ref, 1e-11, max(1.0, np.max(np.abs(ref))))
 
 
+@pytest.mark.parametrize("pair", ["1", "0"])
+def test_screened_wc_xcd_mapping(gpu, pair, monkeypatch):
+    """27 frequencies in three stream groups of 9 (more frequencies than XCDs, not a
+    multiple of 8) and four 64-blocks: the per-XCD workgroup mapping of the update and
+    panel kernels computes every tile with the same operands in the same order as launch
+    order, so Wc is bit-identical with QXGW_POTRF_XCD=0, and matches the oracle."""
+    from quasix_amd import core
+    monkeypatch.setenv("QXGW_POTRF_PAIR", pair)
+    rng = np.random.default_rng(5)
+    n, nf = 230, 27
+    pis = []
+    for k in range(nf):
+        Q = np.linalg.qr(rng.standard_normal((n, n)))[0]
+        pi = (Q * rng.uniform(-3.0 / (1 + k), 0.0, n)) @ Q.T
+        pis.append(0.5 * (pi + pi.T))
+    pis = np.stack(pis)
+    monkeypatch.setenv("QXGW_POTRF_XCD", "1")
+    wc1, nfb1 = core.compute_screening_wc(pis)
+    monkeypatch.setenv("QXGW_POTRF_XCD", "0")
+    wc0, nfb0 = core.compute_screening_wc(pis)
+    assert nfb0 == nfb1 == 0
+    assert np.array_equal(wc0, wc1)
+    for k in (0, 8, 13, 26):
+        ref = O.screened_wc(pis[k])
+        _close(f"wc xcd k={k}", wc1[k], ref, 1e-11, max(1.0, np.max(np.abs(ref))))
+
+
 def test_screened_wc_lu_fallback(gpu):
     """A Pi with an eigenvalue above 1 makes eps indefinite: Cholesky fails, LU is used."""
     from quasix_amd import core
