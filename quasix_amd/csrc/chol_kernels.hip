@@ -279,18 +279,23 @@ __global__ __launch_bounds__(256) void potrf_diag_kernel(double* __restrict__ a,
     diag_body(a, x, linv_diag, npad, j, fail, blockIdx.x, P, Lp, idg);
 }
 
-// Workgroup w -> (tile bx, frequency f) for a launch of nb tiles per frequency and nfg
-// frequencies, with the tiles of frequency f on XCD f % 8.  Workgroups go to the 8 XCDs
-// round-robin (w % 8), so XCD x runs w = x, x + 8, ...  The items are listed by home XCD
-// (frequencies h, h + 8, ... for home h; with diag, the diagonal tiles bx = 0 of a home
-// block first, so they are dispatched first on their XCD), and XCD x takes the items from
-// x (W / 8) + min(x, W % 8) on, W = nb nfg: a bijection for any nb, nfg.  Every tile of a
-// frequency then reads the operand it shares with the others (the rows of block column
-// j, j + 1 for the update, Linv_jj for the panel) through one L2 instead of eight.
-__device__ __forceinline__ void xcd_item(int nb, int nfg, bool diag, int& bx, int& f) {
-    const int w = blockIdx.x + gridDim.x * blockIdx.y;
-    const int W = nb * nfg, x = w & 7;
-    const int item = x * (W >> 3) + min(x, W & 7) + (w >> 3);
+// Work item -> (tile bx, frequency f) for nb tiles per frequency and nfg frequencies, with
+// the tiles of frequency f on XCD f % 8.  Workgroups go to the 8 XCDs round-robin, so the
+// item at launch position p = o + l (l local to this instance, o = the first position of
+// the instance mod 8) runs on XCD p % 8.  The items are listed by home XCD (frequencies
+// h, h + 8, ... for home h; with diag, the diagonal tiles bx = 0 of a home block first,
+// so they are dispatched first on their XCD), and XCD x takes, in order, as many of them
+// as it has positions in [o, o + W), W = nb nfg: a bijection of [0, W) for any nb, nfg, o.
+// Every tile of a frequency then reads the operand it shares with the others (the rows of
+// block columns j, j + 1 for the update, Linv_jj for the panel) through one L2, not eight.
+__device__ __forceinline__ int xcd_count(int m, int y) {   // #{p < m : p % 8 == y}
+    return m > y ? (m - y + 7) >> 3 : 0;
+}
+__device__ __forceinline__ void xcd_item(int l, int o, int nb, int nfg, bool diag, int& bx,
+                                         int& f) {
+    const int W = nb * nfg, p = o + l, x = p & 7;
+    int item = (p >> 3) - xcd_count(o, x);
+    for (int y = 0; y < x; ++y) item += xcd_count(o + W, y) - xcd_count(o, y);
     int hs = 0;
     for (int h = 0; h < 8; ++h) {
         const int c = h < nfg ? (nfg - h + 7) >> 3 : 0;
@@ -347,7 +352,7 @@ __global__ __launch_bounds__(256) void potrf_rowpanel_kernel(double* __restrict_
     __shared__ double As[NB][P];
     __shared__ double Ls[NB][P];
     int bx = blockIdx.x, f = blockIdx.y;
-    if (xmap) xcd_item(gridDim.x, gridDim.y, false, bx, f);
+    if (xmap) xcd_item(bx + gridDim.x * f, 0, gridDim.x, gridDim.y, false, bx, f);
     rowpanel_body(a, x, linv_diag, npad, j, bx, f, As, Ls);
 }
 
@@ -508,7 +513,7 @@ __global__ __launch_bounds__(256, 3) void potrf_colupdate_kernel(double* __restr
                                                               int* __restrict__ fail, int xmap) {
     __shared__ __attribute__((aligned(16))) double S[2 * NB * GK > DIAG_LDS ? 2 * NB * GK : DIAG_LDS];
     int bx = blockIdx.x, f = blockIdx.y;
-    if (xmap) xcd_item(gridDim.x, gridDim.y, true, bx, f);
+    if (xmap) xcd_item(bx + gridDim.x * f, 0, gridDim.x, gridDim.y, true, bx, f);
     else if (linv_diag) diag_first(npad / NB, bx, f);
     colupdate_tile(a, x, npad, j, bx, f, kmin, S);
     if (linv_diag && bx == 0) fused_diag(a, x, linv_diag, npad, j, fail, f, S);
@@ -523,7 +528,7 @@ __global__ __launch_bounds__(256, 2) void potrf_colupdate2_kernel(double* __rest
                                                                int* __restrict__ fail, int xmap) {
     __shared__ __attribute__((aligned(16))) double S[3 * NB * GK > DIAG_LDS ? 3 * NB * GK : DIAG_LDS];
     int bx = blockIdx.x, f = blockIdx.y;
-    if (xmap) xcd_item(gridDim.x, gridDim.y, true, bx, f);
+    if (xmap) xcd_item(bx + gridDim.x * f, 0, gridDim.x, gridDim.y, true, bx, f);
     else if (linv_diag) diag_first(npad / NB, bx, f);
     if (bx < npad / NB - j) colupdate_body<true, true>(a, x, npad, j, bx, f, 0, S);
     else colupdate_body<false, true>(a, x, npad, j, bx, f, 0, S);
@@ -536,8 +541,8 @@ bool potrf_fuse_diag() {
     return !(v && std::atoi(v) == 0);
 }
 
-// QXGW_POTRF_XCD=0: workgroups in launch order (tile-major per frequency) instead of the
-// per-XCD frequency mapping of xcd_item.
+// QXGW_POTRF_XCD=0: workgroups in launch order (tiles of a frequency consecutive) instead
+// of the per-XCD frequency mapping of xcd_item.
 bool potrf_xcd_map() {
     const char* v = std::getenv("QXGW_POTRF_XCD");
     return !(v && std::atoi(v) == 0);
@@ -554,13 +559,16 @@ bool potrf_pair_steps() {
 // are a prefix of the table.  off[q] is the first work item of instance q in a launch
 // whose per-instance grid is (items per frequency) x nf.
 __global__ __launch_bounds__(256, 3) void potrf_colupdate_group_kernel(
-    const PotrfInst* __restrict__ inst, const int* __restrict__ off, int nact, int j) {
+    const PotrfInst* __restrict__ inst, const int* __restrict__ off, int nact, int j, int nf,
+    int xmap) {
     __shared__ __attribute__((aligned(16))) double S[2 * NB * GK];
     const int w = blockIdx.x;
     const int q = find_inst(off, nact, w);
     const PotrfInst in = inst[q];
     const int nblk = in.npad / NB, local = w - off[q];
-    colupdate_tile(in.a, in.x, in.npad, j, local % nblk, local / nblk, 0, S);
+    int bx = local % nblk, f = local / nblk;
+    if (xmap) xcd_item(local, off[q] & 7, nblk, nf, false, bx, f);
+    colupdate_tile(in.a, in.x, in.npad, j, bx, f, 0, S);
 }
 
 __global__ __launch_bounds__(256) void potrf_diag_group_kernel(const PotrfInst* __restrict__ inst,
@@ -573,14 +581,17 @@ __global__ __launch_bounds__(256) void potrf_diag_group_kernel(const PotrfInst* 
 }
 
 __global__ __launch_bounds__(256) void potrf_rowpanel_group_kernel(
-    const PotrfInst* __restrict__ inst, const int* __restrict__ off, int nact, int j) {
+    const PotrfInst* __restrict__ inst, const int* __restrict__ off, int nact, int j, int nf,
+    int xmap) {
     __shared__ double As[NB][P];
     __shared__ double Ls[NB][P];
     const int w = blockIdx.x;
     const int q = find_inst(off, nact, w);
     const PotrfInst in = inst[q];
     const int nb1 = in.npad / NB - 1, local = w - off[q];
-    rowpanel_body(in.a, in.x, in.linv, in.npad, j, local % nb1, local / nb1, As, Ls);
+    int bx = local % nb1, f = local / nb1;
+    if (xmap) xcd_item(local, off[q] & 7, nb1, nf, false, bx, f);
+    rowpanel_body(in.a, in.x, in.linv, in.npad, j, bx, f, As, Ls);
 }
 
 // Wc = X^T X - I over the first naux rows/cols, full symmetric output [nf][naux][naux].
@@ -760,18 +771,18 @@ hipError_t launch_potrf_group(const PotrfInst* inst_dev, const int* nblk_sorted,
                               int nf, const int* cu_off_dev, const int* rp_off_dev,
                               const int* cu_off_host, const int* rp_off_host, hipStream_t st) {
     if (ninst <= 0) return hipSuccess;
-    const int maxblk = nblk_sorted[0];
+    const int maxblk = nblk_sorted[0], xmap = potrf_xcd_map() ? 1 : 0;
     int nact = ninst;
     for (int j = 0; j < maxblk; ++j) {
         while (nact > 0 && nblk_sorted[nact - 1] <= j) --nact;
         if (j > 0 && cu_off_host[nact] > 0)
             hipLaunchKernelGGL(potrf_colupdate_group_kernel, dim3(cu_off_host[nact]), dim3(256), 0,
-                               st, inst_dev, cu_off_dev, nact, j);
+                               st, inst_dev, cu_off_dev, nact, j, nf, xmap);
         hipLaunchKernelGGL(potrf_diag_group_kernel, dim3(nact * nf), dim3(256), 0, st, inst_dev,
                            nf, j);
         if (rp_off_host[nact] > 0)
             hipLaunchKernelGGL(potrf_rowpanel_group_kernel, dim3(rp_off_host[nact]), dim3(256), 0,
-                               st, inst_dev, rp_off_dev, nact, j);
+                               st, inst_dev, rp_off_dev, nact, j, nf, xmap);
     }
     return hipGetLastError();
 }

@@ -64,6 +64,23 @@ def test_batch_matches_single_instance_path(gpu, precision):
         assert np.all(b["state"] == 0)
 
 
+def test_batch_factorisation_xcd_mapping(gpu, monkeypatch):
+    """The grouped factorisation kernels place the tiles of each (instance, frequency) on
+    one XCD (instance offsets not a multiple of 8, more frequencies than XCDs); every tile
+    is computed as in launch order, so the FP64 energies are bit-identical with
+    QXGW_POTRF_XCD=0."""
+    import torch
+    from quasix_amd import gw
+    inst = _instances(torch, seed0=300)
+    monkeypatch.setenv("QXGW_POTRF_XCD", "1")
+    r1 = gw.g0w0_batch(inst, nfreq=NF, precision="fp64")
+    monkeypatch.setenv("QXGW_POTRF_XCD", "0")
+    r0 = gw.g0w0_batch(inst, nfreq=NF, precision="fp64")
+    for q, (a, b) in enumerate(zip(r1, r0)):
+        assert np.array_equal(a["qp_energies"], b["qp_energies"]), q
+        assert np.array_equal(a["z_factors"], b["z_factors"]), q
+
+
 def test_batch_bf16x3_within_1mev_and_oracle(gpu):
     import torch
     from quasix_amd import gw
