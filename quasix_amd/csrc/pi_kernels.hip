@@ -1371,6 +1371,34 @@ __global__ __launch_bounds__(256, 2) void pi_group_f64_kernel(
                                            PiOut::EpsLower, I0, J0, f, As, Bs);
 }
 
+// FP64 with two frequencies per tile (even nf): one workgroup per item, an item being a
+// (tile, frequency pair) of one instance (items 2 w, 2 w + 1 of the concatenated list, tiles
+// column by column), run by the two-frequency body of the single-instance kernel (K chunks
+// of 32 rows, one LDS buffer).  Grid total / 2; xcd_remap gives each XCD a contiguous range,
+// so the frequency pairs and neighbouring tiles that stream the same rows share its L2.  Not
+// persistent: a loop would keep the instance descriptor live across the body, which then
+// spills.
+__global__ __launch_bounds__(512, 1) void pi_group_f64_pair_kernel(
+    const PiInst* __restrict__ inst, const int* __restrict__ off, int ninst, int nf, int total) {
+    __shared__ double As[32][TILE + 2];
+    __shared__ double Bs[32][TILE + 2];
+    __shared__ double Ss[2][32];
+    const int w = 2 * xcd_remap(blockIdx.x, total / 2);
+    const int q = find_inst(off, ninst, w);
+    const PiInst in = inst[q];
+    const int local = w - off[q], f = local % nf;
+    int I, J;
+    tri_index_cols(local / nf, (in.npad + TILE - 1) / TILE, I, J);
+    if (tile_full(I * TILE, J * TILE, in.naux))
+        pi_f64_pair_body<32, false, false>(in.B, in.s, in.nov, in.naux, in.npad, in.kpad, nf,
+                                           in.out, PiOut::EpsLower, I * TILE, J * TILE, f, As,
+                                           Bs, Ss);
+    else
+        pi_f64_pair_body<32, false, true>(in.B, in.s, in.nov, in.naux, in.npad, in.kpad, nf,
+                                          in.out, PiOut::EpsLower, I * TILE, J * TILE, f, As, Bs,
+                                          Ss);
+}
+
 // Wide variant: 256 (A, unscaled) x 128 (C = s B, split on the fly) tile per 512-thread
 // workgroup, 8 waves of 64x64.  Converting only the narrow side halves the split work per
 // MFMA.  LDS rows are 64 bf16 (8 chunks of 16 B) with the chunk index XOR-swizzled by
@@ -1641,7 +1669,11 @@ hipError_t launch_pi_group(int precision, const PiInst* inst_dev, const int* off
                            int nf, int total, int grid, hipStream_t st, int variant) {
     if (ninst <= 0 || total <= 0) return hipSuccess;
     grid = std::max(8, grid / 8 * 8);
-    if (precision == 0)
+    if (precision == 0 && variant == 1) {                  // one workgroup per item
+        if (nf % 2) return hipErrorInvalidValue;
+        hipLaunchKernelGGL(pi_group_f64_pair_kernel, dim3(total / 2), dim3(512), 0, st, inst_dev,
+                           off_dev, ninst, nf, total);
+    } else if (precision == 0)
         hipLaunchKernelGGL(pi_group_f64_kernel, dim3(grid), dim3(256), 0, st, inst_dev, off_dev,
                            ninst, nf, total);
     else if (precision == 2 && variant == 4) {
@@ -1665,12 +1697,14 @@ hipError_t launch_pi_group(int precision, const PiInst* inst_dev, const int* off
     return hipGetLastError();
 }
 
-int pi_group_variant_count(int precision) { return precision == 2 ? 5 : 1; }
+int pi_group_variant_count(int precision) { return precision == 2 ? 5 : precision == 0 ? 2 : 1; }
 
 int pi_group_occupancy(int precision, int variant) {
     int n = 0;
     hipError_t e;
-    if (precision == 0)
+    if (precision == 0 && variant == 1)
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, pi_group_f64_pair_kernel, 512, 0);
+    else if (precision == 0)
         e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, pi_group_f64_kernel, 256, 0);
     else if (variant == 4)
         e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, pi_group_bf16x3_ws2_kernel, 512, 0);

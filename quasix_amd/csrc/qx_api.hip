@@ -2234,7 +2234,13 @@ void enqueue_batch(qx_handle h, BatchPlan& P, bool direct) {
             HIPCK(h, launch_pi_group(prec, w.pi, w.pi_off, n, nf, w.pi_total,
                                      P.pi_grid * pi_group_occupancy(prec, var), st, var));
         };
-        if (direct) {
+        if (direct && prec == 0) {
+            // FP64: the frequency-pair instance (1) for even nf, without a timing run (one
+            // FP64 wave of a large batch takes seconds); QXGW_PI_VARIANT=0 forces the other
+            const char* fv = std::getenv("QXGW_PI_VARIANT");
+            w.pi_variant = nf % 2 ? 0 : (fv && std::atoi(fv) == 0 ? 0 : 1);
+            w.pi_grid = P.pi_grid * pi_group_occupancy(prec, w.pi_variant);
+        } else if (direct) {
             // the two-frequency instance (4) needs an even frequency count
             w.pi_variant =
                 nf % 2 ? choose_variant(h, "pi_group", prec, pi_group_variant_count(prec),

@@ -98,7 +98,8 @@ struct PiInst {
 };
 // Persistent grouped Pi over all instances; off_dev: [ninst + 1] prefix sums of
 // ntiles_q * nf with ntiles_q = T (T + 1) / 2, T = ceil(npad_q / 128).  grid is rounded
-// down to a multiple of 8.  precision 0 (FP64) or 2 (BF16x3, variants 0 = K32, 1 = K64,
+// down to a multiple of 8.  precision 0 (FP64, variants 0 = one frequency per tile, 1 = two
+// frequencies per tile, even nf only) or 2 (BF16x3, variants 0 = K32, 1 = K64,
 // 2 = double-buffered K32, 3 = K64 on 32x32x16 MFMAs, 4 = producer/consumer with two
 // frequencies per tile, even nf only).
 hipError_t launch_pi_group(int precision, const PiInst* inst_dev, const int* off_dev, int ninst,

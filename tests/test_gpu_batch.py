@@ -81,6 +81,25 @@ def test_batch_factorisation_xcd_mapping(gpu, monkeypatch):
         assert np.array_equal(a["z_factors"], b["z_factors"]), q
 
 
+def test_batch_fp64_pi_instances(gpu, monkeypatch):
+    """The FP64 batch runs its Pi with two frequencies per tile (grouped instance 1) when nf is
+    even; the one-frequency grouped instance (QXGW_PI_VARIANT=0) gives the same energies to
+    1e-10 Ha (the K chunking differs, so the FP64 sums differ in rounding only)."""
+    import torch
+    from quasix_amd import gw
+    inst = _instances(torch, seed0=320)
+    bs = gw.G0W0BatchSolver(inst, nfreq=NF, precision="fp64")
+    bs.run()
+    assert bs.stats.pi_variant == 1
+    pair = bs.results()
+    monkeypatch.setenv("QXGW_PI_VARIANT", "0")
+    bs0 = gw.G0W0BatchSolver(inst, nfreq=NF, precision="fp64")
+    bs0.run()
+    assert bs0.stats.pi_variant == 0
+    for q, (a, b) in enumerate(zip(pair, bs0.results())):
+        _close(f"fp64 pi pair vs single inst {q}", a["qp_energies"], b["qp_energies"], 1e-10)
+
+
 def test_batch_bf16x3_within_1mev_and_oracle(gpu):
     import torch
     from quasix_amd import gw
