@@ -17,6 +17,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -63,6 +64,14 @@ struct qx_context {
     char* pin = nullptr;
     size_t pin_cap = 0, pin_off = 0;
     int pin_hold = 0;
+    // the CD frequency grid of the last solve (buffers "cd_fr"/"cd_wt", written only by
+    // run_g0w0) and the zeroed static terms ("zero_static"): re-uploaded only when the key
+    // or the buffer changes; never used in no-allocation mode
+    int grid_nf = 0;
+    double grid_x0 = 0.0;
+    const double* grid_fr = nullptr;
+    const double* zero_ptr = nullptr;
+    int zero_n = 0;
 };
 
 // Device scratch of the no-allocation entry point (qx_g0w0_cd_ov_noalloc_dev): every GPU the
@@ -469,6 +478,10 @@ struct WnmResult {
     int chol_fallbacks = 0;
     int singular = 0;
     int pi_variant = 0;
+    // deferred failure check (build_wnm with defer_check): the caller downloads fail_dev with
+    // its own results and, if any frequency failed, runs fallback (returns the singular count)
+    const int* fail_dev = nullptr;
+    std::function<int(const std::vector<int>&)> fallback;
 };
 
 // Returns the variant to use for `kind` at this shape, timing the candidates on first use
@@ -563,7 +576,7 @@ const StreamFan* potrf_fan(qx_handle h, int& groups) {
 
 WnmResult build_wnm(qx_handle h, int precision, int nocc, int nvir, int naux,
                     const double* eps_dev, const double* B_ov, int nf, const double* freqs_dev,
-                    int npair, const double* B_pair, double* wnm_out) {
+                    int npair, const double* B_pair, double* wnm_out, bool defer_check = false) {
     WnmResult r;
     const int nov = nocc * nvir;
     const int npad = round_up(naux, NB), nblk = npad / NB, kpad = round_up(nov, KPAD);
@@ -632,12 +645,9 @@ WnmResult build_wnm(qx_handle h, int precision, int nocc, int nvir, int naux,
         Phase ph(h, QX_PHASE_WNM);
         HIPCK(h, launch_wnm(precision, bp, x, npair, npad, nf, wpart, wnm_out, &mx, st, false));
     }
-    std::vector<int> fail_h(nf);
-    d2h(h, fail_h.data(), fail, nf);
-    for (int f = 0; f < nf; ++f) r.chol_fallbacks += fail_h[f] != 0;
-    if (r.chol_fallbacks > 0) {
-        // Reference chain: Cholesky failed somewhere -> LU inverse (here Gauss-Jordan with
-        // partial pivoting on every frequency) -> identity if singular.
+    // Reference chain: Cholesky failed somewhere -> LU inverse (here Gauss-Jordan with
+    // partial pivoting on every frequency) -> identity if singular.  Recomputes wnm_out.
+    auto fallback = [=](const std::vector<int>&) {
         double* pi = ws<double>(h, "xinv", (size_t)nf * npad * npad);
         fill_f64(h, pi, 0.0, (size_t)nf * npad * npad);
         HIPCK(h, launch_pi_syrk(precision, B_ov, ops, s, nov, naux, npad, nf, pi, PiOut::PiFull,
@@ -647,10 +657,21 @@ WnmResult build_wnm(qx_handle h, int precision, int nocc, int nvir, int naux,
         HIPCK(h, launch_lu_fallback_wc(pi, 0, work, wc, npad, nf, nullptr, st));
         std::vector<int> sing(nf);
         d2h(h, sing.data(), reinterpret_cast<int*>(work + (size_t)nf * npad * 2 * npad), nf);
-        for (int f = 0; f < nf; ++f) r.singular += sing[f] != 0;
+        int nsing = 0;
+        for (int f = 0; f < nf; ++f) nsing += sing[f] != 0;
         if (npair > 0)
             HIPCK(h, launch_wnm_from_wc(bp, wc, npair, npad, nf, wpart, wnm_out, st));
+        return nsing;
+    };
+    if (defer_check) {
+        r.fail_dev = fail;
+        r.fallback = fallback;
+        return r;
     }
+    std::vector<int> fail_h(nf);
+    d2h(h, fail_h.data(), fail, nf);
+    for (int f = 0; f < nf; ++f) r.chol_fallbacks += fail_h[f] != 0;
+    if (r.chol_fallbacks > 0) r.singular = fallback(fail_h);
     return r;
 }
 
@@ -664,6 +685,7 @@ struct HostCD {
     std::vector<double> eps_g, eps_ref;
     std::vector<int> targets, pair_m, pair_off;
     double ef, eta;
+    bool host_ready = false;       // eps_g, eps_ref, pair_m and ef hold the device values
     SigmaProblem prob;             // device problem (all targets)
     int residue_jobs = 0;
 
@@ -803,11 +825,33 @@ void validate_desc(qx_handle h, const qx_g0w0_desc* d) {
 }
 
 // Builds the host+device CD problem (pair CSR, grid) shared by g0w0 and sigma_cd.
+// known (the ov path, whose pair list the library generated from targets it validated):
+// the host targets and pair offsets are given and pair_off_dev already holds the offsets, so
+// nothing is downloaded here; the host copies of the energies and pair partners that only
+// the residue path reads are fetched by cd_host_data when a target needs it.
+struct KnownPairs {
+    const std::vector<int>* targets;
+    const std::vector<int>* off;
+    int* off_dev;
+};
+
+void cd_host_data(qx_handle h, const qx_g0w0_desc* d, const double* mo_energy,
+                  const double* mo_energy_ref, const int* pair_m, HostCD& cd) {
+    D2HBatch bt(h);
+    bt.add(cd.eps_g.data(), mo_energy, cd.nmo);
+    bt.add(cd.eps_ref.data(), mo_energy_ref ? mo_energy_ref : mo_energy, cd.nmo);
+    bt.add(cd.pair_m.data(), pair_m, cd.npair);
+    bt.run();
+    cd.ef = 0.5 * (cd.eps_g[d->nocc - 1] + cd.eps_g[d->nocc]);
+    cd.prob.ef = cd.ef;
+    cd.host_ready = true;
+}
+
 void setup_cd(qx_handle h, const qx_g0w0_desc* d, const double* mo_energy,
               const double* mo_energy_ref, const double* B_ov, const double* B_pair,
               const int* pair_target, const int* pair_m, const int* targets,
               const double* static_terms, const double* freqs_dev, const double* wts_dev,
-              const double* wnm, HostCD& cd) {
+              const double* wnm, HostCD& cd, const KnownPairs* known = nullptr) {
     const int nmo = d->nocc + d->nvir, nt = d->ntarget, npair = d->npair;
     cd.h = h;
     cd.nocc = d->nocc;
@@ -824,8 +868,31 @@ void setup_cd(qx_handle h, const qx_g0w0_desc* d, const double* mo_energy,
     cd.eps_g.resize(nmo);
     cd.eps_ref.resize(nmo);
     cd.targets.resize(nt);
-    std::vector<int> pt(npair);
     cd.pair_m.resize(npair);
+    SigmaProblem& p = cd.prob;
+    p.nt = nt;
+    p.npair = npair;
+    p.nf = d->nfreq;
+    p.nocc = d->nocc;
+    p.nmo = nmo;
+    p.targets = targets;
+    p.pair_m = pair_m;
+    p.wnm = wnm;
+    p.eps_g = mo_energy;
+    p.eps_ref = mo_energy_ref ? mo_energy_ref : mo_energy;
+    p.freqs = freqs_dev;
+    p.wts = wts_dev;
+    p.static_terms = static_terms;
+    p.eta = d->eta;
+    if (known) {
+        cd.targets = *known->targets;
+        cd.pair_off = *known->off;
+        p.pair_off = known->off_dev;
+        p.ef = 0.0;                          // formed on the device by the Newton kernel
+        cd.host_ready = false;
+        return;
+    }
+    std::vector<int> pt(npair);
     {
         D2HBatch bt(h);
         bt.add(cd.eps_g.data(), mo_energy, nmo);
@@ -852,23 +919,9 @@ void setup_cd(qx_handle h, const qx_g0w0_desc* d, const double* mo_energy,
     cd.ef = 0.5 * (cd.eps_g[d->nocc - 1] + cd.eps_g[d->nocc]);
     int* off_dev = ws<int>(h, "pair_off", nt + 1);
     h2d(h, off_dev, cd.pair_off.data(), nt + 1);
-    SigmaProblem& p = cd.prob;
-    p.nt = nt;
-    p.npair = npair;
-    p.nf = d->nfreq;
-    p.nocc = d->nocc;
-    p.nmo = nmo;
-    p.targets = targets;
     p.pair_off = off_dev;
-    p.pair_m = pair_m;
-    p.wnm = wnm;
-    p.eps_g = mo_energy;
-    p.eps_ref = mo_energy_ref ? mo_energy_ref : mo_energy;
-    p.freqs = freqs_dev;
-    p.wts = wts_dev;
-    p.static_terms = static_terms;
-    p.eta = d->eta;
     p.ef = cd.ef;
+    cd.host_ready = true;
 }
 
 qx_status run_g0w0(qx_handle h, const qx_g0w0_desc* d, const double* mo_energy,
@@ -876,7 +929,7 @@ qx_status run_g0w0(qx_handle h, const qx_g0w0_desc* d, const double* mo_energy,
                    const int* pair_target, const int* pair_m, const int* targets,
                    const double* static_terms, const double* guess, double* qp_energy,
                    double* z_factor, double* sigma_re, double* sigma_im, int* state,
-                   qx_stats* stats) {
+                   qx_stats* stats, const KnownPairs* known = nullptr) {
     const auto t0 = std::chrono::steady_clock::now();
     validate_desc(h, d);
     REQUIRE(h, mo_energy && B_ov && targets && qp_energy && z_factor && sigma_re && sigma_im,
@@ -884,62 +937,95 @@ qx_status run_g0w0(qx_handle h, const qx_g0w0_desc* d, const double* mo_energy,
     REQUIRE(h, d->npair == 0 || (B_pair && pair_target && pair_m), QX_ERR_INVALID_INPUT,
             "pair arrays are NULL");
     const int nf = d->nfreq, nt = d->ntarget, npair = d->npair;
-    std::vector<double> fr, wt;
-    scaled_roots(nf, d->x0, fr, wt);
-    double* fr_dev = ws<double>(h, "freqs", nf);
-    double* wt_dev = ws<double>(h, "wts", nf);
-    h2d(h, fr_dev, fr.data(), nf);
-    h2d(h, wt_dev, wt.data(), nf);
-    double* zero_static = nullptr;
+    double* fr_dev = ws<double>(h, "cd_fr", nf);
+    double* wt_dev = ws<double>(h, "cd_wt", nf);
+    if (h->arena || h->grid_fr != fr_dev || h->grid_nf != nf || h->grid_x0 != d->x0) {
+        std::vector<double> fr, wt;
+        scaled_roots(nf, d->x0, fr, wt);
+        h2d(h, fr_dev, fr.data(), nf);
+        h2d(h, wt_dev, wt.data(), nf);
+        // a buffer that grows is reallocated and both names are taken together, so an
+        // unchanged fr pointer and key mean both still hold this grid
+        h->grid_fr = h->arena ? nullptr : fr_dev;
+        h->grid_nf = nf;
+        h->grid_x0 = d->x0;
+    }
     if (!static_terms) {
-        zero_static = ws<double>(h, "zero_static", nt);
-        fill_f64(h, zero_static, 0.0, nt);
+        double* zero_static = ws<double>(h, "zero_static", nt);
+        if (h->arena || h->zero_ptr != zero_static || h->zero_n < nt) {
+            fill_f64(h, zero_static, 0.0, nt);
+            h->zero_ptr = h->arena ? nullptr : zero_static;
+            h->zero_n = nt;
+        }
         static_terms = zero_static;
     }
     double* wnm = ws<double>(h, "wnm", (size_t)nf * std::max(npair, 1));
     HostCD cd;
     setup_cd(h, d, mo_energy, mo_energy_ref, B_ov, B_pair, pair_target, pair_m, targets,
-             static_terms, fr_dev, wt_dev, wnm, cd);
+             static_terms, fr_dev, wt_dev, wnm, cd, known);
+    // the Cholesky failure flags come back with the Newton states (one synchronisation)
     WnmResult wr = build_wnm(h, d->precision, d->nocc, d->nvir, d->naux, mo_energy, B_ov, nf,
-                             fr_dev, npair, B_pair, wnm);
+                             fr_dev, npair, B_pair, wnm, true);
     const NewtonParams np = to_params(d->newton, d->qp_solver);
     int* st_dev = ws<int>(h, "newton_state", nt);
     int* it_dev = ws<int>(h, "newton_iters", nt);
-    {
+    auto newton = [&] {
         Phase ph(h, QX_PHASE_NEWTON);
+        // the kernel writes the state code of every target it finished into `state`
         HIPCK(h, launch_newton_cd(cd.prob, np, guess, qp_energy, z_factor, sigma_re, sigma_im,
-                                  st_dev, it_dev, h->stream));
-    }
-    std::vector<int> st(nt);
-    int residue_targets = 0, newton_fail = 0;
-    std::vector<double> guess_h;
-    std::vector<double> stat_h(nt);
+                                  st_dev, it_dev, h->stream, state));
+    };
+    newton();
+    // one synchronisation ends the call unless a frequency's Cholesky failed or a target
+    // needs the residue path on the host
+    std::vector<int> st(nt), fail_h(nf);
     {
         D2HBatch bt(h);
         bt.add(st.data(), st_dev, nt);
-        if (guess) {
-            guess_h.resize(nt);
-            bt.add(guess_h.data(), guess, nt);
-        }
-        bt.add(stat_h.data(), static_terms, nt);
+        bt.add(fail_h.data(), wr.fail_dev, nf);
         bt.run();
     }
-    for (int t = 0; t < nt; ++t) {
-        if (st[t] == kNewtonConverged) { st[t] = QX_QP_NEWTON; continue; }
-        if (st[t] == kNewtonBisection) { st[t] = QX_QP_BISECTION; continue; }
-        if (st[t] == kNewtonLinearized) { st[t] = QX_QP_LINEARIZED; continue; }
-        ++residue_targets;
-        double E, Z;
-        C2 s;
-        st[t] = host_solve(cd, np, t, stat_h[t], guess != nullptr, guess ? guess_h[t] : 0.0, E, Z, s);
-        if (st[t] == QX_QP_LINEARIZED) ++newton_fail;
-        h2d(h, qp_energy + t, &E, 1);
-        h2d(h, z_factor + t, &Z, 1);
-        h2d(h, sigma_re + t, &s.re, 1);
-        h2d(h, sigma_im + t, &s.im, 1);
+    for (int f = 0; f < nf; ++f) wr.chol_fallbacks += fail_h[f] != 0;
+    if (wr.chol_fallbacks > 0) {
+        // Wc_nm and the Newton results above used a failed factor: redo both
+        wr.singular = wr.fallback(fail_h);
+        newton();
+        d2h(h, st.data(), st_dev, nt);
     }
-    if (state) h2d(h, state, st.data(), nt);
-    HIPCK(h, hipStreamSynchronize(h->stream));
+    int residue_targets = 0, newton_fail = 0;
+    for (int t = 0; t < nt; ++t)
+        if (st[t] != kNewtonConverged && st[t] != kNewtonBisection && st[t] != kNewtonLinearized)
+            ++residue_targets;
+    if (residue_targets > 0) {
+        if (!cd.host_ready) cd_host_data(h, d, mo_energy, mo_energy_ref, pair_m, cd);
+        std::vector<double> guess_h;
+        std::vector<double> stat_h(nt);
+        {
+            D2HBatch bt(h);
+            if (guess) {
+                guess_h.resize(nt);
+                bt.add(guess_h.data(), guess, nt);
+            }
+            bt.add(stat_h.data(), static_terms, nt);
+            bt.run();
+        }
+        for (int t = 0; t < nt; ++t) {
+            if (st[t] == kNewtonConverged || st[t] == kNewtonBisection ||
+                st[t] == kNewtonLinearized)
+                continue;
+            double E, Z;
+            C2 s;
+            const int code = host_solve(cd, np, t, stat_h[t], guess != nullptr,
+                                        guess ? guess_h[t] : 0.0, E, Z, s);
+            if (code == QX_QP_LINEARIZED) ++newton_fail;
+            h2d(h, qp_energy + t, &E, 1);
+            h2d(h, z_factor + t, &Z, 1);
+            h2d(h, sigma_re + t, &s.re, 1);
+            h2d(h, sigma_im + t, &s.im, 1);
+            if (state) h2d(h, state + t, &code, 1);
+        }
+        HIPCK(h, hipStreamSynchronize(h->stream));
+    }
     resolve_profile(h);
     if (stats) {
         stats->chol_fallbacks = wr.chol_fallbacks;
@@ -1070,6 +1156,8 @@ qx_status qx_release_workspace(qx_handle h) {
         h->batch = nullptr;
         for (auto& kv : h->bufs) HIPCK(h, hipFree(kv.second.first));
         h->bufs.clear();
+        h->grid_fr = nullptr;              // a new buffer may get a freed one's address
+        h->zero_ptr = nullptr;
         return QX_OK;
     });
 }
@@ -1635,8 +1723,9 @@ qx_status run_g0w0_ov(qx_handle h, const qx_g0w0_desc* d, const double* mo_energ
     HIPCK(h, hipGetLastError());
     qx_g0w0_desc dd = *d;
     dd.npair = npair;
+    const KnownPairs known{&tg, &off, off_dev};
     return run_g0w0(h, &dd, mo_energy, mo_energy_ref, B_ov, rows, pt, pm, targets, static_terms,
-                    guess, qp_energy, z_factor, sigma_re, sigma_im, state, stats);
+                    guess, qp_energy, z_factor, sigma_re, sigma_im, state, stats, &known);
 }
 }  // namespace
 

@@ -219,11 +219,13 @@ enum NewtonState : int {
 };
 
 // Device Newton loop (one wavefront per target), imaginary-axis part only; targets whose
-// iterates reach a residue pole stop with kNewtonNeedsResidue.
+// iterates reach a residue pole stop with kNewtonNeedsResidue.  The Fermi level is formed on
+// the device from p.eps_g (p.ef is not read).  qp_state (nullable): the C API's QX_QP_* code
+// of every target that finished on the device.
 hipError_t launch_newton_cd(const SigmaProblem& p, const NewtonParams& np,
                             const double* guess,        // nullable: linearised guess
                             double* e_out, double* z_out, double* sig_re, double* sig_im,
-                            int* state, int* iters, hipStream_t st);
+                            int* state, int* iters, hipStream_t st, int* qp_state = nullptr);
 // Grouped Newton over the targets of a batch (no guesses); target t of instance q writes
 // output index out_off_q + t.
 struct NewtonInst {

@@ -16,6 +16,7 @@
 //   * Newton: quasix_core/src/qp/newton_solver.rs:174-391 (Richardson 5-point derivative,
 //     adaptive damping, step clip, bisection fallback); iteration-0 linearised guess
 //     quasix_core/src/qp/evgw.rs:730-741.
+#include "../../include/qxgw.h"
 #include "qx_internal.h"
 #include "qx_newton.h"
 
@@ -787,9 +788,15 @@ struct DeviceEval {
 // iteration on the same reduced values, the pair sums are split over all lanes.
 constexpr int NEWTON_NW = 4;
 
+// The Fermi level is formed here from the device energies (the same expression as on the
+// host), so the host needs no copy of them to launch.  qp_state (optional): the C API's code
+// for targets that finished on the device (QX_QP_NEWTON, QX_QP_BISECTION, QX_QP_LINEARIZED);
+// targets that need the residue path are left for the host to fill.
 __global__ __launch_bounds__(64 * NEWTON_NW) void newton_cd_kernel(
-    SigmaProblem p, NewtonParams np, const double* __restrict__ guess, double* e_out,
-    double* z_out, double* sig_re, double* sig_im, int* state, int* iters) {
+    SigmaProblem p0, NewtonParams np, const double* __restrict__ guess, double* e_out,
+    double* z_out, double* sig_re, double* sig_im, int* state, int* iters, int* qp_state) {
+    SigmaProblem p = p0;
+    p.ef = 0.5 * (p.eps_g[p.nocc - 1] + p.eps_g[p.nocc]);
     const int t = blockIdx.x;
     const int n = p.targets[t];
     DeviceEval<NEWTON_NW> ev{&p, t};
@@ -805,6 +812,11 @@ __global__ __launch_bounds__(64 * NEWTON_NW) void newton_cd_kernel(
         sig_im[t] = s.im;
         state[t] = stt;
         iters[t] = it;
+        if (qp_state) {
+            if (stt == kNewtonConverged) qp_state[t] = QX_QP_NEWTON;
+            else if (stt == kNewtonBisection) qp_state[t] = QX_QP_BISECTION;
+            else if (stt == kNewtonLinearized) qp_state[t] = QX_QP_LINEARIZED;
+        }
     }
 }
 
@@ -948,10 +960,10 @@ hipError_t launch_wnm_from_wc(const double* bpairs, const double* wc, int npair,
 
 hipError_t launch_newton_cd(const SigmaProblem& p, const NewtonParams& np, const double* guess,
                             double* e_out, double* z_out, double* sig_re, double* sig_im,
-                            int* state, int* iters, hipStream_t st) {
+                            int* state, int* iters, hipStream_t st, int* qp_state) {
     if (p.nt <= 0) return hipSuccess;
     hipLaunchKernelGGL(newton_cd_kernel, dim3(p.nt), dim3(64 * NEWTON_NW), 0, st, p, np, guess, e_out, z_out,
-                       sig_re, sig_im, state, iters);
+                       sig_re, sig_im, state, iters, qp_state);
     return hipGetLastError();
 }
 

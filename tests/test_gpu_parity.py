@@ -178,6 +178,26 @@ def test_g0w0_full_blocks_with_residues(gpu):
     _close("Z", r["z_factors"], ref["Z"], 1e-5)
 
 
+def test_g0w0_ov_cholesky_fallback(gpu):
+    """Single-instance ov solve with eps = I - Pi indefinite at 9 of 16 frequencies (an
+    occupied level at 3.4 Ha above the virtuals, B scaled by 2.5; smallest |eigenvalue| 0.078
+    per the oracle): the failure flags come back with the Newton states, Wc_nm is redone by
+    the LU fallback and Newton relaunched.  FP64 against the oracle's Cholesky -> LU chain."""
+    from quasix_amd import gw
+    nocc, nvir, naux, nf = 4, 20, 64, 16
+    e, B = O.make_inputs(nocc, nvir, naux, seed=72)
+    e[nocc - 1] = 3.4
+    B = B * 2.5
+    tg = np.arange(nocc - 2, nocc + 2)
+    fr, wt = O.get_scaled_legendre_roots(nf, 0.5)
+    ref = O.g0w0_cd(e, B, nocc, tg, fr, wt, eta=0.01)
+    r = gw.g0w0(e, B, nocc, tg, nfreq=nf, eta=0.01, stats=True)
+    print("fallback stats", r["stats"], "states", r["state"].tolist())
+    assert r["stats"]["chol_fallbacks"] == 9 and r["stats"]["singular_fallbacks"] == 0
+    _close("g0w0 ov fallback", r["qp_energies"], ref["E"], 1e-6, 1.0)
+    _close("g0w0 ov fallback Z", r["z_factors"], ref["Z"], 1e-5, 1.0)
+
+
 def test_g0w0_c3_fp64_and_mixed(gpu):
     """nocc 20 / nvir 200 / naux 900 (a ragged naux), 32 frequencies, HOMO-5..LUMO+5."""
     from quasix_amd import gw

@@ -70,3 +70,41 @@ def test_residues_full_blocks_naux1000_timed(gpu):
     d = float(np.max(np.abs(rm["qp_energies"] - r["qp_energies"]))) * 27211.386245988
     print(f"mixed vs FP64 with residues: {d:.3e} meV")
     assert d < 1.0
+
+
+def test_ov_path_residue_targets(gpu):
+    """The ov-only entry (pairs gathered on the device) launches without downloading the
+    energies or the pair list; a target whose Newton iterate needs the residue path makes it
+    fetch them afterwards.  An occupied level inside the virtual range (HOMO at 2.0 Ha, Fermi
+    level near 1.1 Ha) puts virtual poles between the HOMO's QP energy and the Fermi level.
+    Checked against the oracle's G0W0 solve, and with static terms, guesses and reference
+    energies against its evGW step."""
+    from quasix_amd import gw
+    nocc, nvir, naux, nf = 4, 20, 64, 16
+    e, B = O.make_inputs(nocc, nvir, naux, seed=70)
+    e[nocc - 1] = 2.0
+    tg = np.arange(nocc - 2, nocc + 2)
+    fr, wt = O.get_scaled_legendre_roots(nf, 0.5)
+    ref = O.g0w0_cd(e, B, nocc, tg, fr, wt, eta=0.01)
+    r = gw.g0w0(e, B, nocc, tg, nfreq=nf, eta=0.01, stats=True)
+    st = r["stats"]
+    dE = float(np.max(np.abs(r["qp_energies"] - ref["E"])))
+    print(f"ov residues: {st['residue_targets']} targets, {st['residue_jobs']} jobs, "
+          f"states {r['state'].tolist()}, max |dE| {dE:.3e} Ha")
+    assert st["residue_targets"] > 0 and st["residue_jobs"] > 0
+    assert dE < 1e-6
+    assert np.max(np.abs(r["z_factors"] - ref["Z"])) < 1e-5
+    rng = np.random.default_rng(71)
+    static = 0.05 * rng.standard_normal(len(tg))
+    eref = e + 0.01 * rng.standard_normal(e.shape)
+    guess = ref["E"] + 0.002
+    # the library's mo_energy_ref anchors the QP equation and sets the eta sign, as in the
+    # oracle's evGW step (Green's function and poles from e, anchor at eref)
+    ref2 = O._g0w0_iter(e, eref, B, nocc, tg, fr, wt, static, 0.01, guess)
+    r2 = gw.g0w0(e, B, nocc, tg, nfreq=nf, eta=0.01, static=static, mo_energy_ref=eref,
+                 guess=guess, stats=True)
+    dE2 = float(np.max(np.abs(r2["qp_energies"] - ref2["E"])))
+    print(f"ov residues with static/ref/guess: {r2['stats']['residue_targets']} targets, "
+          f"max |dE| {dE2:.3e} Ha")
+    assert r2["stats"]["residue_targets"] > 0
+    assert dE2 < 1e-6
