@@ -225,25 +225,33 @@ __device__ __forceinline__ void split4(const dv4 x, bf16x4& h, bf16x4& l) {
     l = __builtin_convertvector(__builtin_convertvector(r, f4), bf16x4);
 }
 
-// hi/lo split of M_f = X_f - I over the lower 64-blocks (zero above them).
-__global__ void split_m_kernel(const double* __restrict__ x, int npad, int nf,
-                               __bf16* __restrict__ mh, __bf16* __restrict__ ml) {
-    const size_t total = (size_t)nf * npad * npad / 4;
-    for (size_t i4 = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i4 < total;
-         i4 += (size_t)gridDim.x * blockDim.x) {
-        const size_t i = i4 * 4;
-        const size_t rc = i % ((size_t)npad * npad);
-        const int P = rc / npad, Q = rc % npad;
+// hi/lo split of row P of M_f = X_f - I (row r = f npad + P of the [nf npad][npad] arrays).
+// The Wc_nm kernels read row P only at columns Q < min(128 (P / 128) + 128, npad), their K
+// range for the 128-row tile of P, so only those are written: X over the lower 64-blocks,
+// zeros in the 64-block above the diagonal of an even block row.  The rest of the arrays is
+// never read (and not written).
+__device__ __forceinline__ void split_m_row(const double* __restrict__ x, int npad, size_t r,
+                                            __bf16* __restrict__ mh, __bf16* __restrict__ ml) {
+    const int P = r % npad;
+    const int qlim = min((P / WT + 1) * WT, npad);
+    const size_t base = r * npad;
+    for (int Q = threadIdx.x * 4; Q < qlim; Q += blockDim.x * 4) {
         dv4 v = dv4{0.0, 0.0, 0.0, 0.0};
         if (Q / NB <= P / NB) {
-            v = *reinterpret_cast<const dv4*>(x + i);
+            v = *reinterpret_cast<const dv4*>(x + base + Q);
             if (P >= Q && P < Q + 4) v[P - Q] -= 1.0;
         }
         bf16x4 h, l;
         split4(v, h, l);
-        *reinterpret_cast<bf16x4*>(mh + i) = h;
-        *reinterpret_cast<bf16x4*>(ml + i) = l;
+        *reinterpret_cast<bf16x4*>(mh + base + Q) = h;
+        *reinterpret_cast<bf16x4*>(ml + base + Q) = l;
     }
+}
+
+__global__ void split_m_kernel(const double* __restrict__ x, int npad, int nf,
+                               __bf16* __restrict__ mh, __bf16* __restrict__ ml) {
+    const size_t rows = (size_t)nf * npad;
+    for (size_t r = blockIdx.x; r < rows; r += gridDim.x) split_m_row(x, npad, r, mh, ml);
 }
 
 // hi/lo split of the pair rows, rows padded with zeros to npair_pad.
@@ -555,8 +563,8 @@ __global__ __launch_bounds__(256, 2) void wnm_bf16x3_f2_group_kernel(
 }
 
 // Grouped hi/lo split for a batch, blockIdx.y = instance: the pair rows (zero padded to npp
-// rows) and M_f = X_f - I over the lower 64-blocks (zero above them), as split_rows_kernel
-// and split_m_kernel do for one instance.
+// rows) and the rows of M_f = X_f - I that the Wc_nm kernels read, as split_rows_kernel and
+// split_m_kernel do for one instance.
 __global__ void split_group_kernel(const WnmInst* __restrict__ inst, int nf) {
     const WnmInst in = inst[blockIdx.y];
     const size_t rtot = (size_t)in.npp * in.npad / 4;
@@ -571,22 +579,8 @@ __global__ void split_group_kernel(const WnmInst* __restrict__ inst, int nf) {
         *reinterpret_cast<bf16x4*>(in.bh + i) = h;
         *reinterpret_cast<bf16x4*>(in.bl + i) = l;
     }
-    const size_t mat = (size_t)in.npad * in.npad, mtot = (size_t)nf * mat / 4;
-    for (size_t i4 = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i4 < mtot;
-         i4 += (size_t)gridDim.x * blockDim.x) {
-        const size_t i = i4 * 4;
-        const size_t rc = i % mat;
-        const int P = rc / in.npad, Q = rc % in.npad;
-        dv4 v = dv4{0.0, 0.0, 0.0, 0.0};
-        if (Q / NB <= P / NB) {
-            v = *reinterpret_cast<const dv4*>(in.x + i);
-            if (P >= Q && P < Q + 4) v[P - Q] -= 1.0;
-        }
-        bf16x4 h, l;
-        split4(v, h, l);
-        *reinterpret_cast<bf16x4*>(in.mh + i) = h;
-        *reinterpret_cast<bf16x4*>(in.ml + i) = l;
-    }
+    const size_t rows = (size_t)nf * in.npad;
+    for (size_t r = blockIdx.x; r < rows; r += gridDim.x) split_m_row(in.x, in.npad, r, in.mh, in.ml);
 }
 
 __device__ __forceinline__ double wave_sum(double v) {
