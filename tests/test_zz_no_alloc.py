@@ -60,3 +60,26 @@ def test_noalloc_rejects_oversized_problem(gpu):
     d = gw.make_desc(nocc, nvir, naux, nf, ntarget=len(tg))
     with pytest.raises(L.QxError, match="too large for the no-allocation entry point"):
         _solve(h.lib.qx_g0w0_cd_ov_noalloc_dev, h, d, e, B, tg)
+
+
+def test_noalloc_fallback_and_residues(gpu):
+    """A problem whose Cholesky fails at 9 of 16 frequencies (an occupied level above the
+    virtuals, B scaled by 2.5) and whose HOMO needs the residue path: in no-allocation mode
+    the deferred failure check, the LU fallback, the relaunched Newton and the host residue
+    solve run from the scratch and give the allocating entry's results bit for bit."""
+    import quasix_amd._lib as L
+    from quasix_amd import gw
+    h = L.handle(0)
+    dev = torch.device("cuda", 0)
+    nocc, nvir, naux, nf = 4, 20, 64, 16
+    e, B = O.make_inputs(nocc, nvir, naux, seed=72)
+    e[nocc - 1] = 3.4
+    B = B * 2.5
+    e, B = torch.as_tensor(e, device=dev), torch.as_tensor(B, device=dev)
+    tg = np.arange(nocc - 2, nocc + 2).astype(np.int32)
+    d = gw.make_desc(nocc, nvir, naux, nf, precision="fp64", ntarget=len(tg))
+    ref, st_ref = _solve(h.lib.qx_g0w0_cd_ov_dev, h, d, e, B, tg)
+    got, st = _solve(h.lib.qx_g0w0_cd_ov_noalloc_dev, h, d, e, B, tg)
+    print(f"noalloc fallback: states {st.tolist()}, max diff {np.max(np.abs(got - ref)):.3e}")
+    assert np.array_equal(st, st_ref)
+    assert np.array_equal(got, ref)
