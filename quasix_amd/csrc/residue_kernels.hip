@@ -21,6 +21,7 @@
 #include "qx_internal.h"
 
 #include <algorithm>
+#include <cstdlib>
 
 namespace qx {
 namespace {
@@ -77,6 +78,17 @@ __global__ void res_setup_kernel(double* __restrict__ P, int n, int npad,
             rhs[(size_t)j * 2 * npad + npad + r] = 0.0;
         }
     }
+}
+
+// 1 / p for the pivot p = pr + i pi, with |p|^2 formed without fused multiply-adds, so both
+// panel kernels get the same bits whatever the surrounding code; ok = false for p = 0.
+__device__ __forceinline__ void pivot_inverse(double pr, double pi, double& ir, double& ii,
+                                              bool& ok) {
+#pragma clang fp contract(off)
+    const double pd = pr * pr + pi * pi;
+    ok = pd > 0.0;
+    ir = ok ? pr / pd : 0.0;
+    ii = ok ? -pi / pd : 0.0;
 }
 
 // Panel k0 .. k0 + 63 of job blockIdx.x: partial pivoting over rows c .. npad - 1, whole-row
@@ -137,11 +149,10 @@ __global__ __launch_bounds__(256) void lu_panel_kernel(double* __restrict__ P, i
         }
         __syncthreads();
         if (tid == 0) {
-            const double pr = Ar[(size_t)c * npad + c], pi = Ai[(size_t)c * npad + c];
-            const double pd = pr * pr + pi * pi;
-            if (!(pd > 0.0)) fail[j] = 1;
-            piv_inv[0] = pd > 0.0 ? pr / pd : 0.0;
-            piv_inv[1] = pd > 0.0 ? -pi / pd : 0.0;
+            bool ok;
+            pivot_inverse(Ar[(size_t)c * npad + c], Ai[(size_t)c * npad + c], piv_inv[0],
+                          piv_inv[1], ok);
+            if (!ok) fail[j] = 1;
         }
         __syncthreads();
         const double ir = piv_inv[0], ii = piv_inv[1];
@@ -162,6 +173,156 @@ __global__ __launch_bounds__(256) void lu_panel_kernel(double* __restrict__ P, i
                 const double lr = Ar[orc], li = Ai[orc], ur = Ar[ocq], ui = Ai[ocq];
                 Ar[o] -= lr * ur - li * ui;
                 Ai[o] -= lr * ui + li * ur;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Argmax over the workgroup of (v, row): the larger v, the smaller row on ties (so the
+// pivot is the first row of maximal |a|^2, as in lu_panel_kernel).  Every thread passes
+// its candidate; the result is returned to every thread.  Three barriers.
+constexpr int PNW = 16;                // waves of lu_panel_rows_kernel
+__device__ __forceinline__ int block_argmax(double v, int r, double* sv, int* sr) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    #pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const double ov = __shfl_xor(v, off, 64);
+        const int orow = __shfl_xor(r, off, 64);
+        if (ov > v || (ov == v && orow < r)) {
+            v = ov;
+            r = orow;
+        }
+    }
+    if (lane == 0) {
+        sv[wv] = v;
+        sr[wv] = r;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < PNW; ++w)
+            if (sv[w] > v || (sv[w] == v && sr[w] < r)) {
+                v = sv[w];
+                r = sr[w];
+            }
+        sr[PNW] = r;
+    }
+    __syncthreads();
+    const int piv = sr[PNW];
+    __syncthreads();                      // sv/sr are rewritten by the next call
+    return piv;
+}
+
+// The panel of lu_panel_kernel with few dependent memory round trips per column: 1024
+// threads as 128 row lanes x 8 column groups.  Per column c the pivot row is chosen from
+// candidates tracked during the previous column's update (a search pass only for the
+// panel's first column); one pass swaps rows c and piv and puts the pivot inverse and the
+// U entries of row c right of c into LDS; then each thread takes rows r = c + 1 + (tid / 8)
+// + 128 k and,
+// in each, the panel columns c + 1 + (tid % 8) + 8 i right of c (at most 8): all of a
+// row's loads are issued together, the multiplier of column c is formed (and stored by
+// column group 0), the entries are updated and stored, and column group 0 tracks |a|^2 of
+// column c + 1 as the next pivot candidate.  So a column costs one round trip per 128 rows
+// instead of one per 64 (8 columns per thread is what fits in the registers of 16 waves).  Same pivots (the first row of maximal |a|^2) and the same
+// arithmetic as lu_panel_kernel, so the factors are bit-identical.
+__global__ __launch_bounds__(64 * PNW) void lu_panel_rows_kernel(double* __restrict__ P,
+                                                                 int npad, int k0,
+                                                                 double* __restrict__ rhs,
+                                                                 int* __restrict__ fail) {
+    __shared__ double sv[PNW];
+    __shared__ int sr[PNW + 1];
+    __shared__ double Ur[LB], Ui[LB], pinv[2];
+    constexpr int NT = 64 * PNW, CG = 8, RG = NT / CG, CPT = LB / CG;
+    const int j = blockIdx.x, tid = threadIdx.x, rg = tid / CG, cg = tid % CG;
+    const size_t nn = (size_t)npad * npad;
+    double* Ar = P + (size_t)(2 * j) * nn;
+    double* Ai = Ar + nn;
+    double* rr = rhs + (size_t)j * 2 * npad;
+    double* ri = rr + npad;
+    double best = -1.0;
+    int brow = k0;
+    for (int r = k0 + tid; r < npad; r += NT) {
+        const double x = Ar[(size_t)r * npad + k0], y = Ai[(size_t)r * npad + k0];
+        const double v = x * x + y * y;
+        if (v > best) {
+            best = v;
+            brow = r;
+        }
+    }
+    for (int c = k0; c < k0 + LB; ++c) {
+        const int piv = block_argmax(best, brow, sv, sr);
+        const int w = k0 + LB - c - 1;            // panel columns right of c
+        // one pass: swap rows c and piv (whole rows) and hand the new row c's panel entries
+        // to LDS: the pivot's inverse from column c, the U entries from columns c + 1 ..
+        for (int q = tid; q < npad; q += NT) {
+            const bool inpanel = q >= c && q <= c + w;
+            if (piv == c && !inpanel) continue;
+            const size_t ac = (size_t)c * npad + q;
+            double cr = Ar[ac], ci = Ai[ac];
+            if (piv != c) {
+                const size_t ap = (size_t)piv * npad + q;
+                const double xr = Ar[ap], xi = Ai[ap];
+                Ar[ac] = xr;
+                Ai[ac] = xi;
+                Ar[ap] = cr;
+                Ai[ap] = ci;
+                cr = xr;
+                ci = xi;
+            }
+            if (q == c) {
+                bool ok;
+                pivot_inverse(cr, ci, pinv[0], pinv[1], ok);
+                if (!ok) fail[j] = 1;
+            } else if (inpanel) {
+                Ur[q - c - 1] = cr;
+                Ui[q - c - 1] = ci;
+            }
+        }
+        if (piv != c && tid == 0) {
+            double t = rr[c];
+            rr[c] = rr[piv];
+            rr[piv] = t;
+            t = ri[c];
+            ri[c] = ri[piv];
+            ri[piv] = t;
+        }
+        __syncthreads();
+        const double ir = pinv[0], ii = pinv[1];
+        best = -1.0;
+        brow = c + 1;
+        for (int r = c + 1 + rg; r < npad; r += RG) {
+            double* rowr = Ar + (size_t)r * npad;
+            double* rowi = Ai + (size_t)r * npad;
+            const double x = rowr[c], y = rowi[c];
+            double a[CPT], b[CPT];
+            #pragma unroll
+            for (int i = 0; i < CPT; ++i) {
+                const int qi = cg + CG * i;            // column c + 1 + qi
+                a[i] = qi < w ? rowr[c + 1 + qi] : 0.0;
+                b[i] = qi < w ? rowi[c + 1 + qi] : 0.0;
+            }
+            const double lr = x * ir - y * ii, li = x * ii + y * ir;
+            #pragma unroll
+            for (int i = 0; i < CPT; ++i) {
+                const int qi = cg + CG * i;
+                if (qi < w) {
+                    const double ur = Ur[qi], ui = Ui[qi];
+                    const double ar = a[i] - (lr * ur - li * ui);
+                    const double ai = b[i] - (lr * ui + li * ur);
+                    rowr[c + 1 + qi] = ar;
+                    rowi[c + 1 + qi] = ai;
+                    if (qi == 0) {
+                        const double v = ar * ar + ai * ai;
+                        if (v > best) {
+                            best = v;
+                            brow = r;
+                        }
+                    }
+                }
+            }
+            if (cg == 0) {
+                rowr[c] = lr;
+                rowi[c] = li;
             }
         }
         __syncthreads();
@@ -409,9 +570,18 @@ hipError_t launch_residue_lu_solve(double* P, int n, int npad, int njobs, const 
     dim3 gs((unsigned)std::min<size_t>((nn + 255) / 256, 4096), njobs);
     hipLaunchKernelGGL(res_setup_kernel, gs, dim3(256), 0, st, P, n, npad, brows, rhs);
     const int nb = npad / LB;
+    // QXGW_LU_PANEL=0: the four-wave panel kernel with separate search, scale and update
+    // passes instead of lu_panel_rows_kernel
+    const char* pv = std::getenv("QXGW_LU_PANEL");
+    const bool fused = !(pv && std::atoi(pv) == 0);
     for (int k = 0; k < nb; ++k) {
         const int k0 = k * LB, rest = nb - k - 1;
-        hipLaunchKernelGGL(lu_panel_kernel, dim3(njobs), dim3(256), 0, st, P, npad, k0, rhs, fail);
+        if (fused)
+            hipLaunchKernelGGL(lu_panel_rows_kernel, dim3(njobs), dim3(64 * PNW), 0, st, P,
+                               npad, k0, rhs, fail);
+        else
+            hipLaunchKernelGGL(lu_panel_kernel, dim3(njobs), dim3(256), 0, st, P, npad, k0, rhs,
+                               fail);
         if (rest > 0) {
             hipLaunchKernelGGL(lu_trsm_kernel, dim3(rest, njobs), dim3(64), 0, st, P, npad, k0);
             hipLaunchKernelGGL(lu_update_kernel, dim3(rest, rest, njobs), dim3(256), 0, st, P,

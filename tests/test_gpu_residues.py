@@ -108,3 +108,25 @@ def test_ov_path_residue_targets(gpu):
           f"max |dE| {dE2:.3e} Ha")
     assert r2["stats"]["residue_targets"] > 0
     assert dE2 < 1e-6
+
+
+def test_residue_lu_panel_forms_agree(gpu, monkeypatch):
+    """The 16-wave fused LU panel (default) and the four-wave panel with separate search,
+    scale and update passes (QXGW_LU_PANEL=0) choose the same pivots and do the same
+    arithmetic: bit-identical QP energies on a residue workload (full blocks, naux 320)."""
+    from quasix_amd import gw
+    nocc, nvir, naux, nf = 6, 30, 320, 24
+    e, B, miP, abP = _full_model(nocc, nvir, naux, seed=52)
+    mn = O.assemble_mnP(B, miP, abP, nocc, nvir, naux)
+    Bs = np.ascontiguousarray(mn[:nocc, nocc:].reshape(nocc * nvir, naux))
+    tg = np.arange(nocc - 4, nocc + 3)
+    pairs = gw.pairs_from_blocks(B, miP, abP, nocc, nvir, naux, tg)
+    monkeypatch.setenv("QXGW_LU_PANEL", "0")
+    r0 = gw.g0w0(e, Bs, nocc, tg, nfreq=nf, eta=0.01, pairs=pairs, stats=True)
+    monkeypatch.setenv("QXGW_LU_PANEL", "1")
+    r1 = gw.g0w0(e, Bs, nocc, tg, nfreq=nf, eta=0.01, pairs=pairs, stats=True)
+    print(f"lu panel forms: {r1['stats']['residue_jobs']} jobs, max |dE| "
+          f"{np.max(np.abs(r1['qp_energies'] - r0['qp_energies'])):.3e} Ha")
+    assert r1["stats"]["residue_jobs"] > 0
+    assert np.array_equal(r1["qp_energies"], r0["qp_energies"])
+    assert np.array_equal(r1["z_factors"], r0["z_factors"])
