@@ -21,6 +21,9 @@
 #include <map>
 #include <memory>
 #include <mutex>
+#include <exception>
+#include <thread>
+#include <condition_variable>
 #include <string>
 #include <tuple>
 #include <vector>
@@ -689,39 +692,71 @@ struct HostCD {
     SigmaProblem prob;             // device problem (all targets)
     int residue_jobs = 0;
 
+    // One evaluation request: Sigma_c(E) of target t at nq (<= 8) energies into out.
+    struct Req {
+        int t;
+        const double* E;
+        int nq;
+        C2* out;
+    };
+
     // Sigma_c(E) for target t at nq energies: I on the GPU quadrature kernel, residues as
     // batched real-axis solves.
     void eval(int t, const double* E, int nq, C2* out) {
-        SigmaProblem sub = prob;
-        sub.nt = 1;
-        sub.targets = prob.targets + t;
-        sub.pair_off = prob.pair_off + t;
-        sub.static_terms = prob.static_terms + t;
-        double* e_dev = ws<double>(h, "hcd_E", 8);
-        double* o_dev = ws<double>(h, "hcd_out", 16);
-        h2d(h, e_dev, E, nq);
-        HIPCK(h, launch_sigma_imag_eval(sub, e_dev, nq, o_dev, h->stream));
-        std::vector<double> o(2 * nq);
-        d2h(h, o.data(), o_dev, 2 * nq);
-        for (int q = 0; q < nq; ++q) out[q] = C2{o[2 * q], o[2 * q + 1]};
-        // residue jobs
+        std::vector<Req> one{Req{t, E, nq, out}};
+        eval_many(one);
+    }
+
+    // Several requests at once (the Newton iterations of several residue targets in
+    // lockstep): the quadrature part of each request on the GPU, read back together, then
+    // every residue job of every request in as few batches as the workspace allows.  Each
+    // job's value does not depend on which batch it is in, and each request accumulates its
+    // jobs in the order eval() would, so the results equal those of one eval() per request.
+    void eval_many(std::vector<Req>& reqs) {
+        const int nr = (int)reqs.size();
+        double* e_dev = ws<double>(h, "hcd_E", (size_t)8 * nr);
+        double* o_dev = ws<double>(h, "hcd_out", (size_t)16 * nr);
+        std::vector<std::vector<double>> o(nr);
+        for (int k = 0; k < nr; ++k) {
+            const Req& r = reqs[k];
+            SigmaProblem sub = prob;
+            sub.nt = 1;
+            sub.targets = prob.targets + r.t;
+            sub.pair_off = prob.pair_off + r.t;
+            sub.static_terms = prob.static_terms + r.t;
+            h2d(h, e_dev + 8 * k, r.E, r.nq);
+            HIPCK(h, launch_sigma_imag_eval(sub, e_dev + 8 * k, r.nq, o_dev + 16 * k, h->stream));
+            o[k].resize(2 * r.nq);
+        }
+        {
+            D2HBatch bt(h);
+            for (int k = 0; k < nr; ++k) bt.add(o[k].data(), o_dev + 16 * k, 2 * reqs[k].nq);
+            bt.run();
+        }
+        for (int k = 0; k < nr; ++k)
+            for (int q = 0; q < reqs[k].nq; ++q) reqs[k].out[q] = C2{o[k][2 * q], o[k][2 * q + 1]};
+        // residue jobs of every request, request by request in eval()'s order
         std::vector<double> om;
-        std::vector<int> row, qidx;
+        std::vector<int> row, qidx, ridx;
         std::vector<double> sgn;
-        for (int q = 0; q < nq; ++q)
-            for (int j = pair_off[t]; j < pair_off[t + 1]; ++j) {
-                const double em = eps_g[pair_m[j]];
-                if (E[q] > ef && ef < em && em < E[q]) {
-                    om.push_back(E[q] - em); row.push_back(j); qidx.push_back(q); sgn.push_back(1.0);
-                } else if (E[q] <= ef && E[q] < em && em < ef) {
-                    om.push_back(em - E[q]); row.push_back(j); qidx.push_back(q); sgn.push_back(-1.0);
+        for (int k = 0; k < nr; ++k) {
+            const Req& r = reqs[k];
+            for (int q = 0; q < r.nq; ++q)
+                for (int j = pair_off[r.t]; j < pair_off[r.t + 1]; ++j) {
+                    const double em = eps_g[pair_m[j]];
+                    if (r.E[q] > ef && ef < em && em < r.E[q]) {
+                        om.push_back(r.E[q] - em); row.push_back(j); qidx.push_back(q);
+                        ridx.push_back(k); sgn.push_back(1.0);
+                    } else if (r.E[q] <= ef && r.E[q] < em && em < ef) {
+                        om.push_back(em - r.E[q]); row.push_back(j); qidx.push_back(q);
+                        ridx.push_back(k); sgn.push_back(-1.0);
+                    }
                 }
-            }
+        }
         const int nj = (int)om.size();
         if (nj == 0) return;
         residue_jobs += nj;
-        // all jobs of this evaluation in as few batches as the workspace allows: the complex
-        // eps(z) of a job takes two [npad][npad] planes
+        // the complex eps(z) of a job takes two [npad][npad] planes
         const int nov = nocc * nvir, kpad = round_up(nov, KPAD), npad = round_up(naux, NB);
         const size_t per = (size_t)2 * npad * npad * sizeof(double);
         const int chunk = (int)std::max<size_t>(1, std::min<size_t>(64, (size_t)(1ull << 30) / per));
@@ -749,32 +784,93 @@ struct HostCD {
                                              h->stream));
             std::vector<double> w(2 * cnt);
             std::vector<int> f(cnt);
-            d2h(h, w.data(), wout, 2 * cnt);
-            d2h(h, f.data(), fail, cnt);
+            {
+                D2HBatch bt(h);
+                bt.add(w.data(), wout, 2 * cnt);
+                bt.add(f.data(), fail, cnt);
+                bt.run();
+            }
             for (int k = 0; k < cnt; ++k) {
                 REQUIRE(h, f[k] == 0, QX_ERR_NUMERICAL,
                         "eps(w + i eta) is singular at w = %.6f (residue of partner %d)",
                         om[j0 + k], pair_m[row[j0 + k]]);
-                out[qidx[j0 + k]].re += sgn[j0 + k] * w[2 * k];
-                out[qidx[j0 + k]].im += sgn[j0 + k] * w[2 * k + 1];
+                C2& acc = reqs[ridx[j0 + k]].out[qidx[j0 + k]];
+                acc.re += sgn[j0 + k] * w[2 * k];
+                acc.im += sgn[j0 + k] * w[2 * k + 1];
             }
         }
+    }
+};
+
+// Newton iterations of several residue targets in lockstep: each target's solve runs on a
+// host thread of its own (host arithmetic only); its Sigma evaluations are posted here, and
+// the calling thread, once every live solve has posted one or finished, evaluates all of
+// them with HostCD::eval_many (all device work stays on the calling thread) and wakes them.
+// A failure of the device work stops every solve and is rethrown on the calling thread.
+struct ResidueBatcher {
+    struct Abort {};
+    HostCD* cd;
+    std::mutex mu;
+    std::condition_variable cv_main, cv_thr;
+    int live = 0;
+    long long round = 0;
+    std::vector<HostCD::Req> pending;
+    bool failed = false;
+
+    void eval(int t, const double* E, int nq, C2* out) {     // solve threads
+        std::unique_lock<std::mutex> lk(mu);
+        if (failed) throw Abort{};
+        const long long my = round;
+        pending.push_back(HostCD::Req{t, E, nq, out});
+        cv_main.notify_one();
+        cv_thr.wait(lk, [&] { return round != my; });
+        if (failed) throw Abort{};
+    }
+    void finish() {                                           // solve threads
+        std::lock_guard<std::mutex> lk(mu);
+        --live;
+        cv_main.notify_one();
+    }
+    void run() {                                              // calling thread
+        std::exception_ptr err;
+        std::unique_lock<std::mutex> lk(mu);
+        for (;;) {
+            cv_main.wait(lk, [&] { return (int)pending.size() == live; });
+            if (live == 0) break;
+            std::vector<HostCD::Req> batch;
+            batch.swap(pending);
+            if (!failed) {
+                lk.unlock();
+                try {
+                    cd->eval_many(batch);
+                } catch (...) {
+                    err = std::current_exception();
+                }
+                lk.lock();
+                if (err) failed = true;
+            }
+            ++round;
+            cv_thr.notify_all();
+        }
+        if (err) std::rethrow_exception(err);
     }
 };
 
 struct HostEval {
     HostCD* cd;
     int t;
+    ResidueBatcher* batch = nullptr;     // lockstep solves post their evaluations here
     bool operator()(const double* E, int nq, C2* out) {
-        cd->eval(t, E, nq, out);
+        if (batch) batch->eval(t, E, nq, out);
+        else cd->eval(t, E, nq, out);
         return false;
     }
 };
 
 // Solve target t on the host-driven path; returns the QX_QP_* state.
 int host_solve(HostCD& cd, const NewtonParams& np, int t, double static_t, bool have_guess,
-               double guess, double& E, double& Z, C2& s) {
-    HostEval ev{&cd, t};
+               double guess, double& E, double& Z, C2& s, ResidueBatcher* batch = nullptr) {
+    HostEval ev{&cd, t, batch};
     const int n = cd.targets[t];
     int iters = 0;
     const int st = newton_core(ev, np, cd.eps_ref[n], static_t, cd.ef, have_guess, guess, E, Z,
@@ -1009,20 +1105,55 @@ qx_status run_g0w0(qx_handle h, const qx_g0w0_desc* d, const double* mo_energy,
             bt.add(stat_h.data(), static_terms, nt);
             bt.run();
         }
-        for (int t = 0; t < nt; ++t) {
-            if (st[t] == kNewtonConverged || st[t] == kNewtonBisection ||
-                st[t] == kNewtonLinearized)
-                continue;
-            double E, Z;
-            C2 s;
-            const int code = host_solve(cd, np, t, stat_h[t], guess != nullptr,
-                                        guess ? guess_h[t] : 0.0, E, Z, s);
-            if (code == QX_QP_LINEARIZED) ++newton_fail;
-            h2d(h, qp_energy + t, &E, 1);
-            h2d(h, z_factor + t, &Z, 1);
-            h2d(h, sigma_re + t, &s.re, 1);
-            h2d(h, sigma_im + t, &s.im, 1);
-            if (state) h2d(h, state + t, &code, 1);
+        std::vector<int> rt;                   // targets for the host residue path
+        for (int t = 0; t < nt; ++t)
+            if (st[t] != kNewtonConverged && st[t] != kNewtonBisection &&
+                st[t] != kNewtonLinearized)
+                rt.push_back(t);
+        std::vector<double> Eh(nt), Zh(nt);
+        std::vector<C2> Sh(nt);
+        std::vector<int> code(nt);
+        auto solve = [&](int t, ResidueBatcher* b) {
+            code[t] = host_solve(cd, np, t, stat_h[t], guess != nullptr, guess ? guess_h[t] : 0.0,
+                                 Eh[t], Zh[t], Sh[t], b);
+        };
+        // QXGW_RES_LOCKSTEP=0: one target after the other (each evaluation its own batch)
+        const char* lv = std::getenv("QXGW_RES_LOCKSTEP");
+        if (rt.size() > 1 && !(lv && std::atoi(lv) == 0)) {
+            ResidueBatcher bat;
+            bat.cd = &cd;
+            bat.live = (int)rt.size();
+            std::vector<std::thread> th;
+            std::vector<char> bad(nt, 0);
+            for (int t : rt)
+                th.emplace_back([&, t] {
+                    try {
+                        solve(t, &bat);
+                    } catch (const ResidueBatcher::Abort&) {   // a failed device batch
+                    } catch (...) {
+                        bad[t] = 1;
+                    }
+                    bat.finish();
+                });
+            try {
+                bat.run();
+            } catch (...) {
+                for (auto& x : th) x.join();
+                throw;
+            }
+            for (auto& x : th) x.join();
+            for (int t : rt)
+                REQUIRE(h, !bad[t], QX_ERR_NUMERICAL, "host residue solve of target %d failed", t);
+        } else {
+            for (int t : rt) solve(t, nullptr);
+        }
+        for (int t : rt) {
+            if (code[t] == QX_QP_LINEARIZED) ++newton_fail;
+            h2d(h, qp_energy + t, &Eh[t], 1);
+            h2d(h, z_factor + t, &Zh[t], 1);
+            h2d(h, sigma_re + t, &Sh[t].re, 1);
+            h2d(h, sigma_im + t, &Sh[t].im, 1);
+            if (state) h2d(h, state + t, &code[t], 1);
         }
         HIPCK(h, hipStreamSynchronize(h->stream));
     }

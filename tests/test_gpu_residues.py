@@ -130,3 +130,28 @@ def test_residue_lu_panel_forms_agree(gpu, monkeypatch):
     assert r1["stats"]["residue_jobs"] > 0
     assert np.array_equal(r1["qp_energies"], r0["qp_energies"])
     assert np.array_equal(r1["z_factors"], r0["z_factors"])
+
+
+def test_residue_targets_in_lockstep(gpu, monkeypatch):
+    """Several residue targets: their Newton iterations run in lockstep, every target's
+    residue jobs of a round in one batch (default), or one target after the other
+    (QXGW_RES_LOCKSTEP=0).  A job's value does not depend on its batch, so the results are
+    bit-identical, with the same number of jobs."""
+    from quasix_amd import gw
+    nocc, nvir, naux, nf = 6, 30, 320, 24
+    e, B, miP, abP = _full_model(nocc, nvir, naux, seed=53)
+    mn = O.assemble_mnP(B, miP, abP, nocc, nvir, naux)
+    Bs = np.ascontiguousarray(mn[:nocc, nocc:].reshape(nocc * nvir, naux))
+    tg = np.arange(nocc - 4, nocc + 3)
+    pairs = gw.pairs_from_blocks(B, miP, abP, nocc, nvir, naux, tg)
+    monkeypatch.setenv("QXGW_RES_LOCKSTEP", "0")
+    r0 = gw.g0w0(e, Bs, nocc, tg, nfreq=nf, eta=0.01, pairs=pairs, stats=True)
+    monkeypatch.setenv("QXGW_RES_LOCKSTEP", "1")
+    r1 = gw.g0w0(e, Bs, nocc, tg, nfreq=nf, eta=0.01, pairs=pairs, stats=True)
+    print(f"lockstep: {r1['stats']['residue_targets']} residue targets, "
+          f"{r1['stats']['residue_jobs']} jobs ({r0['stats']['residue_jobs']} sequential)")
+    assert r1["stats"]["residue_targets"] > 1
+    assert r1["stats"]["residue_jobs"] == r0["stats"]["residue_jobs"]
+    for k in ("qp_energies", "z_factors", "state"):
+        assert np.array_equal(r1[k], r0[k]), k
+    assert np.array_equal(r1["sigma_c"], r0["sigma_c"])
