@@ -711,7 +711,8 @@ __global__ void gj_finish_kernel(const double* __restrict__ work, double* __rest
 }  // namespace
 
 hipError_t launch_potrf_batched(double* a, double* x, double* linv_diag, int npad, int nf,
-                                int* fail, hipStream_t st, const StreamFan* fan, int groups) {
+                                int* fail, hipStream_t st, const StreamFan* fan, int groups,
+                                const PotrfAfter* after) {
     const int nblk = npad / NB;
     int ng = fan ? std::min(groups, fan->n + 1) : 1;
     ng = std::max(1, std::min(ng, nf));
@@ -759,6 +760,12 @@ hipError_t launch_potrf_batched(double* a, double* x, double* linv_diag, int npa
         }
         j += pair ? 2 : 1;
     }
+    if (after)
+        for (int g = 0; g < ng; ++g) {
+            const int f0 = g * nf / ng, nfg = (g + 1) * nf / ng - f0;
+            const hipError_t e = (*after)(sg[g], f0, nfg);
+            if (e != hipSuccess) return e;
+        }
     for (int g = 1; g < ng; ++g) {
         hipError_t e = hipEventRecord(fan->join[g - 1], sg[g]);
         if (e == hipSuccess) e = hipStreamWaitEvent(st, fan->join[g - 1], 0);

@@ -637,16 +637,28 @@ WnmResult build_wnm(qx_handle h, int precision, int nocc, int nvir, int naux,
         HIPCK(h, launch_pi_syrk(precision, B_ov, ops, s, nov, naux, npad, nf, eps_mat,
                                 PiOut::EpsLower, st, pv));
     }
+    // mixed: each frequency group's share of the BF16 split of M = X - I runs on that group's
+    // stream right after its factorisation, overlapping the other groups' last steps
+    // (QXGW_SPLIT_EARLY=0: one split pass after the join, inside the Wc_nm launch sequence)
+    const char* se = std::getenv("QXGW_SPLIT_EARLY");
+    const bool split_early =
+        precision == QX_PREC_BF16X3 && npair > 0 && !(se && std::atoi(se) == 0);
+    const size_t mat = (size_t)npad * npad;
+    const PotrfAfter split = [&](hipStream_t gs, int f0, int nfg) {
+        return launch_split_m(x + f0 * mat, npad, nfg, mx.mh + f0 * mat, mx.ml + f0 * mat, gs);
+    };
     {
         Phase ph(h, QX_PHASE_POTRF);
         int groups = 1;
         const StreamFan* fan = potrf_fan(h, groups);
-        HIPCK(h, launch_potrf_batched(eps_mat, x, linv, npad, nf, fail, st, fan, groups));
+        HIPCK(h, launch_potrf_batched(eps_mat, x, linv, npad, nf, fail, st, fan, groups,
+                                      split_early ? &split : nullptr));
     }
     if (npair > 0) {
         // Wc_nm before the failure check: on the rare Cholesky failure it is recomputed below
         Phase ph(h, QX_PHASE_WNM);
-        HIPCK(h, launch_wnm(precision, bp, x, npair, npad, nf, wpart, wnm_out, &mx, st, false));
+        HIPCK(h, launch_wnm(precision, bp, x, npair, npad, nf, wpart, wnm_out, &mx, st, false,
+                            !split_early));
     }
     // Reference chain: Cholesky failed somewhere -> LU inverse (here Gauss-Jordan with
     // partial pivoting on every frequency) -> identity if singular.  Recomputes wnm_out.

@@ -13,6 +13,7 @@
 #include <hip/hip_runtime.h>
 #include <cstdint>
 #include <cstddef>
+#include <functional>
 
 namespace qx {
 
@@ -122,9 +123,12 @@ struct StreamFan {
 // With a fan, the frequencies are split into `groups` contiguous groups whose step loops run
 // on separate streams, so one group's latency-bound diagonal and panel kernels overlap the
 // other groups' column updates.
+// after (optional): called once per group, on that group's stream after its last kernel and
+// before the join, with the group's first frequency and frequency count.
+typedef std::function<hipError_t(hipStream_t, int, int)> PotrfAfter;
 hipError_t launch_potrf_batched(double* a, double* x, double* linv_diag, int npad, int nf,
                                 int* fail, hipStream_t st, const StreamFan* fan = nullptr,
-                                int groups = 1);
+                                int groups = 1, const PotrfAfter* after = nullptr);
 // Grouped factorisation of a batch of instances with different sizes and a common nf.
 // inst_dev: [ninst] sorted by npad descending (nblk_sorted[q] = npad_q / NB on the host);
 // cu_off / rp_off: [ninst + 1] prefix sums of nblk_q * nf and (nblk_q - 1) * nf, on the
@@ -161,7 +165,11 @@ struct WnmMixed {
 // split_rows = false reuses bh/bl from an earlier launch_wnm_split_rows.
 hipError_t launch_wnm(int precision, const double* bpairs, const double* x, int npair,
                       int npad, int nf, double* wpart, double* wnm, const WnmMixed* mixed,
-                      hipStream_t st, bool split_rows = true);
+                      hipStream_t st, bool split_rows = true, bool split_m = true);
+// The BF16 split of M_f = X_f - I (what the mixed Wc_nm kernels read) for nf frequencies
+// starting at x, into mh / ml at the same offsets; launch_wnm with split_m = false expects it.
+hipError_t launch_split_m(const double* x, int npad, int nf, __bf16* mh, __bf16* ml,
+                          hipStream_t st);
 hipError_t launch_wnm_split_rows(const double* bpairs, int npair, int npad,
                                  const WnmMixed* mixed, hipStream_t st);
 // wnm from an explicit Wc: wnm[f][j] = b_j^T Wc_f b_j  (FP64)

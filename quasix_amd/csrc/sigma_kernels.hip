@@ -855,9 +855,16 @@ hipError_t launch_wnm_split_rows(const double* bpairs, int npair, int npad,
     return hipGetLastError();
 }
 
+hipError_t launch_split_m(const double* x, int npad, int nf, __bf16* mh, __bf16* ml,
+                          hipStream_t st) {
+    if (nf <= 0) return hipSuccess;
+    hipLaunchKernelGGL(split_m_kernel, dim3(8192), dim3(256), 0, st, x, npad, nf, mh, ml);
+    return hipGetLastError();
+}
+
 hipError_t launch_wnm(int precision, const double* bpairs, const double* x, int npair, int npad,
                       int nf, double* wpart, double* wnm, const WnmMixed* mixed, hipStream_t st,
-                      bool split_rows) {
+                      bool split_rows, bool split_m) {
     const int nblk = npad / NB;
     if (precision == 2) {
         if (!mixed) return hipErrorInvalidValue;
@@ -865,8 +872,9 @@ hipError_t launch_wnm(int precision, const double* bpairs, const double* x, int 
         if (split_rows)
             hipLaunchKernelGGL(split_rows_kernel, dim3(2048), dim3(256), 0, st, bpairs, npair, npp,
                                npad, mixed->bh, mixed->bl);
-        hipLaunchKernelGGL(split_m_kernel, dim3(8192), dim3(256), 0, st, x, npad, nf, mixed->mh,
-                           mixed->ml);
+        if (split_m)
+            hipLaunchKernelGGL(split_m_kernel, dim3(8192), dim3(256), 0, st, x, npad, nf,
+                               mixed->mh, mixed->ml);
         // QXGW_WNM_F2=0 selects the one-frequency kernel
         const char* fv = std::getenv("QXGW_WNM_F2");
         const bool f2 = !(fv && std::atoi(fv) == 0);
