@@ -778,22 +778,22 @@ struct DeviceEval {
     }
 };
 
-// One workgroup of NEWTON_NW wavefronts per target: every thread runs the same Newton
-// iteration on the same reduced values, the pair sums are split over all lanes.
-constexpr int NEWTON_NW = 4;
+// One workgroup of NW wavefronts per target: every thread runs the same Newton iteration on
+// the same reduced values, the pair sums are split over all lanes.
 
 // The Fermi level is formed here from the device energies (the same expression as on the
 // host), so the host needs no copy of them to launch.  qp_state (optional): the C API's code
 // for targets that finished on the device (QX_QP_NEWTON, QX_QP_BISECTION, QX_QP_LINEARIZED);
 // targets that need the residue path are left for the host to fill.
-__global__ __launch_bounds__(64 * NEWTON_NW) void newton_cd_kernel(
+template <int NW>
+__global__ __launch_bounds__(64 * NW) void newton_cd_kernel(
     SigmaProblem p0, NewtonParams np, const double* __restrict__ guess, double* e_out,
     double* z_out, double* sig_re, double* sig_im, int* state, int* iters, int* qp_state) {
     SigmaProblem p = p0;
     p.ef = 0.5 * (p.eps_g[p.nocc - 1] + p.eps_g[p.nocc]);
     const int t = blockIdx.x;
     const int n = p.targets[t];
-    DeviceEval<NEWTON_NW> ev{&p, t};
+    DeviceEval<NW> ev{&p, t};
     double E = 0.0, Z = 0.0;
     C2 s = {0.0, 0.0};
     int it = 0;
@@ -964,8 +964,15 @@ hipError_t launch_newton_cd(const SigmaProblem& p, const NewtonParams& np, const
                             double* e_out, double* z_out, double* sig_re, double* sig_im,
                             int* state, int* iters, hipStream_t st, int* qp_state) {
     if (p.nt <= 0) return hipSuccess;
-    hipLaunchKernelGGL(newton_cd_kernel, dim3(p.nt), dim3(64 * NEWTON_NW), 0, st, p, np, guess, e_out, z_out,
-                       sig_re, sig_im, state, iters, qp_state);
+    // QXGW_NEWTON_NW = 8 (default) or 4 wavefronts per target (16 spills)
+    const char* v = std::getenv("QXGW_NEWTON_NW");
+    const int nw = v ? std::atoi(v) : 8;
+    if (nw == 8)
+        hipLaunchKernelGGL(newton_cd_kernel<8>, dim3(p.nt), dim3(512), 0, st, p, np, guess,
+                           e_out, z_out, sig_re, sig_im, state, iters, qp_state);
+    else
+        hipLaunchKernelGGL(newton_cd_kernel<4>, dim3(p.nt), dim3(256), 0, st, p, np, guess,
+                           e_out, z_out, sig_re, sig_im, state, iters, qp_state);
     return hipGetLastError();
 }
 
