@@ -577,36 +577,55 @@ const StreamFan* potrf_fan(qx_handle h, int& groups) {
     return &h->fan;
 }
 
+// Pi operand preparation: the frequency weights s_f(r) (and their FP32 copy), and for BF16x3
+// the transposed hi/lo/FP32 copies of B.  launch = false only looks the buffers up (the
+// launches were queued earlier in the call).
+PiOperands pi_prep(qx_handle h, int precision, int nocc, int nvir, int naux, int nf,
+                   const double* eps_dev, const double* B_ov, const double* freqs_dev,
+                   double** s_out, bool launch) {
+    const int nov = nocc * nvir;
+    const int npad = round_up(naux, NB), kpad = round_up(nov, KPAD);
+    hipStream_t st = h->stream;
+    PiOperands ops;
+    double* s = ws<double>(h, "pi_weights", (size_t)nf * kpad);
+    float* s32 = precision == QX_PREC_BF16X3 ? ws<float>(h, "pi_weights32", (size_t)nf * kpad)
+                                             : nullptr;
+    if (launch) HIPCK(h, launch_pi_weights(eps_dev, nocc, nvir, nf, freqs_dev, s, s32, st));
+    ops.s32 = s32;
+    if (precision == QX_PREC_BF16X3) {
+        const int brows = round_up(npad, 256);
+        uint16_t* hi = ws<uint16_t>(h, "bt_hi", (size_t)brows * kpad);
+        uint16_t* lo = ws<uint16_t>(h, "bt_lo", (size_t)brows * kpad);
+        float* f32 = ws<float>(h, "bt_f32", (size_t)brows * kpad);
+        if (launch)
+            HIPCK(h, launch_pi_prepare_mixed(B_ov, nov, naux, brows, kpad, hi, lo, f32, st));
+        ops.bt_hi = hi;
+        ops.bt_lo = lo;
+        ops.bt_f32 = f32;
+        ops.kpad = kpad;
+        ops.rows = brows;
+    }
+    *s_out = s;
+    return ops;
+}
+
 WnmResult build_wnm(qx_handle h, int precision, int nocc, int nvir, int naux,
                     const double* eps_dev, const double* B_ov, int nf, const double* freqs_dev,
-                    int npair, const double* B_pair, double* wnm_out, bool defer_check = false) {
+                    int npair, const double* B_pair, double* wnm_out, bool defer_check = false,
+                    bool prep_queued = false) {
     WnmResult r;
     const int nov = nocc * nvir;
     const int npad = round_up(naux, NB), nblk = npad / NB, kpad = round_up(nov, KPAD);
     hipStream_t st = h->stream;
-    double* s = ws<double>(h, "pi_weights", (size_t)nf * kpad);
     double* eps_mat = ws<double>(h, "eps_mat", (size_t)nf * npad * npad);
     double* linv = ws<double>(h, "linv_diag", (size_t)nf * nblk * NB * NB);
     int* fail = ws<int>(h, "chol_fail", nf);
+    double* s = nullptr;
     PiOperands ops;
     {
         Phase ph(h, QX_PHASE_PREP);
-        float* s32 = precision == QX_PREC_BF16X3 ? ws<float>(h, "pi_weights32", (size_t)nf * kpad)
-                                                 : nullptr;
-        HIPCK(h, launch_pi_weights(eps_dev, nocc, nvir, nf, freqs_dev, s, s32, st));
-        ops.s32 = s32;
-        if (precision == QX_PREC_BF16X3) {
-            const int brows = round_up(npad, 256);
-            uint16_t* hi = ws<uint16_t>(h, "bt_hi", (size_t)brows * kpad);
-            uint16_t* lo = ws<uint16_t>(h, "bt_lo", (size_t)brows * kpad);
-            float* f32 = ws<float>(h, "bt_f32", (size_t)brows * kpad);
-            HIPCK(h, launch_pi_prepare_mixed(B_ov, nov, naux, brows, kpad, hi, lo, f32, st));
-            ops.bt_hi = hi;
-            ops.bt_lo = lo;
-            ops.bt_f32 = f32;
-            ops.kpad = kpad;
-            ops.rows = brows;
-        }
+        ops = pi_prep(h, precision, nocc, nvir, naux, nf, eps_dev, B_ov, freqs_dev, &s,
+                      !prep_queued);
         fill_i32(h, fail, 0, nf);
     }
     double* x = ws<double>(h, "xinv", (size_t)nf * npad * npad);
@@ -933,6 +952,26 @@ void validate_desc(qx_handle h, const qx_g0w0_desc* d) {
 }
 
 // Builds the host+device CD problem (pair CSR, grid) shared by g0w0 and sigma_cd.
+// The CD frequency grid (nodes, weights) on the device in the buffers "cd_fr" / "cd_wt",
+// uploaded only when the key or the buffer changes.
+void cd_grid(qx_handle h, int nf, double x0, double** fr, double** wt) {
+    double* fr_dev = ws<double>(h, "cd_fr", nf);
+    double* wt_dev = ws<double>(h, "cd_wt", nf);
+    if (h->arena || h->grid_fr != fr_dev || h->grid_nf != nf || h->grid_x0 != x0) {
+        std::vector<double> f, w;
+        scaled_roots(nf, x0, f, w);
+        h2d(h, fr_dev, f.data(), nf);
+        h2d(h, wt_dev, w.data(), nf);
+        // a buffer that grows is reallocated and both names are taken together, so an
+        // unchanged fr pointer and key mean both still hold this grid
+        h->grid_fr = h->arena ? nullptr : fr_dev;
+        h->grid_nf = nf;
+        h->grid_x0 = x0;
+    }
+    *fr = fr_dev;
+    *wt = wt_dev;
+}
+
 // known (the ov path, whose pair list the library generated from targets it validated):
 // the host targets and pair offsets are given and pair_off_dev already holds the offsets, so
 // nothing is downloaded here; the host copies of the energies and pair partners that only
@@ -941,6 +980,7 @@ struct KnownPairs {
     const std::vector<int>* targets;
     const std::vector<int>* off;
     int* off_dev;
+    bool prep_queued;                // the Pi operand preparation is already on the stream
 };
 
 void cd_host_data(qx_handle h, const qx_g0w0_desc* d, const double* mo_energy,
@@ -1045,19 +1085,8 @@ qx_status run_g0w0(qx_handle h, const qx_g0w0_desc* d, const double* mo_energy,
     REQUIRE(h, d->npair == 0 || (B_pair && pair_target && pair_m), QX_ERR_INVALID_INPUT,
             "pair arrays are NULL");
     const int nf = d->nfreq, nt = d->ntarget, npair = d->npair;
-    double* fr_dev = ws<double>(h, "cd_fr", nf);
-    double* wt_dev = ws<double>(h, "cd_wt", nf);
-    if (h->arena || h->grid_fr != fr_dev || h->grid_nf != nf || h->grid_x0 != d->x0) {
-        std::vector<double> fr, wt;
-        scaled_roots(nf, d->x0, fr, wt);
-        h2d(h, fr_dev, fr.data(), nf);
-        h2d(h, wt_dev, wt.data(), nf);
-        // a buffer that grows is reallocated and both names are taken together, so an
-        // unchanged fr pointer and key mean both still hold this grid
-        h->grid_fr = h->arena ? nullptr : fr_dev;
-        h->grid_nf = nf;
-        h->grid_x0 = d->x0;
-    }
+    double *fr_dev, *wt_dev;
+    cd_grid(h, nf, d->x0, &fr_dev, &wt_dev);
     if (!static_terms) {
         double* zero_static = ws<double>(h, "zero_static", nt);
         if (h->arena || h->zero_ptr != zero_static || h->zero_n < nt) {
@@ -1073,7 +1102,7 @@ qx_status run_g0w0(qx_handle h, const qx_g0w0_desc* d, const double* mo_energy,
              static_terms, fr_dev, wt_dev, wnm, cd, known);
     // the Cholesky failure flags come back with the Newton states (one synchronisation)
     WnmResult wr = build_wnm(h, d->precision, d->nocc, d->nvir, d->naux, mo_energy, B_ov, nf,
-                             fr_dev, npair, B_pair, wnm, true);
+                             fr_dev, npair, B_pair, wnm, true, known && known->prep_queued);
     const NewtonParams np = to_params(d->newton, d->qp_solver);
     int* st_dev = ws<int>(h, "newton_state", nt);
     int* it_dev = ws<int>(h, "newton_iters", nt);
@@ -1844,10 +1873,37 @@ qx_status run_g0w0_ov(qx_handle h, const qx_g0w0_desc* d, const double* mo_energ
                       double* z_factor, double* sigma_re, double* sigma_im, int* state,
                       qx_stats* stats) {
     REQUIRE(h, d != nullptr && targets != nullptr, QX_ERR_INVALID_INPUT, "NULL argument");
+    validate_desc(h, d);
     const int nt = d->ntarget;
     REQUIRE(h, nt > 0, QX_ERR_INVALID_INPUT, "ntarget must be positive");
+    REQUIRE(h, mo_energy && B_ov, QX_ERR_INVALID_INPUT, "required pointer is NULL");
     std::vector<int> tg(nt), off(nt + 1, 0);
-    d2h(h, tg.data(), targets, nt);
+    // the targets' download is queued first and the Pi operand preparation (which does not
+    // depend on them) right behind it; the host waits for the copy alone, so the GPU works
+    // while the host builds the pair offsets (with no pinned ring: a plain download)
+    double *fr_dev, *wt_dev;
+    cd_grid(h, d->nfreq, d->x0, &fr_dev, &wt_dev);
+    char* pin = h->arena ? nullptr : pin_take(h, (size_t)nt * sizeof(int));
+    hipEvent_t ev = nullptr;
+    if (pin) {
+        HIPCK(h, hipMemcpyAsync(pin, targets, (size_t)nt * sizeof(int), hipMemcpyDeviceToHost,
+                                h->stream));
+        ev = take_event(h);
+        HIPCK(h, hipEventRecord(ev, h->stream));
+    }
+    double* s_unused = nullptr;
+    {
+        Phase ph(h, QX_PHASE_PREP);
+        pi_prep(h, d->precision, d->nocc, d->nvir, d->naux, d->nfreq, mo_energy, B_ov, fr_dev,
+                &s_unused, true);
+    }
+    if (ev) {
+        HIPCK(h, hipEventSynchronize(ev));
+        std::memcpy(tg.data(), pin, (size_t)nt * sizeof(int));
+        h->ev_free.push_back(ev);
+    } else {
+        d2h(h, tg.data(), targets, nt);
+    }
     for (int t = 0; t < nt; ++t) {
         REQUIRE(h, tg[t] >= 0 && tg[t] < d->nocc + d->nvir, QX_ERR_INVALID_INPUT,
                 "target %d out of range", tg[t]);
@@ -1866,7 +1922,7 @@ qx_status run_g0w0_ov(qx_handle h, const qx_g0w0_desc* d, const double* mo_energ
     HIPCK(h, hipGetLastError());
     qx_g0w0_desc dd = *d;
     dd.npair = npair;
-    const KnownPairs known{&tg, &off, off_dev};
+    const KnownPairs known{&tg, &off, off_dev, true};
     return run_g0w0(h, &dd, mo_energy, mo_energy_ref, B_ov, rows, pt, pm, targets, static_terms,
                     guess, qp_energy, z_factor, sigma_re, sigma_im, state, stats, &known);
 }
