@@ -367,10 +367,11 @@ __global__ void eps_from_pi_kernel(const double* __restrict__ pi, int naux, int 
 
 // ov-only pair rows: occupied target n pairs with every virtual a (row n*nvir + a of B_ov),
 // virtual target n = nocc + a pairs with every occupied i (row i*nvir + a).
+// rows: [npair][ld], columns naux .. ld - 1 zero-filled (ld = naux: unpadded).
 __global__ void ov_pairs_kernel(const double* __restrict__ B, int nocc, int nvir, int naux,
                                 const int* __restrict__ targets, const int* __restrict__ off,
                                 int nt, int* __restrict__ pt, int* __restrict__ pm,
-                                double* __restrict__ rows) {
+                                double* __restrict__ rows, int ld) {
     const int t = blockIdx.y;
     if (t >= nt) return;
     const int n = targets[t];
@@ -381,8 +382,8 @@ __global__ void ov_pairs_kernel(const double* __restrict__ B, int nocc, int nvir
         if (n < nocc) { r = n * nvir + k; m = nocc + k; }
         else { r = k * nvir + (n - nocc); m = k; }
         if (threadIdx.x == 0) { pt[j] = t; pm[j] = m; }
-        for (int c = threadIdx.x; c < naux; c += blockDim.x)
-            rows[(size_t)j * naux + c] = B[(size_t)r * naux + c];
+        for (int c = threadIdx.x; c < ld; c += blockDim.x)
+            rows[(size_t)j * ld + c] = c < naux ? B[(size_t)r * naux + c] : 0.0;
     }
 }
 
@@ -612,7 +613,7 @@ PiOperands pi_prep(qx_handle h, int precision, int nocc, int nvir, int naux, int
 WnmResult build_wnm(qx_handle h, int precision, int nocc, int nvir, int naux,
                     const double* eps_dev, const double* B_ov, int nf, const double* freqs_dev,
                     int npair, const double* B_pair, double* wnm_out, bool defer_check = false,
-                    bool prep_queued = false) {
+                    bool prep_queued = false, bool pairs_padded = false) {
     WnmResult r;
     const int nov = nocc * nvir;
     const int npad = round_up(naux, NB), nblk = npad / NB, kpad = round_up(nov, KPAD);
@@ -629,8 +630,11 @@ WnmResult build_wnm(qx_handle h, int precision, int nocc, int nvir, int naux,
         fill_i32(h, fail, 0, nf);
     }
     double* x = ws<double>(h, "xinv", (size_t)nf * npad * npad);
+    // padded pair rows already are the "bpairs" buffer
     double* bp = ws<double>(h, "bpairs", (size_t)std::max(npair, 1) * npad);
-    if (npair > 0) {
+    REQUIRE(h, !pairs_padded || bp == B_pair, QX_ERR_INVALID_INPUT,
+            "padded pair rows must be the pair-row workspace");
+    if (npair > 0 && !pairs_padded) {
         hipLaunchKernelGGL(pad_rows_kernel, grid1d((size_t)npair * npad), dim3(256), 0, st,
                            B_pair, npair, naux, bp, npad);
         HIPCK(h, hipGetLastError());
@@ -715,7 +719,8 @@ struct HostCD {
     int nocc, nvir, naux, nmo, nt, npair, nf;
     const double* B_ov;            // device
     const double* eps_g_dev;       // device
-    const double* B_pair;          // device [npair][naux]
+    const double* B_pair;          // device [npair][b_ld]
+    int b_ld = 0;                  // row stride of B_pair: naux, or npad for padded rows
     std::vector<double> eps_g, eps_ref;
     std::vector<int> targets, pair_m, pair_off;
     double ef, eta;
@@ -802,7 +807,7 @@ struct HostCD {
             double* wout = ws<double>(h, "res_out", (size_t)chunk * 2);
             h2d(h, om_dev, om.data() + j0, cnt);
             for (int k = 0; k < cnt; ++k)
-                HIPCK(h, hipMemcpyAsync(rows + (size_t)k * naux, B_pair + (size_t)row[j0 + k] * naux,
+                HIPCK(h, hipMemcpyAsync(rows + (size_t)k * naux, B_pair + (size_t)row[j0 + k] * b_ld,
                                         naux * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
             Phase ph(h, QX_PHASE_RESIDUE);
             fill_i32(h, fail, 0, cnt);
@@ -981,6 +986,7 @@ struct KnownPairs {
     const std::vector<int>* off;
     int* off_dev;
     bool prep_queued;                // the Pi operand preparation is already on the stream
+    bool padded;                     // B_pair is [npair][npad], zero-padded ("bpairs")
 };
 
 void cd_host_data(qx_handle h, const qx_g0w0_desc* d, const double* mo_energy,
@@ -1012,6 +1018,7 @@ void setup_cd(qx_handle h, const qx_g0w0_desc* d, const double* mo_energy,
     cd.B_ov = B_ov;
     cd.eps_g_dev = mo_energy;
     cd.B_pair = B_pair;
+    cd.b_ld = known && known->padded ? round_up(d->naux, NB) : d->naux;
     cd.eta = d->eta;
     cd.eps_g.resize(nmo);
     cd.eps_ref.resize(nmo);
@@ -1102,7 +1109,8 @@ qx_status run_g0w0(qx_handle h, const qx_g0w0_desc* d, const double* mo_energy,
              static_terms, fr_dev, wt_dev, wnm, cd, known);
     // the Cholesky failure flags come back with the Newton states (one synchronisation)
     WnmResult wr = build_wnm(h, d->precision, d->nocc, d->nvir, d->naux, mo_energy, B_ov, nf,
-                             fr_dev, npair, B_pair, wnm, true, known && known->prep_queued);
+                             fr_dev, npair, B_pair, wnm, true, known && known->prep_queued,
+                             known && known->padded);
     const NewtonParams np = to_params(d->newton, d->qp_solver);
     int* st_dev = ws<int>(h, "newton_state", nt);
     int* it_dev = ws<int>(h, "newton_iters", nt);
@@ -1914,15 +1922,18 @@ qx_status run_g0w0_ov(qx_handle h, const qx_g0w0_desc* d, const double* mo_energ
     h2d(h, off_dev, off.data(), nt + 1);
     int* pt = ws<int>(h, "ov_pt", npair);
     int* pm = ws<int>(h, "ov_pm", npair);
-    double* rows = ws<double>(h, "ov_rows", (size_t)npair * d->naux);
+    // the pair rows go straight into the zero-padded [npair][npad] buffer the Wc_nm kernels
+    // read (build_wnm's "bpairs")
+    const int npad = round_up(d->naux, NB);
+    double* rows = ws<double>(h, "bpairs", (size_t)std::max(npair, 1) * npad);
     const int maxcnt = std::max(d->nocc, d->nvir);
     hipLaunchKernelGGL(ov_pairs_kernel, dim3(std::min(maxcnt, 1024), nt), dim3(256), 0,
                        h->stream, B_ov, d->nocc, d->nvir, d->naux, targets, off_dev, nt, pt, pm,
-                       rows);
+                       rows, npad);
     HIPCK(h, hipGetLastError());
     qx_g0w0_desc dd = *d;
     dd.npair = npair;
-    const KnownPairs known{&tg, &off, off_dev, true};
+    const KnownPairs known{&tg, &off, off_dev, true, true};
     return run_g0w0(h, &dd, mo_energy, mo_energy_ref, B_ov, rows, pt, pm, targets, static_terms,
                     guess, qp_energy, z_factor, sigma_re, sigma_im, state, stats, &known);
 }
@@ -2007,7 +2018,7 @@ qx_status qx_g0w0_adjoint_ov_dev(qx_handle h, const qx_g0w0_desc* d, const doubl
         double* rows = ws<double>(h, "adj_rows", (size_t)npair * naux);
         hipLaunchKernelGGL(ov_pairs_kernel, dim3(std::min(std::max(nocc, nvir), 1024), nt),
                            dim3(256), 0, st, B_ov, nocc, nvir, naux, targets, off_dev, nt, pt, pm,
-                           rows);
+                           rows, naux);
         HIPCK(h, hipGetLastError());
         // forward checkpoints in FP64: X_k = L_k^-1 for every frequency, Wc_nm
         std::vector<double> fr, wt;
@@ -2511,7 +2522,7 @@ void enqueue_batch(qx_handle h, BatchPlan& P, bool direct) {
                                    dim3(std::min(std::max(in.nocc, in.nvir), 1024), in.nt),
                                    dim3(256), 0, st, in.B_ov, in.nocc, in.nvir, in.naux,
                                    in.targets_dev, in.pair_off_dev, in.nt, in.pt, in.pm,
-                                   in.rows_naux);
+                                   in.rows_naux, in.naux);
                 hipLaunchKernelGGL(pad_rows_kernel, grid1d((size_t)in.npair * in.npad), dim3(256),
                                    0, st, in.rows_naux, in.npair, in.naux, in.bp, in.npad);
                 HIPCK(h, hipGetLastError());
