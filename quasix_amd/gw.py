@@ -500,15 +500,18 @@ class G0W0Solver:
         self.out = torch.empty((4, nt), dtype=torch.float64, device=self.dev)
         self.state = torch.empty(nt, dtype=torch.int32, device=self.dev)
         self.stats = _lib.Stats()
+        self._args = None
         torch.cuda.synchronize(self.dev)
 
     def run(self):
         _bind_current_stream(self)
-        o = self.out
-        self.h.check(self.h.lib.qx_g0w0_cd_ov_dev(
-            self.h.h, C.byref(self.desc), _ptr(self.e), None, _ptr(self.B), _ptr(self.t_dev),
-            None, None, _ptr(o[0]), _ptr(o[1]), _ptr(o[2]), _ptr(o[3]), _ptr(self.state),
-            C.byref(self.stats)))
+        if self._args is None:
+            # the buffers live as long as the solver, so the call's arguments are built once
+            o = self.out
+            self._args = (self.h.h, C.byref(self.desc), _ptr(self.e), None, _ptr(self.B),
+                          _ptr(self.t_dev), None, None, _ptr(o[0]), _ptr(o[1]), _ptr(o[2]),
+                          _ptr(o[3]), _ptr(self.state), C.byref(self.stats))
+        self.h.check(self.h.lib.qx_g0w0_cd_ov_dev(*self._args))
 
     def set_profiling(self, on: bool):
         self.h.check(self.h.lib.qx_set_profiling(self.h.h, 1 if on else 0))
