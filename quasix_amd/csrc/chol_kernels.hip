@@ -712,7 +712,7 @@ __global__ void gj_finish_kernel(const double* __restrict__ work, double* __rest
 
 hipError_t launch_potrf_batched(double* a, double* x, double* linv_diag, int npad, int nf,
                                 int* fail, hipStream_t st, const StreamFan* fan, int groups,
-                                const PotrfAfter* after) {
+                                const PotrfAfter* after, const PotrfAfter* lead) {
     const int nblk = npad / NB;
     int ng = fan ? std::min(groups, fan->n + 1) : 1;
     ng = std::max(1, std::min(ng, nf));
@@ -724,6 +724,10 @@ hipError_t launch_potrf_batched(double* a, double* x, double* linv_diag, int npa
             sg[g] = fan->side[g - 1];
             e = hipStreamWaitEvent(sg[g], fan->fork, 0);
         }
+        if (e != hipSuccess) return e;
+    }
+    if (lead) {
+        const hipError_t e = (*lead)(st, 0, nf / ng);
         if (e != hipSuccess) return e;
     }
     const size_t mat = (size_t)npad * npad, dblk = (size_t)nblk * NB * NB;

@@ -613,7 +613,8 @@ PiOperands pi_prep(qx_handle h, int precision, int nocc, int nvir, int naux, int
 WnmResult build_wnm(qx_handle h, int precision, int nocc, int nvir, int naux,
                     const double* eps_dev, const double* B_ov, int nf, const double* freqs_dev,
                     int npair, const double* B_pair, double* wnm_out, bool defer_check = false,
-                    bool prep_queued = false, bool pairs_padded = false) {
+                    bool prep_queued = false, bool pairs_padded = false,
+                    const std::function<void()>* pair_rows = nullptr) {
     WnmResult r;
     const int nov = nocc * nvir;
     const int npad = round_up(naux, NB), nblk = npad / NB, kpad = round_up(nov, KPAD);
@@ -647,7 +648,6 @@ WnmResult build_wnm(qx_handle h, int precision, int nocc, int nvir, int naux,
         mx.bl = ws<__bf16>(h, "wnm_bl", npp * npad);
         mx.mh = ws<__bf16>(h, "wnm_mh", (size_t)nf * npad * npad);
         mx.ml = ws<__bf16>(h, "wnm_ml", (size_t)nf * npad * npad);
-        HIPCK(h, launch_wnm_split_rows(bp, npair, npad, &mx, st));
     }
     const int pv = choose_variant(h, "pi", precision, pi_variant_count(precision),
                                   {naux, nov, nf}, [&](int v) {
@@ -670,12 +670,20 @@ WnmResult build_wnm(qx_handle h, int precision, int nocc, int nvir, int naux,
     const PotrfAfter split = [&](hipStream_t gs, int f0, int nfg) {
         return launch_split_m(x + f0 * mat, npad, nfg, mx.mh + f0 * mat, mx.ml + f0 * mat, gs);
     };
+    // what only Wc_nm reads (the pair rows the caller queued, their BF16 split) runs after Pi,
+    // on the main stream beside the other frequency groups' first steps, not in front of Pi
+    const PotrfAfter lead = [&](hipStream_t ls, int, int) {
+        if (pair_rows) (*pair_rows)();
+        if (precision == QX_PREC_BF16X3 && npair > 0)
+            return launch_wnm_split_rows(bp, npair, npad, &mx, ls);
+        return hipSuccess;
+    };
     {
         Phase ph(h, QX_PHASE_POTRF);
         int groups = 1;
         const StreamFan* fan = potrf_fan(h, groups);
         HIPCK(h, launch_potrf_batched(eps_mat, x, linv, npad, nf, fail, st, fan, groups,
-                                      split_early ? &split : nullptr));
+                                      split_early ? &split : nullptr, &lead));
     }
     if (npair > 0) {
         // Wc_nm before the failure check: on the rare Cholesky failure it is recomputed below
@@ -978,15 +986,19 @@ void cd_grid(qx_handle h, int nf, double x0, double** fr, double** wt) {
 }
 
 // known (the ov path, whose pair list the library generated from targets it validated):
-// the host targets and pair offsets are given and pair_off_dev already holds the offsets, so
-// nothing is downloaded here; the host copies of the energies and pair partners that only
-// the residue path reads are fetched by cd_host_data when a target needs it.
+// the host targets and pair offsets are given and off_dev holds the offsets once the pair
+// rows are queued (pair_rows), so nothing is downloaded here; the host copies of the
+// energies and pair partners that only the residue path reads are fetched by cd_host_data
+// when a target needs it.
 struct KnownPairs {
     const std::vector<int>* targets;
     const std::vector<int>* off;
     int* off_dev;
     bool prep_queued;                // the Pi operand preparation is already on the stream
     bool padded;                     // B_pair is [npair][npad], zero-padded ("bpairs")
+    // queues the writes of the pair rows and partners (B_pair, pair_target, pair_m, off_dev)
+    // on the library's stream; build_wnm runs it after the Pi launch (nullptr: already queued)
+    const std::function<void()>* pair_rows = nullptr;
 };
 
 void cd_host_data(qx_handle h, const qx_g0w0_desc* d, const double* mo_energy,
@@ -1110,7 +1122,7 @@ qx_status run_g0w0(qx_handle h, const qx_g0w0_desc* d, const double* mo_energy,
     // the Cholesky failure flags come back with the Newton states (one synchronisation)
     WnmResult wr = build_wnm(h, d->precision, d->nocc, d->nvir, d->naux, mo_energy, B_ov, nf,
                              fr_dev, npair, B_pair, wnm, true, known && known->prep_queued,
-                             known && known->padded);
+                             known && known->padded, known ? known->pair_rows : nullptr);
     const NewtonParams np = to_params(d->newton, d->qp_solver);
     int* st_dev = ws<int>(h, "newton_state", nt);
     int* it_dev = ws<int>(h, "newton_iters", nt);
@@ -1919,21 +1931,27 @@ qx_status run_g0w0_ov(qx_handle h, const qx_g0w0_desc* d, const double* mo_energ
     }
     const int npair = off[nt];
     int* off_dev = ws<int>(h, "ov_off", nt + 1);
-    h2d(h, off_dev, off.data(), nt + 1);
     int* pt = ws<int>(h, "ov_pt", npair);
     int* pm = ws<int>(h, "ov_pm", npair);
     // the pair rows go straight into the zero-padded [npair][npad] buffer the Wc_nm kernels
-    // read (build_wnm's "bpairs")
+    // read (build_wnm's "bpairs"); only Wc_nm, the residues and Newton read them, so they are
+    // queued behind the Pi launch
     const int npad = round_up(d->naux, NB);
     double* rows = ws<double>(h, "bpairs", (size_t)std::max(npair, 1) * npad);
     const int maxcnt = std::max(d->nocc, d->nvir);
-    hipLaunchKernelGGL(ov_pairs_kernel, dim3(std::min(maxcnt, 1024), nt), dim3(256), 0,
-                       h->stream, B_ov, d->nocc, d->nvir, d->naux, targets, off_dev, nt, pt, pm,
-                       rows, npad);
-    HIPCK(h, hipGetLastError());
+    const std::function<void()> pair_rows = [&] {
+        h2d(h, off_dev, off.data(), nt + 1);
+        hipLaunchKernelGGL(ov_pairs_kernel, dim3(std::min(maxcnt, 1024), nt), dim3(256), 0,
+                           h->stream, B_ov, d->nocc, d->nvir, d->naux, targets, off_dev, nt, pt,
+                           pm, rows, npad);
+        HIPCK(h, hipGetLastError());
+    };
+    // without the pinned ring (caller's arena) the offsets' upload synchronises, which behind
+    // Pi would hold the factorisation's launches back: queue the pair rows here then
+    if (h->arena) pair_rows();
     qx_g0w0_desc dd = *d;
     dd.npair = npair;
-    const KnownPairs known{&tg, &off, off_dev, true, true};
+    const KnownPairs known{&tg, &off, off_dev, true, true, h->arena ? nullptr : &pair_rows};
     return run_g0w0(h, &dd, mo_energy, mo_energy_ref, B_ov, rows, pt, pm, targets, static_terms,
                     guess, qp_energy, z_factor, sigma_re, sigma_im, state, stats, &known);
 }

@@ -125,10 +125,13 @@ struct StreamFan {
 // other groups' column updates.
 // after (optional): called once per group, on that group's stream after its last kernel and
 // before the join, with the group's first frequency and frequency count.
+// lead (optional): called once on st after the fork and before the first group's kernels
+// (same arguments for that group): work queued there runs beside the other groups' steps.
 typedef std::function<hipError_t(hipStream_t, int, int)> PotrfAfter;
 hipError_t launch_potrf_batched(double* a, double* x, double* linv_diag, int npad, int nf,
                                 int* fail, hipStream_t st, const StreamFan* fan = nullptr,
-                                int groups = 1, const PotrfAfter* after = nullptr);
+                                int groups = 1, const PotrfAfter* after = nullptr,
+                                const PotrfAfter* lead = nullptr);
 // Grouped factorisation of a batch of instances with different sizes and a common nf.
 // inst_dev: [ninst] sorted by npad descending (nblk_sorted[q] = npad_q / NB on the host);
 // cu_off / rp_off: [ninst + 1] prefix sums of nblk_q * nf and (nblk_q - 1) * nf, on the
